@@ -1,0 +1,66 @@
+"""ctypes binding of libfe_hip.so, the gfx950 library behind every entry point.
+
+The library must be built (build.py) before import; a missing or unloadable
+library raises ImportError, so nothing in this package can silently run a
+CPU substitute.
+"""
+import ctypes as ct
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libfe_hip.so"
+
+P = ct.c_void_p
+I32, I64, F64 = ct.c_int32, ct.c_int64, ct.c_double
+
+# name: (restype, argtypes); every symbol is declared in include/fe_hip.h.
+SIGNATURES = {
+    # EDT (reference-shaped)
+    "edt_squared": (ct.c_int, [P, ct.c_int, ct.c_int, ct.c_int, P]),
+    "edt_squared_large": (ct.c_int, [P, ct.c_int, ct.c_int, ct.c_int, P]),
+    "edt_pair_create": (P, [ct.c_int, ct.c_int, ct.c_int]),
+    "edt_pair_query": (ct.c_int, [P, P, F64, P, P]),
+    "edt_pair_destroy": (None, [P]),
+    # EDT (uncapped / device-resident)
+    "fe_edt_squared_host": (ct.c_int, [P, ct.c_int, ct.c_int, ct.c_int, P]),
+    "fe_edt_pair_create": (P, [ct.c_int, ct.c_int, ct.c_int]),
+    "fe_edt_device": (ct.c_int, [P, ct.c_int, ct.c_int, ct.c_int, ct.c_int, P, P, P]),
+    "fe_edt_signed_device": (ct.c_int, [P, I64, F64, P, P]),
+}
+
+_lib = None
+
+
+class NativeLibraryMissing(ImportError):
+    pass
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise NativeLibraryMissing(f"{LIB_PATH} is not built; run build.py (hipcc, gfx950)")
+        try:
+            L = ct.CDLL(str(LIB_PATH))
+        except OSError as e:
+            raise NativeLibraryMissing(f"cannot load {LIB_PATH}: {e}") from e
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def exported_symbols():
+    return list(SIGNATURES)
+
+
+def stream_handle(stream=None):
+    """Raw hipStream_t of a torch stream (None: torch's current stream)."""
+    import torch
+    s = torch.cuda.current_stream() if stream is None else stream
+    return ct.c_void_p(s.cuda_stream)
+
+
+def ptr(t):
+    return ct.c_void_p(t.data_ptr())

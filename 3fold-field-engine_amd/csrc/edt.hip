@@ -1,0 +1,464 @@
+// Exact integer squared Euclidean distance transform for gfx950.
+//
+// Mathematics (the reference's, FE/field_edt_integer_v1.py:14-26): with
+// f = 0 on site voxels and f = sentinel = sum (n-1)^2 + 1 elsewhere, the
+// result is min over s of f(s) + |p - s|^2, computed one axis at a time.  The
+// per-axis minimum is exact in integers, so the axis order does not change
+// the result; this implementation runs z first (the contiguous axis, straight
+// from the uint8 mask) and then y and x (strided axes, LDS-staged tiles).
+//
+//   z pass  one wave per row: 64-bit ballots of the site flags, nearest site
+//           left/right from clz/ctz, squared 1-D distance.  Both polarities of
+//           a pair come from one read of the mask.
+//   y,x     W adjacent lines (contiguous along z, so every row of the tile is
+//           one coalesced segment) are staged in LDS; T threads share a line,
+//           each owns a chunk of positions, builds the exact lower envelope
+//           of its chunk's parabolas (Felzenszwalb, pop test by int64 cross
+//           multiplication, no division) and then evaluates the envelopes of
+//           its own and neighbouring chunks at its positions, visiting a
+//           neighbour only while its lower bound min f + gap^2 can still win.
+//   signed  FE/edt_persistent_pair_v1/edt.cu:36-42 semantics with IEEE
+//           correctly rounded f64 sqrt/mul and f32 sub/mul.
+#include "fe_common.hpp"
+#include <climits>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+namespace {
+
+// ---------------------------------------------------------------- z pass
+// rows: number of (x,y) rows; sites of plane 0 are voxels whose mask equals
+// site_one (1) or is zero (0); plane 1 (PAIR) takes the complement.
+template <bool PAIR>
+__global__ __launch_bounds__(256) void k_edt_zpass(const uint8_t* __restrict__ mask, int32_t* __restrict__ out,
+                                                   int64_t rows, int nz, int64_t plane, int32_t sentinel, int site_one)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (row >= rows) return;  // uniform per wave
+    const uint8_t* m = mask + row * nz;
+    const int nseg = (nz + 63) >> 6;
+    uint64_t w0 = 0, valid = 0;
+    for (int s = 0; s < nseg; ++s) {
+        int p = (s << 6) + lane;
+        bool in = p < nz;
+        bool one = in && m[p] != 0;
+        uint64_t b = __ballot(site_one ? one : (in && !one));
+        uint64_t v = __ballot(in);
+        if (lane == s) { w0 = b; valid = v; }
+    }
+    const uint64_t w1 = ~w0 & valid;
+    const uint64_t seg0 = __ballot(w0 != 0), seg1 = __ballot(w1 != 0);
+    int32_t* o0 = out + row * nz;
+    int32_t* o1 = o0 + plane;
+    const uint64_t low = lane == 63 ? ~0ull : ((2ull << lane) - 1);
+    for (int s = 0; s < nseg; ++s) {
+        const int p = (s << 6) + lane;
+#pragma unroll
+        for (int pl = 0; pl < (PAIR ? 2 : 1); ++pl) {
+            const uint64_t mw = pl ? w1 : w0, segs = pl ? seg1 : seg0;
+            const uint64_t w = readlane64(mw, s);
+            // nearest site in earlier / later segments (wave-uniform)
+            int carry_l = INT_MIN / 2, carry_r = INT_MAX / 2;
+            uint64_t prev = segs & ((1ull << s) - 1);
+            if (prev) {
+                int j = 63 - __builtin_clzll(prev);
+                carry_l = (j << 6) + 63 - __builtin_clzll(readlane64(mw, j));
+            }
+            uint64_t next = s == 63 ? 0 : (segs >> (s + 1));
+            if (next) {
+                int j = s + 1 + __builtin_ctzll(next);
+                carry_r = (j << 6) + __builtin_ctzll(readlane64(mw, j));
+            }
+            uint64_t wl = w & low, wr = w >> lane;
+            int left = wl ? (s << 6) + 63 - __builtin_clzll(wl) : carry_l;
+            int right = wr ? p + __builtin_ctzll(wr) : carry_r;
+            int d = min(p - left, right - p);
+            int32_t v = (segs == 0) ? sentinel : d * d;
+            if (p < nz) (pl ? o1 : o0)[p] = v;
+        }
+    }
+}
+
+// ---------------------------------------------------------------- line passes
+// Parabola value of site s at position p over the LDS line f.
+__device__ __forceinline__ int32_t pval(const int32_t* f, int s, int p) { return f[s] + (p - s) * (p - s); }
+
+template <int W, int T, int CMAX>
+__global__ __launch_bounds__(W* T) void k_edt_lines(const int32_t* __restrict__ src, int32_t* __restrict__ dst, int L,
+                                                    int64_t axis_stride, int nz, int ztiles, int64_t outer_stride,
+                                                    int outer_count, int64_t plane_stride, uint32_t tiles,
+                                                    uint32_t grid)
+{
+    extern __shared__ int32_t lds[];
+    const uint32_t tile = xcd_tile(blockIdx.x, grid);
+    if (tile >= tiles) return;  // uniform per block
+    const int zt = tile % ztiles;
+    const int rest = tile / ztiles;
+    const int outer = rest % outer_count, plane = rest / outer_count;
+    const int64_t base = (int64_t)plane * plane_stride + (int64_t)outer * outer_stride + (int64_t)zt * W;
+    const int nlines = min(W, nz - zt * W);
+    const int LP = L + 1;
+    const int NT = W * T;
+    int32_t* f = lds;                                        // [W][LP]
+    int32_t* minf = f + W * LP;                              // [NT]
+    int32_t* ksz = minf + NT;                                // [NT]
+    uint16_t* stk = reinterpret_cast<uint16_t*>(ksz + NT);  // [CMAX][NT]
+    const int tid = threadIdx.x;
+
+    for (int i = tid; i < W * L; i += NT) {
+        int line = i % W, q = i / W;
+        if (line < nlines) f[line * LP + q] = src[base + (int64_t)q * axis_stride + line];
+    }
+    __syncthreads();
+
+    const int line = tid % W, c = tid / W;
+    const int C = (L + T - 1) / T;
+    const int p0 = min(c * C, L), p1 = min(p0 + C, L);
+    const bool active = line < nlines;
+    const int32_t* fl = f + line * LP;
+
+    // Step 1: exact lower envelope of this chunk's parabolas.
+    int k = -1;
+    int32_t mf = INT_MAX;
+    if (active) {
+        for (int q = p0; q < p1; ++q) {
+            const int32_t fq = fl[q];
+            mf = min(mf, fq);
+            const int64_t Fq = (int64_t)fq + (int64_t)q * q;
+            while (k >= 1) {
+                const int a = stk[(k - 1) * NT + tid], b = stk[k * NT + tid];
+                const int64_t Fa = (int64_t)fl[a] + (int64_t)a * a, Fb = (int64_t)fl[b] + (int64_t)b * b;
+                // parabola b is nowhere strictly lowest when x(q,b) <= x(b,a)
+                if ((Fq - Fb) * (int64_t)(b - a) <= (Fb - Fa) * (int64_t)(q - b)) --k;
+                else break;
+            }
+            stk[++k * NT + tid] = (uint16_t)q;
+        }
+    }
+    minf[tid] = mf;
+    ksz[tid] = k + 1;
+    __syncthreads();
+    if (!active || p0 >= p1) return;
+
+    // Step 2: evaluate own and neighbouring chunk envelopes.
+    int32_t best[CMAX];
+    {
+        int kk = 0;
+        const int n = k + 1;
+#pragma unroll
+        for (int i = 0; i < CMAX; ++i) {
+            const int p = p0 + i;
+            if (p < p1) {
+                while (kk + 1 < n && pval(fl, stk[(kk + 1) * NT + tid], p) <= pval(fl, stk[kk * NT + tid], p)) ++kk;
+                best[i] = pval(fl, stk[kk * NT + tid], p);
+            }
+        }
+    }
+    int32_t worst = 0;
+#pragma unroll
+    for (int i = 0; i < CMAX; ++i)
+        if (p0 + i < p1) worst = max(worst, best[i]);
+
+    bool go_l = true, go_r = true;
+    for (int d = 1; (go_l || go_r); ++d) {
+#pragma unroll
+        for (int side = 0; side < 2; ++side) {
+            const int cc = side ? c + d : c - d;
+            bool& go = side ? go_r : go_l;
+            if (!go) continue;
+            if (cc < 0 || cc >= T) { go = false; continue; }
+            const int q0 = min(cc * C, L), q1 = min(q0 + C, L);
+            if (q0 >= q1) { go = false; continue; }
+            const int gap = side ? q0 - (p1 - 1) : p0 - (q1 - 1);
+            const int64_t g2 = (int64_t)gap * gap;
+            if (g2 >= worst) { go = false; continue; }
+            const int ct = cc * W + line;
+            if ((int64_t)minf[ct] + g2 >= worst) continue;
+            const int n = ksz[ct];
+            int kk = 0;
+#pragma unroll
+            for (int i = 0; i < CMAX; ++i) {
+                const int p = p0 + i;
+                if (p < p1) {
+                    while (kk + 1 < n && pval(fl, stk[(kk + 1) * NT + ct], p) <= pval(fl, stk[kk * NT + ct], p)) ++kk;
+                    best[i] = min(best[i], pval(fl, stk[kk * NT + ct], p));
+                }
+            }
+            worst = 0;
+#pragma unroll
+            for (int i = 0; i < CMAX; ++i)
+                if (p0 + i < p1) worst = max(worst, best[i]);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < CMAX; ++i) {
+        const int p = p0 + i;
+        if (p < p1) dst[base + (int64_t)p * axis_stride + line] = best[i];
+    }
+}
+
+// ---------------------------------------------------------------- signed output
+__global__ __launch_bounds__(256) void k_edt_signed(const int32_t* __restrict__ pair, float* __restrict__ out,
+                                                    int64_t n, double pitch)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float o = (float)(sqrt((double)pair[i]) * pitch);
+    const float in = (float)(sqrt((double)pair[n + i]) * pitch);
+    const float d = o - in;
+    const float sg = d > 0.0f ? 1.0f : (d < 0.0f ? -1.0f : 0.0f);
+    const float half = (float)(0.5 * pitch);
+    out[i] = d - sg * half;
+}
+
+struct LineCfg {
+    int W, T, CMAX;
+};
+
+// Tile shape by line length: LDS holds W*(L+1) int32 values plus CMAX*W*T
+// uint16 stack slots; the choice keeps it near or under 64 KB.
+LineCfg pick_cfg(int L)
+{
+    if (L <= 128) return {32, 8, 16};
+    if (L <= 512) return {16, 16, 32};
+    if (L <= 1024) return {8, 32, 32};
+    return {4, 64, 64};
+}
+
+template <int W, int T, int CMAX>
+hipError_t launch_lines_t(const int32_t* src, int32_t* dst, int L, int64_t axis_stride, int nz, int64_t outer_stride,
+                          int outer_count, int planes, int64_t plane_stride, hipStream_t st)
+{
+    const int ztiles = (nz + W - 1) / W;
+    const uint64_t tiles = (uint64_t)ztiles * outer_count * planes;
+    const uint32_t grid = round8(tiles);
+    const size_t lds = (size_t)W * (L + 1) * 4 + (size_t)W * T * 8 + (size_t)CMAX * W * T * 2;
+    if (lds > 65536) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_edt_lines<W, T, CMAX>),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL((k_edt_lines<W, T, CMAX>), dim3(grid), dim3(W * T), lds, st, src, dst, L, axis_stride, nz,
+                       ztiles, outer_stride, outer_count, plane_stride, (uint32_t)tiles, grid);
+    return hipGetLastError();
+}
+
+hipError_t launch_lines(const int32_t* src, int32_t* dst, int L, int64_t axis_stride, int nz, int64_t outer_stride,
+                        int outer_count, int planes, int64_t plane_stride, hipStream_t st)
+{
+    LineCfg c = pick_cfg(L);
+    if (c.W == 32) return launch_lines_t<32, 8, 16>(src, dst, L, axis_stride, nz, outer_stride, outer_count, planes, plane_stride, st);
+    if (c.W == 16) return launch_lines_t<16, 16, 32>(src, dst, L, axis_stride, nz, outer_stride, outer_count, planes, plane_stride, st);
+    if (c.W == 8) return launch_lines_t<8, 32, 32>(src, dst, L, axis_stride, nz, outer_stride, outer_count, planes, plane_stride, st);
+    return launch_lines_t<4, 64, 64>(src, dst, L, axis_stride, nz, outer_stride, outer_count, planes, plane_stride, st);
+}
+
+}  // namespace
+
+// Largest line length the device passes accept (stack indices are uint16 and
+// the widest tile configuration covers 64 chunks of 64 positions).
+#define FE_EDT_MAX_AXIS 4096
+
+// Device-resident EDT of `planes` (1 or 2) polarities.  mask: nx*ny*nz bytes
+// on the device; work and out: planes*N int32 each; the result lands in out.
+// planes == 1: squared distance of every nonzero voxel to the nearest zero
+// voxel (FE/edt_native_linear_v1/edt.cu semantics).  planes == 2: plane 0 is
+// the squared distance of non-mask voxels to the mask, plane 1 of mask voxels
+// to the complement (FE/edt_persistent_pair_v1/edt.cu:32-35 semantics).
+extern "C" int fe_edt_device(const uint8_t* mask, int nx, int ny, int nz, int planes, int32_t* work, int32_t* out,
+                             void* stream) noexcept
+{
+    if (!mask || !work || !out || nx < 1 || ny < 1 || nz < 1 || nx > FE_EDT_MAX_AXIS || ny > FE_EDT_MAX_AXIS ||
+        nz > FE_EDT_MAX_AXIS || (planes != 1 && planes != 2))
+        return FE_INVALID;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const int64_t n = (int64_t)nx * ny * nz;
+    const int64_t sentinel64 = (int64_t)(nx - 1) * (nx - 1) + (int64_t)(ny - 1) * (ny - 1) + (int64_t)(nz - 1) * (nz - 1) + 1;
+    const int32_t sentinel = (int32_t)sentinel64;
+    const int64_t rows = (int64_t)nx * ny;
+    // z pass into `out`, y pass into `work`, x pass back into `out`.
+    // planes == 1 sites: mask == 0; planes == 2, plane 0 sites: mask != 0.
+    const int site_one = planes == 2 ? 1 : 0;
+    const unsigned zgrid = (unsigned)((rows + 3) / 4);
+    if (planes == 2)
+        hipLaunchKernelGGL(k_edt_zpass<true>, dim3(zgrid), dim3(256), 0, st, mask, out, rows, nz, n, sentinel, site_one);
+    else
+        hipLaunchKernelGGL(k_edt_zpass<false>, dim3(zgrid), dim3(256), 0, st, mask, out, rows, nz, n, sentinel, site_one);
+    FE_TRY(hipGetLastError());
+    // y lines: outer = x (stride ny*nz), axis stride nz
+    FE_TRY(launch_lines(out, work, ny, nz, nz, (int64_t)ny * nz, nx, planes, n, st));
+    // x lines: outer = y (stride nz), axis stride ny*nz
+    FE_TRY(launch_lines(work, out, nx, (int64_t)ny * nz, nz, nz, ny, planes, n, st));
+    return FE_OK;
+}
+
+// Signed distance from a device pair (2N int32) into N float32.
+extern "C" int fe_edt_signed_device(const int32_t* pair, int64_t n, double pitch, float* out, void* stream) noexcept
+{
+    if (!pair || !out || n < 1 || !std::isfinite(pitch)) return FE_INVALID;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    hipLaunchKernelGGL(k_edt_signed, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, pair, out, n, pitch);
+    FE_TRY(hipGetLastError());
+    return FE_OK;
+}
+
+// ---------------------------------------------------------------- reference-shaped host ABI
+namespace {
+
+// Host validation of FE/edt_native_v1/edt.cu:64-69: bytes <= 1, both classes present.
+int validate_mask(const uint8_t* mask, int64_t n)
+{
+    int64_t ones = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        if (mask[i] > 1) return FE_INVALID;
+        ones += mask[i];
+    }
+    return (ones == 0 || ones == n) ? FE_INVALID : FE_OK;
+}
+
+// Plain per-call squared EDT: allocate, upload, three passes, download, free.
+int edt_squared_capped(const uint8_t* mask, int nx, int ny, int nz, int32_t* output, int axis_cap, int64_t size_cap)
+{
+    if (!mask || !output || nx < 1 || ny < 1 || nz < 1 || nx > axis_cap || ny > axis_cap || nz > axis_cap) return FE_INVALID;
+    const int64_t n = (int64_t)nx * ny * nz;
+    if (n > size_cap) return FE_INVALID;
+    if (validate_mask(mask, n)) return FE_INVALID;
+    uint8_t* dm = nullptr;
+    int32_t *a = nullptr, *b = nullptr;
+    int status = FE_RUNTIME;
+    do {
+        if (hipMalloc(&dm, (size_t)n) != hipSuccess) break;
+        if (hipMalloc(&a, (size_t)n * 4) != hipSuccess) break;
+        if (hipMalloc(&b, (size_t)n * 4) != hipSuccess) break;
+        if (hipMemcpy(dm, mask, (size_t)n, hipMemcpyHostToDevice) != hipSuccess) break;
+        if (fe_edt_device(dm, nx, ny, nz, 1, b, a, nullptr) != FE_OK) break;
+        if (hipMemcpy(output, a, (size_t)n * 4, hipMemcpyDeviceToHost) != hipSuccess) break;
+        status = FE_OK;
+    } while (0);
+    if (dm && hipFree(dm) != hipSuccess) status = FE_RUNTIME;
+    if (a && hipFree(a) != hipSuccess) status = FE_RUNTIME;
+    if (b && hipFree(b) != hipSuccess) status = FE_RUNTIME;
+    return status;
+}
+
+struct PairEDT {
+    int nx, ny, nz;
+    int64_t size;
+    uint8_t* mask = nullptr;
+    int32_t* a = nullptr;
+    int32_t* b = nullptr;
+    float* distance = nullptr;
+    hipStream_t stream = nullptr;
+    bool failed = false;
+};
+
+}  // namespace
+
+// FE/edt_native_v1/edt.cu:18-48 contract (<=128 per axis, <=262144 voxels).
+extern "C" int edt_squared(const uint8_t* mask, int nx, int ny, int nz, int32_t* output) noexcept
+{
+    try {
+        return edt_squared_capped(mask, nx, ny, nz, output, 128, 262144);
+    } catch (...) {
+        return FE_RUNTIME;
+    }
+}
+
+// FE/edt_native_large_v1/edt.cu:20 and FE/edt_native_linear_v1/edt.cu:33 contract
+// (<=512 per axis, <=1048576 voxels).
+extern "C" int edt_squared_large(const uint8_t* mask, int nx, int ny, int nz, int32_t* output) noexcept
+{
+    try {
+        return edt_squared_capped(mask, nx, ny, nz, output, 512, 1048576);
+    } catch (...) {
+        return FE_RUNTIME;
+    }
+}
+
+// Plain per-call squared EDT without the reference's size caps (axes <= 4096).
+extern "C" int fe_edt_squared_host(const uint8_t* mask, int nx, int ny, int nz, int32_t* output) noexcept
+{
+    try {
+        return edt_squared_capped(mask, nx, ny, nz, output, FE_EDT_MAX_AXIS, INT64_MAX);
+    } catch (...) {
+        return FE_RUNTIME;
+    }
+}
+
+extern "C" void edt_pair_destroy(void* pointer) noexcept
+{
+    auto* p = static_cast<PairEDT*>(pointer);
+    if (!p) return;
+    if (p->mask) (void)hipFree(p->mask);
+    if (p->a) (void)hipFree(p->a);
+    if (p->b) (void)hipFree(p->b);
+    if (p->distance) (void)hipFree(p->distance);
+    if (p->stream) (void)hipStreamDestroy(p->stream);
+    delete p;
+}
+
+static void* pair_create(int nx, int ny, int nz, int axis_cap, int64_t size_cap) noexcept
+{
+    if (nx < 1 || ny < 1 || nz < 1 || nx > axis_cap || ny > axis_cap || nz > axis_cap ||
+        (int64_t)nx * ny * nz > size_cap)
+        return nullptr;
+    PairEDT* p = nullptr;
+    try {
+        p = new PairEDT();
+        p->nx = nx;
+        p->ny = ny;
+        p->nz = nz;
+        p->size = (int64_t)nx * ny * nz;
+        if (hipMalloc(&p->mask, (size_t)p->size) != hipSuccess || hipMalloc(&p->a, (size_t)p->size * 8) != hipSuccess ||
+            hipMalloc(&p->b, (size_t)p->size * 8) != hipSuccess ||
+            hipMalloc(&p->distance, (size_t)p->size * 4) != hipSuccess ||
+            hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess) {
+            edt_pair_destroy(p);
+            return nullptr;
+        }
+        return p;
+    } catch (...) {
+        edt_pair_destroy(p);
+        return nullptr;
+    }
+}
+
+// FE/edt_persistent_pair_v1/edt.cu:48-58 contract (<=512 per axis, <=1048576 voxels).
+extern "C" void* edt_pair_create(int nx, int ny, int nz) noexcept { return pair_create(nx, ny, nz, 512, 1048576); }
+
+// Same persistent pair without the reference's size caps (axes <= 4096).
+extern "C" void* fe_edt_pair_create(int nx, int ny, int nz) noexcept
+{
+    return pair_create(nx, ny, nz, FE_EDT_MAX_AXIS, INT64_MAX);
+}
+
+// FE/edt_persistent_pair_v1/edt.cu:59-78: validate, upload, paired passes,
+// signed output on the device, download; status 2 latches the handle.
+extern "C" int edt_pair_query(void* pointer, const uint8_t* mask, double pitch, float* output, int32_t* squared) noexcept
+{
+    auto* p = static_cast<PairEDT*>(pointer);
+    if (!p || !mask || !output || !std::isfinite(pitch) || pitch < 1e-12 || pitch > 1e12) return FE_INVALID;
+    if (p->failed) return FE_RUNTIME;
+    if (validate_mask(mask, p->size)) return FE_INVALID;
+#define FE_CHECK(x)                  \
+    if ((x) != hipSuccess) {         \
+        p->failed = true;            \
+        return FE_RUNTIME;           \
+    }
+    FE_CHECK(hipMemcpyAsync(p->mask, mask, (size_t)p->size, hipMemcpyHostToDevice, p->stream));
+    if (fe_edt_device(p->mask, p->nx, p->ny, p->nz, 2, p->b, p->a, p->stream) != FE_OK) {
+        p->failed = true;
+        return FE_RUNTIME;
+    }
+    if (fe_edt_signed_device(p->a, p->size, pitch, p->distance, p->stream) != FE_OK) {
+        p->failed = true;
+        return FE_RUNTIME;
+    }
+    FE_CHECK(hipMemcpyAsync(output, p->distance, (size_t)p->size * 4, hipMemcpyDeviceToHost, p->stream));
+    if (squared) FE_CHECK(hipMemcpyAsync(squared, p->a, (size_t)p->size * 8, hipMemcpyDeviceToHost, p->stream));
+    FE_CHECK(hipStreamSynchronize(p->stream));
+#undef FE_CHECK
+    return FE_OK;
+}

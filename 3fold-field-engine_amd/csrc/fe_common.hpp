@@ -1,0 +1,40 @@
+// Shared helpers of the gfx950 field-engine library (wave64, HIP runtime only).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#define FE_WAVE 64
+
+// Status codes of the reference C ABI (FE/native_edt_v1.py:18-33):
+// 0 success, 1 invalid input (not latched), 2 runtime failure (latched by callers).
+enum { FE_OK = 0, FE_INVALID = 1, FE_RUNTIME = 2 };
+
+#define FE_TRY(expr)                                  \
+    do {                                              \
+        if ((expr) != hipSuccess) return FE_RUNTIME;  \
+    } while (0)
+
+// Selection min/max with the NumPy/Warp tie rule (a tie returns the second
+// operand, which matters for signed zeros).
+__host__ __device__ __forceinline__ float sel_min(float a, float b) { return a < b ? a : b; }
+__host__ __device__ __forceinline__ float sel_max(float a, float b) { return a > b ? a : b; }
+
+// Uniform 64-bit lane read (idx must be wave-uniform).
+__device__ __forceinline__ uint64_t readlane64(uint64_t v, int idx)
+{
+    uint32_t lo = __builtin_amdgcn_readlane((uint32_t)v, idx);
+    uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(v >> 32), idx);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// Blocks are dispatched round-robin over the 8 XCDs of MI355X.  Remapping the
+// hardware block id so that each XCD walks a contiguous range of logical tiles
+// keeps neighbouring tiles (which share cache lines) in one L2.  `grid` must be
+// a multiple of 8; logical ids >= the real tile count are idle.
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t bid, uint32_t grid)
+{
+    uint32_t per = grid >> 3;
+    return (bid & 7u) * per + (bid >> 3);
+}
+
+static inline uint32_t round8(uint64_t n) { return (uint32_t)((n + 7) & ~(uint64_t)7); }
