@@ -25,6 +25,14 @@ SIGNATURES = {
     "fe_edt_pair_create": (P, [ct.c_int, ct.c_int, ct.c_int]),
     "fe_edt_device": (ct.c_int, [P, ct.c_int, ct.c_int, ct.c_int, ct.c_int, P, P, P]),
     "fe_edt_signed_device": (ct.c_int, [P, I64, F64, P, P]),
+    # field build
+    "fe_field_build": (ct.c_int, [P, ct.c_int, ct.c_int, P, ct.c_int, P, ct.c_int, P, ct.c_int, P]),
+    "fe_field_classify": (ct.c_int, [P, ct.c_int, ct.c_int, P, ct.c_int, P]),
+    "fe_field_tiles_ids": (ct.c_int, [P, P, I64, P]),
+    "fe_field_reconstruct": (ct.c_int, [P, ct.c_int, I64, P, P, P]),
+    "fe_field_checksum": (ct.c_int, [P, ct.c_int, F64, P, P]),
+    "fe_dense_classify": (ct.c_int, [P, ct.c_int, ct.c_int, ct.c_int, ct.c_int, ct.c_float, P, P]),
+    "fe_dense_tiles": (ct.c_int, [P, ct.c_int, ct.c_int, ct.c_int, ct.c_int, P, I64, P, P]),
 }
 
 _lib = None

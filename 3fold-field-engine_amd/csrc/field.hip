@@ -1,0 +1,619 @@
+// Block-sparse SDF build for gfx950: per-block eight-corner classification,
+// ascending surface-block compaction, tile evaluation, reconstruction and the
+// deterministic int64 checksum.
+//
+// Reference semantics (FE/faltkarna_v1.py): a block of B^3 voxels is
+// classified from the recipe's fp32 values at its eight corners o + {0,B*h}
+// (:128-164); mixed signs or min|v| <= margin (= fp32(sqrt(3)*B*h)) make it a
+// surface block whose B^3 voxel values are stored in ascending block order
+// (:166-190, :277); interior/exterior blocks store nothing.  Every value here
+// is bit-identical to evaluating the full recipe at that point.
+//
+// MI355X design:
+//  * One work item = one superblock (SB^3 blocks, 32-64 voxels on a side) of
+//    one recipe; any number of recipes run in one launch through a table of
+//    field descriptors (batched builds and the server use the same kernels).
+//  * Exact Lipschitz culling per superblock: each primitive is bounded on the
+//    superblock's box by value(centre) +- (half diagonal + fp32 slack); the
+//    ordered min/max fold of those intervals is a composition of clamps, so a
+//    workgroup-wide scan gives the fold interval before every step, and a step
+//    that provably selects the running value everywhere (strict inequality) is
+//    dropped.  Dropped steps are identities, so results do not change.
+//  * Corner and voxel loops run the primitive list in the outer loop with
+//    scalar (wave-uniform) parameter loads and several points per lane.
+//  * Compaction: wave ballots + popcount prefix; each chunk of 1024 blocks
+//    finds its output offset by summing the counts of earlier chunks.
+//  * Tiles: a persistent grid of waves walks every recipe's active blocks;
+//    n_active is read on the device, so the whole build needs no host sync.
+#include "fe_common.hpp"
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#define FE_BIG 1.0e6f
+#define FE_MAX_PRIMS 4096
+#define FE_CHUNK 1024
+
+struct Prim {
+    int32_t kind, op;
+    float cx, cy, cz, a, b, c;
+};
+static_assert(sizeof(Prim) == 32, "prim layout");
+
+// One field to build (160 bytes).  Pointers are device addresses.
+struct FieldDesc {
+    float x0, y0, z0, pitch;
+    float margin, slack;
+    int32_t nprims, B;
+    int32_t nx, ny, nz, SB;
+    int32_t nbx, nby, nbz, cap;
+    int32_t nsbx, nsby, nsbz, corrupt;
+    const Prim* prims;
+    int32_t* kind;
+    float* cma;
+    uint16_t* lists;
+    int32_t* list_len;
+    int32_t* ids;
+    int32_t* slot;
+    float* tiles;
+    int32_t* counts;  // [0] n_active, [1] overflow (active blocks beyond tile_cap)
+    int64_t tile_cap;
+};
+static_assert(sizeof(FieldDesc) == 160, "descriptor layout");
+
+// ---------------------------------------------------------------- primitives
+// Same operation order as oracle/fe_oracle.c or_prim_eval (IKARUS emitted
+// fp32 source, FE/ikarus_v1/eval_warp.py:182-224; FE/faltkarna_v1.py:91-104).
+__device__ __forceinline__ float prim_eval(int kind, float cx, float cy, float cz, float a, float b, float c, float x,
+                                           float y, float z)
+{
+    float lx = x - cx, ly = y - cy, lz = z - cz;
+    switch (kind) {
+    case 0: {
+        float s = lx * lx + ly * ly;
+        s = s + lz * lz;
+        return sqrtf(s) - a;
+    }
+    case 1: {
+        float qx = fabsf(lx) - a, qy = fabsf(ly) - b, qz = fabsf(lz) - c;
+        float ox = sel_max(qx, 0.0f), oy = sel_max(qy, 0.0f), oz = sel_max(qz, 0.0f);
+        float s = ox * ox + oy * oy;
+        s = s + oz * oz;
+        return sqrtf(s) + sel_min(sel_max(qx, sel_max(qy, qz)), 0.0f);
+    }
+    case 2:
+    case 3:
+    case 4: {
+        float r2, ax;
+        if (kind == 4) { r2 = lx * lx + ly * ly; ax = lz; }
+        else if (kind == 3) { r2 = lx * lx + lz * lz; ax = ly; }
+        else { r2 = ly * ly + lz * lz; ax = lx; }
+        float qx = sqrtf(r2) - a, qy = fabsf(ax) - b;
+        float ox = sel_max(qx, 0.0f), oy = sel_max(qy, 0.0f);
+        return sqrtf(ox * ox + oy * oy) + sel_min(sel_max(qx, qy), 0.0f);
+    }
+    case 5: {
+        float qx = sqrtf(lx * lx + ly * ly) - a;
+        return sqrtf(qx * qx + lz * lz) - b;
+    }
+    default:  // 6: infinite z-cylinder
+        return sqrtf(lx * lx + ly * ly) - a;
+    }
+}
+
+__device__ __forceinline__ float fold(int op, float f, float v)
+{
+    return op == 0 ? sel_min(f, v) : (op == 1 ? sel_max(f, -v) : sel_max(f, v));
+}
+
+// Evaluates K points over a primitive list (list == nullptr: all primitives).
+// Primitive parameters are wave-uniform scalar loads shared by the K points.
+template <int K>
+__device__ __forceinline__ void eval_points(const Prim* __restrict__ prims, const uint16_t* list, int n, const float* x,
+                                            const float* y, const float* z, float* f)
+{
+    for (int j = 0; j < n; ++j) {
+        int idx = list ? __builtin_amdgcn_readfirstlane((int)list[j]) : j;
+        const Prim p = prims[idx];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            float v = prim_eval(p.kind, p.cx, p.cy, p.cz, p.a, p.b, p.c, x[k], y[k], z[k]);
+            f[k] = j == 0 ? v : fold(p.op, f[k], v);
+        }
+    }
+}
+
+__device__ __forceinline__ int find_item(const int32_t* prefix, int R, int item)
+{
+    int lo = 0, hi = R;  // largest r with prefix[r] <= item
+    while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (prefix[mid] <= item) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// ---------------------------------------------------------------- culling scan
+// A clamp x -> min(max(x,l),u).  c2(c1(x)) is the clamp (c2(l1), c2(u1)).
+struct Clamp2 {
+    float ll, lu, hl, hu;  // clamps of the lower and upper fold bounds
+};
+__device__ __forceinline__ float capply(float x, float l, float u) { return fminf(fmaxf(x, l), u); }
+__device__ __forceinline__ Clamp2 ccompose(const Clamp2& a, const Clamp2& b)  // a first, then b
+{
+    return {capply(a.ll, b.ll, b.lu), capply(a.lu, b.ll, b.lu), capply(a.hl, b.hl, b.hu), capply(a.hu, b.hl, b.hu)};
+}
+__device__ __forceinline__ Clamp2 cshfl_up(const Clamp2& v, int d)
+{
+    return {__shfl_up(v.ll, d), __shfl_up(v.lu, d), __shfl_up(v.hl, d), __shfl_up(v.hu, d)};
+}
+
+// Interval of one primitive's fp32 values over a box given by centre and
+// half diagonal; `slack` covers fp32 rounding of both evaluations.
+__device__ __forceinline__ void prim_interval(const Prim& p, float cx, float cy, float cz, float rad, float slack,
+                                              float& lo, float& hi)
+{
+    float v = prim_eval(p.kind, p.cx, p.cy, p.cz, p.a, p.b, p.c, cx, cy, cz);
+    lo = v - rad - slack;
+    hi = v + rad + slack;
+}
+
+// Builds the ordered list of primitives that can change the fold somewhere in
+// the box.  Called by all 256 threads; returns the list length (list in LDS).
+__device__ int cull_box(const Prim* __restrict__ prims, int nprims, float cx, float cy, float cz, float rad,
+                        float slack, uint16_t* list, Clamp2* wtot, int* wcnt)
+{
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    float Flo = 0.0f, Fhi = 0.0f;  // fold interval after the processed chunks
+    int n = 0;
+    for (int base = 0; base < nprims; base += 256) {
+        const int i = base + tid;
+        Clamp2 cl = {-INFINITY, INFINITY, -INFINITY, INFINITY};
+        float lo = 0.0f, hi = 0.0f;
+        int op = 0;
+        if (i < nprims) {
+            const Prim p = prims[i];
+            prim_interval(p, cx, cy, cz, rad, slack, lo, hi);
+            op = p.op;
+            if (i == 0) cl = {lo, lo, hi, hi};
+            else if (op == 0) cl = {-INFINITY, lo, -INFINITY, hi};
+            else if (op == 1) cl = {-hi, INFINITY, -lo, INFINITY};
+            else cl = {lo, INFINITY, hi, INFINITY};
+        }
+        // inclusive wave scan
+        Clamp2 inc = cl;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            Clamp2 o = cshfl_up(inc, d);
+            if (lane >= d) inc = ccompose(o, inc);
+        }
+        if (lane == 63) wtot[wid] = inc;
+        __syncthreads();
+        Clamp2 pre = {-INFINITY, INFINITY, -INFINITY, INFINITY};
+        for (int w = 0; w < wid; ++w) pre = ccompose(pre, wtot[w]);
+        Clamp2 exc = cshfl_up(inc, 1);
+        if (lane == 0) exc = {-INFINITY, INFINITY, -INFINITY, INFINITY};
+        const Clamp2 before = ccompose(pre, exc);
+        const float blo = capply(Flo, before.ll, before.lu), bhi = capply(Fhi, before.hl, before.hu);
+        bool keep = false;
+        if (i < nprims) {
+            if (i == 0) keep = true;
+            else if (op == 0) keep = !(lo > bhi);
+            else if (op == 1) keep = !(-lo < blo);
+            else keep = !(hi < blo);
+        }
+        const uint64_t bal = __ballot(keep);
+        if (lane == 0) wcnt[wid] = __popcll(bal);
+        __syncthreads();
+        int off = n;
+        for (int w = 0; w < wid; ++w) off += wcnt[w];
+        if (keep) list[off + __popcll(bal & ((1ull << lane) - 1))] = (uint16_t)i;
+        Clamp2 all = {-INFINITY, INFINITY, -INFINITY, INFINITY};
+        for (int w = 0; w < 4; ++w) all = ccompose(all, wtot[w]);
+        n += wcnt[0] + wcnt[1] + wcnt[2] + wcnt[3];
+        const float nlo = capply(Flo, all.ll, all.lu), nhi = capply(Fhi, all.hl, all.hu);
+        Flo = nlo;
+        Fhi = nhi;
+        __syncthreads();
+    }
+    return n;
+}
+
+// ---------------------------------------------------------------- classify
+// K = (block,corner) pairs per thread = SB^3*8/256.
+template <int K>
+__global__ __launch_bounds__(256) void k_classify(const FieldDesc* __restrict__ descs, const int32_t* __restrict__ sb_prefix,
+                                                  int R, int total, uint32_t grid)
+{
+    __shared__ uint16_t list[FE_MAX_PRIMS];
+    __shared__ Clamp2 wtot[4];
+    __shared__ int wcnt[4];
+    const uint32_t item = xcd_tile(blockIdx.x, grid);
+    if (item >= (uint32_t)total) return;
+    const int r = find_item(sb_prefix, R, item);
+    const FieldDesc& D = descs[r];
+    const int sb = item - sb_prefix[r];
+    const int SB = D.SB, B = D.B;
+    const int sbz = sb % D.nsbz, sby = (sb / D.nsbz) % D.nsby, sbx = sb / (D.nsbz * D.nsby);
+    const int bx0 = sbx * SB, by0 = sby * SB, bz0 = sbz * SB;
+    // superblock box: every corner and voxel of its blocks lies inside
+    const float lx = D.x0 + (float)(bx0 * B) * D.pitch, ux = D.x0 + (float)((bx0 + SB) * B) * D.pitch;
+    const float ly = D.y0 + (float)(by0 * B) * D.pitch, uy = D.y0 + (float)((by0 + SB) * B) * D.pitch;
+    const float lz = D.z0 + (float)(bz0 * B) * D.pitch, uz = D.z0 + (float)((bz0 + SB) * B) * D.pitch;
+    const float cx = 0.5f * (lx + ux), cy = 0.5f * (ly + uy), cz = 0.5f * (lz + uz);
+    const float ex = ux - lx, ey = uy - ly, ez = uz - lz;
+    const float rad = 0.5f * sqrtf(ex * ex + ey * ey + ez * ez) * 1.0001f;
+    const int n = cull_box(D.prims, D.nprims, cx, cy, cz, rad, D.slack, list, wtot, wcnt);
+    const int tid = threadIdx.x;
+    if (tid == 0) D.list_len[sb] = n <= D.cap ? n : -1;
+    if (n <= D.cap)
+        for (int j = tid; j < n; j += 256) D.lists[(int64_t)sb * D.cap + j] = list[j];
+
+    float x[K], y[K], z[K], f[K];
+    int blk[K];
+    const float side = (float)B * D.pitch;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int q = tid + k * 256, bl = q >> 3, c = q & 7;
+        const int bi = bx0 + bl / (SB * SB), bj = by0 + (bl / SB) % SB, bk = bz0 + bl % SB;
+        blk[k] = (bi < D.nbx && bj < D.nby && bk < D.nbz) ? (bi * D.nby + bj) * D.nbz + bk : -1;
+        const float ox = D.x0 + (float)(bi * B) * D.pitch;
+        const float oy = D.y0 + (float)(bj * B) * D.pitch;
+        const float oz = D.z0 + (float)(bk * B) * D.pitch;
+        x[k] = ox + (float)(c & 1) * side;
+        y[k] = oy + (float)((c >> 1) & 1) * side;
+        z[k] = oz + (float)((c >> 2) & 1) * side;
+    }
+    eval_points<K>(D.prims, list, n, x, y, z, f);
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        float mn = fabsf(f[k]);
+        int pos = f[k] >= 0.0f, neg = !pos;
+#pragma unroll
+        for (int s = 1; s < 8; s <<= 1) {
+            mn = sel_min(mn, __shfl_xor(mn, s));
+            pos |= __shfl_xor(pos, s);
+            neg |= __shfl_xor(neg, s);
+        }
+        mn = sel_min(1.0e30f, mn);
+        if ((tid & 7) == 0 && blk[k] >= 0) {
+            D.cma[blk[k]] = mn;
+            D.kind[blk[k]] = (pos && neg) ? 0 : (mn <= D.margin ? 0 : (neg ? -1 : 1));
+        }
+    }
+}
+
+// ---------------------------------------------------------------- compaction
+__global__ __launch_bounds__(256) void k_compact_count(const FieldDesc* __restrict__ descs,
+                                                       const int32_t* __restrict__ ch_prefix, int R, int total,
+                                                       int32_t* __restrict__ chunk_count)
+{
+    __shared__ int wc[4];
+    const int item = blockIdx.x;
+    if (item >= total) return;
+    const int r = find_item(ch_prefix, R, item);
+    const FieldDesc& D = descs[r];
+    const int64_t nb = (int64_t)D.nbx * D.nby * D.nbz;
+    const int64_t b0 = (int64_t)(item - ch_prefix[r]) * FE_CHUNK;
+    int c = 0;
+    for (int k = 0; k < FE_CHUNK / 256; ++k) {
+        const int64_t b = b0 + k * 256 + threadIdx.x;
+        c += __popcll(__ballot(b < nb && D.kind[b] == 0));
+    }
+    if ((threadIdx.x & 63) == 0) wc[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) chunk_count[item] = wc[0] + wc[1] + wc[2] + wc[3];
+}
+
+__global__ __launch_bounds__(256) void k_compact_write(const FieldDesc* __restrict__ descs,
+                                                       const int32_t* __restrict__ ch_prefix, int R, int total,
+                                                       const int32_t* __restrict__ chunk_count)
+{
+    __shared__ int wsum[4];
+    __shared__ int wc[4];
+    const int item = blockIdx.x;
+    if (item >= total) return;
+    const int r = find_item(ch_prefix, R, item);
+    const FieldDesc& D = descs[r];
+    const int local = item - ch_prefix[r];
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    int s = 0;
+    for (int j = tid; j < local; j += 256) s += chunk_count[ch_prefix[r] + j];
+    for (int d = 32; d >= 1; d >>= 1) s += __shfl_xor(s, d);
+    if (lane == 0) wsum[wid] = s;
+    __syncthreads();
+    int off = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+    const int64_t nb = (int64_t)D.nbx * D.nby * D.nbz;
+    const int64_t b0 = (int64_t)local * FE_CHUNK;
+    for (int k = 0; k < FE_CHUNK / 256; ++k) {
+        const int64_t b = b0 + k * 256 + tid;
+        const bool act = b < nb && D.kind[b] == 0;
+        const uint64_t bal = __ballot(act);
+        __syncthreads();
+        if (lane == 0) wc[wid] = __popcll(bal);
+        __syncthreads();
+        int o = off;
+        for (int w = 0; w < wid; ++w) o += wc[w];
+        o += __popcll(bal & ((1ull << lane) - 1));
+        if (b < nb) {
+            D.slot[b] = act ? o : -1;
+            if (act) D.ids[o] = (int32_t)b;
+        }
+        off += wc[0] + wc[1] + wc[2] + wc[3];
+    }
+    if (local == ch_prefix[r + 1] - ch_prefix[r] - 1 && tid == 0) {
+        D.counts[0] = off;
+        D.counts[1] = off > D.tile_cap ? 1 : 0;
+    }
+}
+
+// ---------------------------------------------------------------- tiles
+// Persistent: every wave walks all recipes' active blocks with a global
+// stride; one wave evaluates one B^3 tile, KV voxels per lane per pass.
+template <int KV>
+__global__ __launch_bounds__(256) void k_tiles(const FieldDesc* __restrict__ descs, int R)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int64_t nwaves = (int64_t)gridDim.x * 4;
+    for (int r = 0; r < R; ++r) {
+        const FieldDesc& D = descs[r];
+        const int64_t na = min((int64_t)D.counts[0], D.tile_cap);
+        const int B = D.B, b3 = B * B * B;
+        for (int64_t a = wave; a < na; a += nwaves) {
+            int lin = D.ids[a] + (D.corrupt ? 1 : 0);
+            const int bk = lin % D.nbz, bj = (lin / D.nbz) % D.nby, bi = lin / (D.nbz * D.nby);
+            const int bl = D.ids[a];
+            const int sb = ((bl / (D.nbz * D.nby)) / D.SB * D.nsby + ((bl / D.nbz) % D.nby) / D.SB) * D.nsbz +
+                           (bl % D.nbz) / D.SB;
+            // the planted fault evaluates a shifted block, so it skips the culled list
+            const int nl = D.corrupt ? -1 : D.list_len[sb];
+            const uint16_t* list = nl >= 0 ? D.lists + (int64_t)sb * D.cap : nullptr;
+            const int n = nl >= 0 ? nl : D.nprims;
+            float* out = D.tiles + a * b3;
+            for (int v0 = 0; v0 < b3; v0 += 64 * KV) {
+                float x[KV], y[KV], z[KV], f[KV];
+#pragma unroll
+                for (int k = 0; k < KV; ++k) {
+                    const int l = min(v0 + k * 64 + lane, b3 - 1);
+                    const int li = l / (B * B), lj = (l / B) % B, lk = l % B;
+                    x[k] = D.x0 + (float)(bi * B + li) * D.pitch;
+                    y[k] = D.y0 + (float)(bj * B + lj) * D.pitch;
+                    z[k] = D.z0 + (float)(bk * B + lk) * D.pitch;
+                }
+                eval_points<KV>(D.prims, list, n, x, y, z, f);
+#pragma unroll
+                for (int k = 0; k < KV; ++k) {
+                    const int l = v0 + k * 64 + lane;
+                    if (l < b3) out[l] = f[k];
+                }
+            }
+        }
+    }
+}
+
+// Tiles for an explicit host-provided id list (the plain per-call path); uses
+// the culled lists that the classify launch stored.
+template <int KV>
+__global__ __launch_bounds__(256) void k_tiles_ids(FieldDesc D, const int32_t* __restrict__ ids, int64_t na)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t a = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (a >= na) return;
+    const int B = D.B, b3 = B * B * B;
+    const int lin = ids[a] + (D.corrupt ? 1 : 0);
+    const int bk = lin % D.nbz, bj = (lin / D.nbz) % D.nby, bi = lin / (D.nbz * D.nby);
+    const int bl = ids[a];
+    const int sb = ((bl / (D.nbz * D.nby)) / D.SB * D.nsby + ((bl / D.nbz) % D.nby) / D.SB) * D.nsbz +
+                   (bl % D.nbz) / D.SB;
+    const int nl = D.corrupt ? -1 : D.list_len[sb];
+    const uint16_t* list = nl >= 0 ? D.lists + (int64_t)sb * D.cap : nullptr;
+    const int n = nl >= 0 ? nl : D.nprims;
+    float* out = D.tiles + a * b3;
+    for (int v0 = 0; v0 < b3; v0 += 64 * KV) {
+        float x[KV], y[KV], z[KV], f[KV];
+#pragma unroll
+        for (int k = 0; k < KV; ++k) {
+            const int l = min(v0 + k * 64 + lane, b3 - 1);
+            const int li = l / (B * B), lj = (l / B) % B, lk = l % B;
+            x[k] = D.x0 + (float)(bi * B + li) * D.pitch;
+            y[k] = D.y0 + (float)(bj * B + lj) * D.pitch;
+            z[k] = D.z0 + (float)(bk * B + lk) * D.pitch;
+        }
+        eval_points<KV>(D.prims, list, n, x, y, z, f);
+#pragma unroll
+        for (int k = 0; k < KV; ++k) {
+            const int l = v0 + k * 64 + lane;
+            if (l < b3) out[l] = f[k];
+        }
+    }
+}
+
+// ---------------------------------------------------------------- reconstruct / checksum
+// Cropped dense field (FE/faltkarna_v1.py:302-325) and, when occ != nullptr,
+// the occupancy dense < 0 (FE/faltkarna_v1_mesh_to_sdf.py:695-710).
+__global__ __launch_bounds__(256) void k_reconstruct(const FieldDesc* __restrict__ descs, int r, float* __restrict__ dense,
+                                                     uint8_t* __restrict__ occ)
+{
+    const FieldDesc& D = descs[r];
+    const int64_t n = (int64_t)D.nx * D.ny * D.nz;
+    const int B = D.B;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+        const int z = (int)(i % D.nz);
+        const int64_t xy = i / D.nz;
+        const int y = (int)(xy % D.ny), x = (int)(xy / D.ny);
+        const int64_t b = ((int64_t)(x / B) * D.nby + y / B) * D.nbz + z / B;
+        const int s = D.slot[b];
+        float v;
+        if (s >= 0 && s < D.tile_cap) v = D.tiles[(int64_t)s * B * B * B + ((x % B) * B + (y % B)) * B + (z % B)];
+        else v = D.kind[b] < 0 ? -FE_BIG : FE_BIG;
+        dense[i] = v;
+        if (occ) occ[i] = v < 0.0f ? 1 : 0;
+    }
+}
+
+// Sum over stored tile values v < 0 of int64(f64(v)*scale) (FE/faltkarna_v1.py:249-255).
+__global__ __launch_bounds__(256) void k_checksum(const FieldDesc* __restrict__ descs, int r, double scale,
+                                                  unsigned long long* __restrict__ out)
+{
+    const FieldDesc& D = descs[r];
+    const int64_t n = min((int64_t)D.counts[0], D.tile_cap) * D.B * D.B * D.B;
+    long long s = 0;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+        const float v = D.tiles[i];
+        if (v < 0.0f) s += (long long)((double)v * scale);
+    }
+    for (int d = 32; d >= 1; d >>= 1) s += __shfl_xor(s, d);
+    if ((threadIdx.x & 63) == 0) atomicAdd(out, (unsigned long long)s);
+}
+
+// ---------------------------------------------------------------- dense-source classify
+// FE/faltkarna_v1_mesh_to_sdf.py:602-632 (corners clamped to n-1).
+__global__ __launch_bounds__(256) void k_classify_dense(const float* __restrict__ dense, int nx, int ny, int nz, int B,
+                                                        int nbx, int nby, int nbz, float margin, int32_t* __restrict__ kind)
+{
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= (int64_t)nbx * nby * nbz) return;
+    const int bk = (int)(t % nbz), bj = (int)((t / nbz) % nby), bi = (int)(t / ((int64_t)nbz * nby));
+    float mn = 1.0e30f;
+    int pos = 0, neg = 0;
+    for (int c = 0; c < 8; ++c) {
+        const int cx = min(bi * B + (c & 1) * B, nx - 1), cy = min(bj * B + ((c >> 1) & 1) * B, ny - 1),
+                  cz = min(bk * B + ((c >> 2) & 1) * B, nz - 1);
+        const float v = dense[((int64_t)cx * ny + cy) * nz + cz];
+        mn = sel_min(mn, fabsf(v));
+        if (v >= 0.0f) pos = 1;
+        else neg = 1;
+    }
+    kind[t] = (pos && neg) ? 0 : (mn <= margin ? 0 : (neg ? -1 : 1));
+}
+
+// FE/faltkarna_v1_mesh_to_sdf.py:635-654 (voxels clamped to n-1).
+__global__ __launch_bounds__(256) void k_tiles_dense(const float* __restrict__ dense, int nx, int ny, int nz, int B,
+                                                     int nby, int nbz, const int32_t* __restrict__ ids, int64_t na,
+                                                     float* __restrict__ tiles)
+{
+    const int64_t b3 = (int64_t)B * B * B;
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= na * b3) return;
+    const int64_t a = t / b3;
+    const int l = (int)(t % b3);
+    const int lin = ids[a];
+    const int bk = lin % nbz, bj = (lin / nbz) % nby, bi = lin / (nbz * nby);
+    const int x = min(bi * B + l / (B * B), nx - 1), y = min(bj * B + (l / B) % B, ny - 1), z = min(bk * B + l % B, nz - 1);
+    tiles[t] = dense[((int64_t)x * ny + y) * nz + z];
+}
+
+// ---------------------------------------------------------------- C ABI
+static int classify_k(int B) { return B == 4 ? 16 : 2; }
+
+// Launches classify -> compaction -> tiles for R recipes described by the
+// device descriptor table.  sb_prefix / ch_prefix: device int32[R+1]
+// exclusive prefixes of superblock and 1024-block chunk counts; chunk_count:
+// device int32[ch_prefix[R]].  Every recipe in one call shares one block edge.
+extern "C" int fe_field_build(const void* descs, int R, int B, const int32_t* sb_prefix, int total_sb,
+                              const int32_t* ch_prefix, int total_chunks, int32_t* chunk_count, int tile_waves,
+                              void* stream) noexcept
+{
+    if (!descs || R < 1 || (B != 4 && B != 8 && B != 16) || total_sb < 1 || total_chunks < 1 || !chunk_count)
+        return FE_INVALID;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const FieldDesc* d = static_cast<const FieldDesc*>(descs);
+    const uint32_t g = round8(total_sb);
+    if (classify_k(B) == 16)
+        hipLaunchKernelGGL(k_classify<16>, dim3(g), dim3(256), 0, st, d, sb_prefix, R, total_sb, g);
+    else
+        hipLaunchKernelGGL(k_classify<2>, dim3(g), dim3(256), 0, st, d, sb_prefix, R, total_sb, g);
+    FE_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_compact_count, dim3(total_chunks), dim3(256), 0, st, d, ch_prefix, R, total_chunks, chunk_count);
+    FE_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_compact_write, dim3(total_chunks), dim3(256), 0, st, d, ch_prefix, R, total_chunks, chunk_count);
+    FE_TRY(hipGetLastError());
+    const int blocks = tile_waves > 0 ? (tile_waves + 3) / 4 : 2048;
+    if (B == 4) hipLaunchKernelGGL(k_tiles<1>, dim3(blocks), dim3(256), 0, st, d, R);
+    else hipLaunchKernelGGL(k_tiles<8>, dim3(blocks), dim3(256), 0, st, d, R);
+    FE_TRY(hipGetLastError());
+    return FE_OK;
+}
+
+// Classification only (the first launch of the plain per-call path).
+extern "C" int fe_field_classify(const void* descs, int R, int B, const int32_t* sb_prefix, int total_sb,
+                                 void* stream) noexcept
+{
+    if (!descs || R < 1 || (B != 4 && B != 8 && B != 16) || total_sb < 1) return FE_INVALID;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const FieldDesc* d = static_cast<const FieldDesc*>(descs);
+    const uint32_t g = round8(total_sb);
+    if (classify_k(B) == 16)
+        hipLaunchKernelGGL(k_classify<16>, dim3(g), dim3(256), 0, st, d, sb_prefix, R, total_sb, g);
+    else
+        hipLaunchKernelGGL(k_classify<2>, dim3(g), dim3(256), 0, st, d, sb_prefix, R, total_sb, g);
+    FE_TRY(hipGetLastError());
+    return FE_OK;
+}
+
+// Tiles for host-compacted ids (plain per-call path): desc is a host copy of
+// one descriptor; ids and the tile buffer are device pointers.
+extern "C" int fe_field_tiles_ids(const void* host_desc, const int32_t* ids, int64_t n_active, void* stream) noexcept
+{
+    if (!host_desc || !ids || n_active < 0) return FE_INVALID;
+    if (n_active == 0) return FE_OK;
+    FieldDesc D;
+    std::memcpy(&D, host_desc, sizeof D);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const unsigned blocks = (unsigned)((n_active + 3) / 4);
+    if (D.B == 4) hipLaunchKernelGGL(k_tiles_ids<1>, dim3(blocks), dim3(256), 0, st, D, ids, n_active);
+    else hipLaunchKernelGGL(k_tiles_ids<8>, dim3(blocks), dim3(256), 0, st, D, ids, n_active);
+    FE_TRY(hipGetLastError());
+    return FE_OK;
+}
+
+// Cropped dense reconstruction of descriptor r (+ optional uint8 occupancy).
+extern "C" int fe_field_reconstruct(const void* descs, int r, int64_t n_voxels, float* dense, uint8_t* occ,
+                                    void* stream) noexcept
+{
+    if (!descs || r < 0 || !dense || n_voxels < 1) return FE_INVALID;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const unsigned blocks = (unsigned)std::min<int64_t>((n_voxels + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_reconstruct, dim3(blocks), dim3(256), 0, st, static_cast<const FieldDesc*>(descs), r, dense, occ);
+    FE_TRY(hipGetLastError());
+    return FE_OK;
+}
+
+// int64 fixed-point checksum of descriptor r's tiles, accumulated into *out.
+extern "C" int fe_field_checksum(const void* descs, int r, double scale, int64_t* out, void* stream) noexcept
+{
+    if (!descs || r < 0 || !out) return FE_INVALID;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    hipLaunchKernelGGL(k_checksum, dim3(1024), dim3(256), 0, st, static_cast<const FieldDesc*>(descs), r, scale,
+                       reinterpret_cast<unsigned long long*>(out));
+    FE_TRY(hipGetLastError());
+    return FE_OK;
+}
+
+// Dense-source classify and tile extraction (FE/faltkarna_v1_mesh_to_sdf.py:660-692 on the device).
+extern "C" int fe_dense_classify(const float* dense, int nx, int ny, int nz, int B, float margin, int32_t* kind,
+                                 void* stream) noexcept
+{
+    if (!dense || !kind || nx < 1 || ny < 1 || nz < 1 || B < 1) return FE_INVALID;
+    const int nbx = (nx + B - 1) / B, nby = (ny + B - 1) / B, nbz = (nz + B - 1) / B;
+    const int64_t nb = (int64_t)nbx * nby * nbz;
+    hipLaunchKernelGGL(k_classify_dense, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0,
+                       static_cast<hipStream_t>(stream), dense, nx, ny, nz, B, nbx, nby, nbz, margin, kind);
+    FE_TRY(hipGetLastError());
+    return FE_OK;
+}
+
+extern "C" int fe_dense_tiles(const float* dense, int nx, int ny, int nz, int B, const int32_t* ids, int64_t n_active,
+                              float* tiles, void* stream) noexcept
+{
+    if (!dense || !tiles || nx < 1 || ny < 1 || nz < 1 || B < 1 || n_active < 0) return FE_INVALID;
+    if (n_active == 0) return FE_OK;
+    const int nby = (ny + B - 1) / B, nbz = (nz + B - 1) / B;
+    const int64_t n = n_active * B * B * B;
+    hipLaunchKernelGGL(k_tiles_dense, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, static_cast<hipStream_t>(stream),
+                       dense, nx, ny, nz, B, nby, nbz, ids, n_active, tiles);
+    FE_TRY(hipGetLastError());
+    return FE_OK;
+}

@@ -52,6 +52,44 @@ int fe_edt_device(const uint8_t *mask, int nx, int ny, int nz, int planes, int32
 /* Signed output of edt_persistent_pair_v1/edt.cu:36-42 from a device pair. */
 int fe_edt_signed_device(const int32_t *pair, int64_t n, double pitch, float *out, void *stream);
 
+/* ---------------------------------------------------------------- field build */
+/*
+ * `descs` is a device table of R field descriptors (160 bytes each: grid
+ * origin/pitch/margin, sizes, block edge B, superblock edge, primitive table
+ * and output buffers; layout in the package's field.py FIELD_DESC).  Recipes
+ * are arrays of 32-byte primitives {int32 kind, op; float cx,cy,cz,a,b,c}.
+ */
+
+/* Replaces the classify -> np.where -> eval launches of faltkarna_v1.py:258-299
+ * and faltkarna_v1_g7_fix.py:56-112 for R recipes at once, with no host sync:
+ * eight-corner classification (faltkarna_v1.py:128-164), ascending compaction
+ * (:277), tile evaluation (:166-190).  sb_prefix/ch_prefix: device int32[R+1]
+ * exclusive prefixes of superblock and 1024-block chunk counts; chunk_count:
+ * device int32[total_chunks] scratch. */
+int fe_field_build(const void *descs, int R, int B, const int32_t *sb_prefix, int total_sb, const int32_t *ch_prefix,
+                   int total_chunks, int32_t *chunk_count, int tile_waves, void *stream);
+
+/* k_classify_testfalt launch of faltkarna_v1.py:269-273 (kinds, corner_min_abs,
+ * culled primitive lists) without compaction or tiles. */
+int fe_field_classify(const void *descs, int R, int B, const int32_t *sb_prefix, int total_sb, void *stream);
+
+/* k_eval_tile_testfalt launch of faltkarna_v1.py:283-287 over host-compacted ids;
+ * host_desc is a host copy of one descriptor whose tiles field names the output. */
+int fe_field_tiles_ids(const void *host_desc, const int32_t *ids, int64_t n_active, void *stream);
+
+/* k_fill_background + k_scatter_tiles of faltkarna_v1.py:192-236 (cropped),
+ * plus the optional occupancy of faltkarna_v1_mesh_to_sdf.py:695-710. */
+int fe_field_reconstruct(const void *descs, int r, int64_t n_voxels, float *dense, uint8_t *occupancy_or_null,
+                         void *stream);
+
+/* k_reduce_checksum_i64_safe of faltkarna_v1.py:249-255 (adds into *out). */
+int fe_field_checksum(const void *descs, int r, double scale, int64_t *out, void *stream);
+
+/* k_classify_dense / k_eval_tile_dense of faltkarna_v1_mesh_to_sdf.py:602-654. */
+int fe_dense_classify(const float *dense, int nx, int ny, int nz, int B, float margin, int32_t *kind, void *stream);
+int fe_dense_tiles(const float *dense, int nx, int ny, int nz, int B, const int32_t *ids, int64_t n_active,
+                   float *tiles, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
