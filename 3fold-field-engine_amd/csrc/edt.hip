@@ -294,6 +294,35 @@ extern "C" int fe_edt_device(const uint8_t* mask, int nx, int ny, int nz, int pl
     return FE_OK;
 }
 
+// One axis pass of fe_edt_device, for profiling and per-kernel timing:
+// axis 2 reads the uint8 mask `src` (z pass), axes 1 and 0 read int32 planes.
+extern "C" int fe_edt_axis_pass(const void* src, int32_t* dst, int nx, int ny, int nz, int planes, int axis,
+                                void* stream) noexcept
+{
+    if (!src || !dst || nx < 1 || ny < 1 || nz < 1 || nx > FE_EDT_MAX_AXIS || ny > FE_EDT_MAX_AXIS ||
+        nz > FE_EDT_MAX_AXIS || (planes != 1 && planes != 2) || axis < 0 || axis > 2)
+        return FE_INVALID;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const int64_t n = (int64_t)nx * ny * nz;
+    if (axis == 2) {
+        const int32_t sentinel =
+            (int32_t)((int64_t)(nx - 1) * (nx - 1) + (int64_t)(ny - 1) * (ny - 1) + (int64_t)(nz - 1) * (nz - 1) + 1);
+        const int64_t rows = (int64_t)nx * ny;
+        const unsigned zgrid = (unsigned)((rows + 3) / 4);
+        const uint8_t* m = static_cast<const uint8_t*>(src);
+        if (planes == 2)
+            hipLaunchKernelGGL(k_edt_zpass<true>, dim3(zgrid), dim3(256), 0, st, m, dst, rows, nz, n, sentinel, 1);
+        else
+            hipLaunchKernelGGL(k_edt_zpass<false>, dim3(zgrid), dim3(256), 0, st, m, dst, rows, nz, n, sentinel, 0);
+        FE_TRY(hipGetLastError());
+        return FE_OK;
+    }
+    const int32_t* a = static_cast<const int32_t*>(src);
+    if (axis == 1) FE_TRY(launch_lines(a, dst, ny, nz, nz, (int64_t)ny * nz, nx, planes, n, st));
+    else FE_TRY(launch_lines(a, dst, nx, (int64_t)ny * nz, nz, nz, ny, planes, n, st));
+    return FE_OK;
+}
+
 // Signed distance from a device pair (2N int32) into N float32.
 extern "C" int fe_edt_signed_device(const int32_t* pair, int64_t n, double pitch, float* out, void* stream) noexcept
 {

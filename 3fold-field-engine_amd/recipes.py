@@ -70,7 +70,7 @@ def plain_part_128(seed=1):
     return make(recs)
 
 
-def mixed_recipe(rng, n_prims, n, spread=0.35):
+def mixed_recipe(rng, n_prims, n, spread=0.3):
     """n_prims primitives, a quarter each of sphere, box, capped cylinder (random
     axis) and z-torus (r/R~U[0.2,0.5]); sizes log-uniform in [1,40] voxels
     (scaled by n/512); centres uniform in a central cube of side spread*n; fold

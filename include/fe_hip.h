@@ -49,6 +49,11 @@ void *fe_edt_pair_create(int nx, int ny, int nz);
 int fe_edt_device(const uint8_t *mask, int nx, int ny, int nz, int planes, int32_t *work, int32_t *out,
                   void *stream);
 
+/* One axis pass of fe_edt_device (per-kernel timing and profiling): axis 2
+ * reads the uint8 mask (z pass), axes 1 and 0 run the lower-envelope pass of
+ * edt_native_linear_v1/edt.cu:13-29 over int32 planes. */
+int fe_edt_axis_pass(const void *src, int32_t *dst, int nx, int ny, int nz, int planes, int axis, void *stream);
+
 /* Signed output of edt_persistent_pair_v1/edt.cu:36-42 from a device pair. */
 int fe_edt_signed_device(const int32_t *pair, int64_t n, double pitch, float *out, void *stream);
 
