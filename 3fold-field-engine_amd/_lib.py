@@ -15,6 +15,7 @@ I32, I64, F64 = ct.c_int32, ct.c_int64, ct.c_double
 # name: (restype, argtypes); every symbol is declared in include/fe_hip.h.
 SIGNATURES = {
     # EDT (reference-shaped)
+    "fe_last_error": (ct.c_char_p, []),
     "edt_squared": (ct.c_int, [P, ct.c_int, ct.c_int, ct.c_int, P]),
     "edt_squared_large": (ct.c_int, [P, ct.c_int, ct.c_int, ct.c_int, P]),
     "edt_pair_create": (P, [ct.c_int, ct.c_int, ct.c_int]),
@@ -48,6 +49,14 @@ def lib():
     if _lib is None:
         if not LIB_PATH.exists():
             raise NativeLibraryMissing(f"{LIB_PATH} is not built; run build.py (hipcc, gfx950)")
+        # One HIP runtime per process: torch's wheel carries its own
+        # libamdhip64.so.7.  Loading torch first registers that soname, so the
+        # library binds to the same runtime (and the same device context)
+        # instead of starting a second HSA runtime that would find no device.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         try:
             L = ct.CDLL(str(LIB_PATH))
         except OSError as e:

@@ -21,9 +21,20 @@
 //           correctly rounded f64 sqrt/mul and f32 sub/mul.
 #include "fe_common.hpp"
 #include <climits>
+#include <cstdio>
 #include <cmath>
 #include <cstring>
 #include <vector>
+
+static thread_local char fe_err_buf[256];
+
+void fe_note_error(hipError_t e, const char* where)
+{
+    snprintf(fe_err_buf, sizeof fe_err_buf, "%s: %s (%d)", where, hipGetErrorString(e), (int)e);
+}
+
+// Text of the last runtime failure on this host thread ("" if none).
+extern "C" const char* fe_last_error() noexcept { return fe_err_buf; }
 
 namespace {
 
@@ -357,15 +368,17 @@ int edt_squared_capped(const uint8_t* mask, int nx, int ny, int nz, int32_t* out
     uint8_t* dm = nullptr;
     int32_t *a = nullptr, *b = nullptr;
     int status = FE_RUNTIME;
+    hipError_t e = hipSuccess;
     do {
-        if (hipMalloc(&dm, (size_t)n) != hipSuccess) break;
-        if (hipMalloc(&a, (size_t)n * 4) != hipSuccess) break;
-        if (hipMalloc(&b, (size_t)n * 4) != hipSuccess) break;
-        if (hipMemcpy(dm, mask, (size_t)n, hipMemcpyHostToDevice) != hipSuccess) break;
+        if ((e = hipMalloc(&dm, (size_t)n)) != hipSuccess) break;
+        if ((e = hipMalloc(&a, (size_t)n * 4)) != hipSuccess) break;
+        if ((e = hipMalloc(&b, (size_t)n * 4)) != hipSuccess) break;
+        if ((e = hipMemcpy(dm, mask, (size_t)n, hipMemcpyHostToDevice)) != hipSuccess) break;
         if (fe_edt_device(dm, nx, ny, nz, 1, b, a, nullptr) != FE_OK) break;
-        if (hipMemcpy(output, a, (size_t)n * 4, hipMemcpyDeviceToHost) != hipSuccess) break;
+        if ((e = hipMemcpy(output, a, (size_t)n * 4, hipMemcpyDeviceToHost)) != hipSuccess) break;
         status = FE_OK;
     } while (0);
+    if (e != hipSuccess) fe_note_error(e, "edt per-call path");
     if (dm && hipFree(dm) != hipSuccess) status = FE_RUNTIME;
     if (a && hipFree(a) != hipSuccess) status = FE_RUNTIME;
     if (b && hipFree(b) != hipSuccess) status = FE_RUNTIME;

@@ -9,9 +9,19 @@
 // 0 success, 1 invalid input (not latched), 2 runtime failure (latched by callers).
 enum { FE_OK = 0, FE_INVALID = 1, FE_RUNTIME = 2 };
 
-#define FE_TRY(expr)                                  \
-    do {                                              \
-        if ((expr) != hipSuccess) return FE_RUNTIME;  \
+// Records the last runtime failure (file:line and the HIP error string) for
+// fe_last_error(); defined in edt.hip.
+void fe_note_error(hipError_t e, const char* where);
+
+#define FE_STR2(x) #x
+#define FE_STR(x) FE_STR2(x)
+#define FE_TRY(expr)                                              \
+    do {                                                          \
+        hipError_t fe_e_ = (expr);                                \
+        if (fe_e_ != hipSuccess) {                                \
+            fe_note_error(fe_e_, __FILE__ ":" FE_STR(__LINE__));  \
+            return FE_RUNTIME;                                    \
+        }                                                         \
     } while (0)
 
 // Selection min/max with the NumPy/Warp tie rule (a tie returns the second

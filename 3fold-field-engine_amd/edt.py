@@ -74,6 +74,15 @@ def _signed_from_pair(sq_out, sq_in, pitch):
     return (signed - np.sign(signed) * np.float32(0.5 * pitch)).astype(np.float32)
 
 
+def _detail(status):
+    if status != 2:
+        return ''
+    try:
+        return ': ' + _lib.lib().fe_last_error().decode()
+    except Exception:
+        return ''
+
+
 def squared_distance(binary):
     """Squared distance from every True voxel to the nearest False voxel (int64).
 
@@ -83,7 +92,7 @@ def squared_distance(binary):
     out = np.empty(source.shape, np.int32)
     status = _lib.lib().edt_squared(source.ctypes.data, *source.shape, out.ctypes.data)
     if status:
-        raise RuntimeError(f'Native EDT status {status}')
+        raise RuntimeError(f'Native EDT status {status}' + _detail(status))
     return out.astype(np.int64)
 
 
@@ -129,7 +138,7 @@ class NativeEDT(_ThreadOwned):
         status = self._fn(source.ctypes.data, *source.shape, output.ctypes.data)
         if status:
             self._failed = status != 1
-            raise RuntimeError(f'Native EDT status {status}')
+            raise RuntimeError(f'Native EDT status {status}' + _detail(status))
         return output
 
     def signed_distance(self, mask, pitch):
@@ -214,7 +223,7 @@ class PersistentPairEDT:
                                               squared.ctypes.data if squared is not None else None)
         if status:
             self._failed = status != 1
-            raise RuntimeError(f'Persistent EDT status {status}')
+            raise RuntimeError(f'Persistent EDT status {status}' + _detail(status))
         return (output, squared) if with_squared else output
 
     def close(self):
@@ -254,7 +263,7 @@ def edt_device(mask, planes=2, out=None, work=None, stream=None):
     status = _lib.lib().fe_edt_device(_lib.ptr(mask), *shape, planes, _lib.ptr(work), _lib.ptr(out),
                                       _lib.stream_handle(stream))
     if status:
-        raise RuntimeError(f'Device EDT status {status}')
+        raise RuntimeError(f'Device EDT status {status}' + _detail(status))
     return out
 
 

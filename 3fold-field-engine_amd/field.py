@@ -82,7 +82,8 @@ def _torch():
 
 def _check(status, what):
     if status:
-        raise RuntimeError(f"{what} failed with status {status}")
+        detail = _lib.lib().fe_last_error().decode() if status == 2 else ""
+        raise RuntimeError(f"{what} failed with status {status} {detail}")
 
 
 def _slack(meta, recipe):

@@ -17,6 +17,11 @@
 extern "C" {
 #endif
 
+/* Text of the last runtime (status 2) failure on the calling thread, "" if
+ * none: the failing call site and the HIP error string.  No reference
+ * counterpart (the reference reports bare status codes). */
+const char *fe_last_error(void);
+
 /* ---------------------------------------------------------------- EDT */
 
 /* Replaces edt_squared, edt_native_v1/edt.cu:18-20 (<=128 per axis, <=262144
