@@ -96,6 +96,26 @@ __global__ __launch_bounds__(256) void k_edt_zpass(const uint8_t* __restrict__ m
 // Parabola value of site s at position p over the LDS line f.
 __device__ __forceinline__ int32_t pval(const int32_t* f, int s, int p) { return f[s] + (p - s) * (p - s); }
 
+// Site b stays on the lower envelope of sites a < b < c iff the parabola of b
+// is strictly lowest somewhere, i.e. x(b,c) > x(a,b) for the real
+// intersections x; evaluated exactly in int64 without division.
+__device__ __forceinline__ bool keep_mid(const int32_t* f, int a, int b, int c)
+{
+    const int64_t Fa = (int64_t)f[a] + (int64_t)a * a, Fb = (int64_t)f[b] + (int64_t)b * b,
+                  Fc = (int64_t)f[c] + (int64_t)c * c;
+    return (Fc - Fb) * (int64_t)(b - a) > (Fb - Fa) * (int64_t)(c - b);
+}
+
+// One exact lower-envelope pass over W lines x L positions.  Thread (c, line)
+// owns chunk c (C positions) of a line.
+//  1. every thread builds the envelope stack of its chunk's sites;
+//  2. one thread per line merges the chunk stacks left to right: at each
+//     chunk boundary it pops the tail of the running envelope and the head of
+//     the next stack until the bridge between them is found (as when merging
+//     the lower hulls of x-separated point sets (s, f(s)+s^2)); a line whose
+//     values are all equal is flagged and copied through unchanged;
+//  3. every thread walks the merged envelope over its positions (the minimum
+//     is unimodal along the envelope and its site moves right with p).
 template <int W, int T, int CMAX>
 __global__ __launch_bounds__(W* T) void k_edt_lines(const int32_t* __restrict__ src, int32_t* __restrict__ dst, int L,
                                                     int64_t axis_stride, int nz, int ztiles, int64_t outer_stride,
@@ -111,11 +131,17 @@ __global__ __launch_bounds__(W* T) void k_edt_lines(const int32_t* __restrict__ 
     const int64_t base = (int64_t)plane * plane_stride + (int64_t)outer * outer_stride + (int64_t)zt * W;
     const int nlines = min(W, nz - zt * W);
     const int LP = L + 1;
-    const int NT = W * T;
-    int32_t* f = lds;                                        // [W][LP]
-    int32_t* minf = f + W * LP;                              // [NT]
-    int32_t* ksz = minf + NT;                                // [NT]
-    uint16_t* stk = reinterpret_cast<uint16_t*>(ksz + NT);  // [CMAX][NT]
+    constexpr int NT = W * T;
+    int32_t* f = lds;                                   // [W][LP]
+    int32_t* cmin = f + W * LP;                         // [NT] chunk minimum
+    int32_t* cmax = cmin + NT;                          // [NT] chunk maximum
+    uint16_t* stk = reinterpret_cast<uint16_t*>(cmax + NT);  // [CMAX][NT]
+    uint16_t* blo = stk + CMAX * NT;                    // [NT] surviving range of a chunk stack
+    uint16_t* bhi = blo + NT;                           // [NT]
+    uint8_t* lsb = reinterpret_cast<uint8_t*>(bhi + NT);  // [NT] nearest nonempty chunk <= c
+    uint8_t* rsb = lsb + NT;                            // [NT] nearest nonempty chunk >= c
+    uint8_t* bsl = rsb + NT;                            // [W][T] merge stack of chunk ids
+    uint8_t* uni = bsl + NT;                            // [W] constant-line flag
     const int tid = threadIdx.x;
 
     for (int i = tid; i < W * L; i += NT) {
@@ -130,83 +156,151 @@ __global__ __launch_bounds__(W* T) void k_edt_lines(const int32_t* __restrict__ 
     const bool active = line < nlines;
     const int32_t* fl = f + line * LP;
 
-    // Step 1: exact lower envelope of this chunk's parabolas.
+    // 1. envelope stack of this chunk
     int k = -1;
-    int32_t mf = INT_MAX;
+    int32_t mn = INT_MAX, mx = INT_MIN;
     if (active) {
         for (int q = p0; q < p1; ++q) {
             const int32_t fq = fl[q];
-            mf = min(mf, fq);
-            const int64_t Fq = (int64_t)fq + (int64_t)q * q;
-            while (k >= 1) {
-                const int a = stk[(k - 1) * NT + tid], b = stk[k * NT + tid];
-                const int64_t Fa = (int64_t)fl[a] + (int64_t)a * a, Fb = (int64_t)fl[b] + (int64_t)b * b;
-                // parabola b is nowhere strictly lowest when x(q,b) <= x(b,a)
-                if ((Fq - Fb) * (int64_t)(b - a) <= (Fb - Fa) * (int64_t)(q - b)) --k;
-                else break;
-            }
+            mn = min(mn, fq);
+            mx = max(mx, fq);
+            while (k >= 1 && !keep_mid(fl, stk[(k - 1) * NT + tid], stk[k * NT + tid], q)) --k;
             stk[++k * NT + tid] = (uint16_t)q;
         }
     }
-    minf[tid] = mf;
-    ksz[tid] = k + 1;
+    cmin[tid] = mn;
+    cmax[tid] = mx;
+    blo[tid] = 0;
+    bhi[tid] = (uint16_t)(k + 1);
+    __syncthreads();
+
+    // 2. merge the chunk stacks of each line (one thread per line)
+    if (tid < W && tid < nlines) {
+        const int ln = tid;
+        const int32_t* fm = f + ln * LP;
+        const int32_t v0 = cmin[ln];
+        bool flat = true;
+        for (int b = 0; b < T; ++b) {
+            const int t = b * W + ln;
+            if (bhi[t] == 0) continue;
+            flat = flat && cmin[t] == v0 && cmax[t] == v0;
+        }
+        uni[ln] = flat;
+        if (!flat) {
+            uint8_t* bs = bsl + ln * T;
+            int nbs = 0;
+            for (int b = 0; b < T; ++b) {
+                const int t = b * W + ln;
+                int lo = 0;
+                const int hi = bhi[t];
+                if (hi == 0) continue;
+                bool changed = nbs > 0;
+                while (changed) {
+                    changed = false;
+                    const int s3 = stk[lo * NT + t];
+                    while (nbs > 0) {  // tail of the running envelope
+                        const int gt = bs[nbs - 1] * W + ln;
+                        const int gh = bhi[gt], gl = blo[gt];
+                        const int s1 = stk[(gh - 1) * NT + gt];
+                        int s2;
+                        if (gh - gl >= 2) s2 = stk[(gh - 2) * NT + gt];
+                        else if (nbs >= 2) {
+                            const int pt = bs[nbs - 2] * W + ln;
+                            s2 = stk[(bhi[pt] - 1) * NT + pt];
+                        } else break;
+                        if (keep_mid(fm, s2, s1, s3)) break;
+                        bhi[gt] = (uint16_t)(gh - 1);
+                        if (gh - 1 == gl) --nbs;
+                    }
+                    while (hi - lo >= 2 && nbs > 0) {  // head of the next stack
+                        const int gt = bs[nbs - 1] * W + ln;
+                        const int s1 = stk[(bhi[gt] - 1) * NT + gt];
+                        if (keep_mid(fm, s1, stk[lo * NT + t], stk[(lo + 1) * NT + t])) break;
+                        ++lo;
+                        changed = true;
+                    }
+                }
+                blo[t] = (uint16_t)lo;
+                bs[nbs++] = (uint8_t)b;
+            }
+            int last = 255;
+            for (int b = 0; b < T; ++b) {
+                const int t = b * W + ln;
+                if (bhi[t] > blo[t]) last = b;
+                lsb[t] = (uint8_t)last;
+            }
+            int nxt = 255;
+            for (int b = T - 1; b >= 0; --b) {
+                const int t = b * W + ln;
+                if (bhi[t] > blo[t]) nxt = b;
+                rsb[t] = (uint8_t)nxt;
+            }
+        }
+    }
     __syncthreads();
     if (!active || p0 >= p1) return;
 
-    // Step 2: evaluate own and neighbouring chunk envelopes.
-    int32_t best[CMAX];
-    {
-        int kk = 0;
-        const int n = k + 1;
-#pragma unroll
-        for (int i = 0; i < CMAX; ++i) {
-            const int p = p0 + i;
-            if (p < p1) {
-                while (kk + 1 < n && pval(fl, stk[(kk + 1) * NT + tid], p) <= pval(fl, stk[kk * NT + tid], p)) ++kk;
-                best[i] = pval(fl, stk[kk * NT + tid], p);
+    int64_t out = base + (int64_t)p0 * axis_stride + line;
+    if (uni[line]) {
+        for (int p = p0; p < p1; ++p, out += axis_stride) dst[out] = fl[p];
+        return;
+    }
+    // 3. walk the merged envelope
+    int cb = (bhi[tid] > blo[tid]) ? c : (lsb[tid] != 255 ? lsb[tid] : rsb[tid]);
+    int ct = cb * W + line;
+    int ci = cb == c ? blo[ct] : (cb < c ? bhi[ct] - 1 : blo[ct]);
+    int cs = stk[ci * NT + ct];
+    int32_t cv = pval(fl, cs, p0);
+    bool moved = false;
+    for (;;) {  // right while not worse
+        int nb = cb, ni = ci + 1;
+        if (ni >= bhi[ct]) {
+            if (cb + 1 >= T) break;
+            nb = rsb[(cb + 1) * W + line];
+            if (nb == 255) break;
+            ni = blo[nb * W + line];
+        }
+        const int nt = nb * W + line;
+        const int ns = stk[ni * NT + nt];
+        const int32_t nv = pval(fl, ns, p0);
+        if (nv > cv) break;
+        cb = nb; ct = nt; ci = ni; cs = ns; cv = nv; moved = true;
+    }
+    if (!moved) {
+        for (;;) {  // left while strictly better
+            int pb = cb, pi = ci - 1;
+            if (pi < blo[ct]) {
+                if (cb == 0) break;
+                pb = lsb[(cb - 1) * W + line];
+                if (pb == 255) break;
+                pi = bhi[pb * W + line] - 1;
             }
+            const int pt = pb * W + line;
+            const int ps = stk[pi * NT + pt];
+            const int32_t pv = pval(fl, ps, p0);
+            if (pv >= cv) break;
+            cb = pb; ct = pt; ci = pi; cs = ps; cv = pv;
         }
     }
-    int32_t worst = 0;
-#pragma unroll
-    for (int i = 0; i < CMAX; ++i)
-        if (p0 + i < p1) worst = max(worst, best[i]);
-
-    bool go_l = true, go_r = true;
-    for (int d = 1; (go_l || go_r); ++d) {
-#pragma unroll
-        for (int side = 0; side < 2; ++side) {
-            const int cc = side ? c + d : c - d;
-            bool& go = side ? go_r : go_l;
-            if (!go) continue;
-            if (cc < 0 || cc >= T) { go = false; continue; }
-            const int q0 = min(cc * C, L), q1 = min(q0 + C, L);
-            if (q0 >= q1) { go = false; continue; }
-            const int gap = side ? q0 - (p1 - 1) : p0 - (q1 - 1);
-            const int64_t g2 = (int64_t)gap * gap;
-            if (g2 >= worst) { go = false; continue; }
-            const int ct = cc * W + line;
-            if ((int64_t)minf[ct] + g2 >= worst) continue;
-            const int n = ksz[ct];
-            int kk = 0;
-#pragma unroll
-            for (int i = 0; i < CMAX; ++i) {
-                const int p = p0 + i;
-                if (p < p1) {
-                    while (kk + 1 < n && pval(fl, stk[(kk + 1) * NT + ct], p) <= pval(fl, stk[kk * NT + ct], p)) ++kk;
-                    best[i] = min(best[i], pval(fl, stk[kk * NT + ct], p));
-                }
+    dst[out] = cv;
+    for (int p = p0 + 1; p < p1; ++p) {
+        out += axis_stride;
+        cv = pval(fl, cs, p);
+        for (;;) {
+            int nb = cb, ni = ci + 1;
+            if (ni >= bhi[ct]) {
+                if (cb + 1 >= T) break;
+                nb = rsb[(cb + 1) * W + line];
+                if (nb == 255) break;
+                ni = blo[nb * W + line];
             }
-            worst = 0;
-#pragma unroll
-            for (int i = 0; i < CMAX; ++i)
-                if (p0 + i < p1) worst = max(worst, best[i]);
+            const int nt = nb * W + line;
+            const int ns = stk[ni * NT + nt];
+            const int32_t nv = pval(fl, ns, p);
+            if (nv > cv) break;
+            cb = nb; ct = nt; ci = ni; cs = ns; cv = nv;
         }
-    }
-#pragma unroll
-    for (int i = 0; i < CMAX; ++i) {
-        const int p = p0 + i;
-        if (p < p1) dst[base + (int64_t)p * axis_stride + line] = best[i];
+        dst[out] = cv;
     }
 }
 
@@ -245,7 +339,7 @@ hipError_t launch_lines_t(const int32_t* src, int32_t* dst, int L, int64_t axis_
     const int ztiles = (nz + W - 1) / W;
     const uint64_t tiles = (uint64_t)ztiles * outer_count * planes;
     const uint32_t grid = round8(tiles);
-    const size_t lds = (size_t)W * (L + 1) * 4 + (size_t)W * T * 8 + (size_t)CMAX * W * T * 2;
+    const size_t lds = (size_t)W * (L + 1) * 4 + (size_t)W * T * 8 + (size_t)CMAX * W * T * 2 + (size_t)W * T * 7 + W;
     if (lds > 65536) {
         hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_edt_lines<W, T, CMAX>),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);

@@ -220,13 +220,83 @@ __device__ int cull_box(const Prim* __restrict__ prims, int nprims, float cx, fl
     return n;
 }
 
+// Wave-level refinement of a primitive list for a smaller box (a group of
+// blocks or one block).  src: the parent list (its first entry is primitive 0)
+// or nullptr for all primitives; out: per-wave LDS array.  Same exact rule as
+// cull_box, scanned 64 candidates at a time.
+__device__ int cull_wave(const Prim* __restrict__ prims, const uint16_t* src, int nsrc, float cx, float cy, float cz,
+                         float rad, float slack, uint16_t* out)
+{
+    const int lane = threadIdx.x & 63;
+    float Flo = 0.0f, Fhi = 0.0f;
+    int n = 0;
+    for (int base = 0; base < nsrc; base += 64) {
+        const int j = base + lane;
+        Clamp2 cl = {-INFINITY, INFINITY, -INFINITY, INFINITY};
+        float lo = 0.0f, hi = 0.0f;
+        int op = 0, idx = 0;
+        if (j < nsrc) {
+            idx = src ? (int)src[j] : j;
+            const Prim p = prims[idx];
+            prim_interval(p, cx, cy, cz, rad, slack, lo, hi);
+            op = p.op;
+            if (j == 0) cl = {lo, lo, hi, hi};
+            else if (op == 0) cl = {-INFINITY, lo, -INFINITY, hi};
+            else if (op == 1) cl = {-hi, INFINITY, -lo, INFINITY};
+            else cl = {lo, INFINITY, hi, INFINITY};
+        }
+        Clamp2 inc = cl;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            Clamp2 o = cshfl_up(inc, d);
+            if (lane >= d) inc = ccompose(o, inc);
+        }
+        Clamp2 exc = cshfl_up(inc, 1);
+        if (lane == 0) exc = {-INFINITY, INFINITY, -INFINITY, INFINITY};
+        const float blo = capply(Flo, exc.ll, exc.lu), bhi = capply(Fhi, exc.hl, exc.hu);
+        bool keep = false;
+        if (j < nsrc) {
+            if (j == 0) keep = true;
+            else if (op == 0) keep = !(lo > bhi);
+            else if (op == 1) keep = !(-lo < blo);
+            else keep = !(hi < blo);
+        }
+        const uint64_t bal = __ballot(keep);
+        if (keep) out[n + __popcll(bal & ((1ull << lane) - 1))] = (uint16_t)idx;
+        n += __popcll(bal);
+        const float all_ll = __shfl(inc.ll, 63), all_lu = __shfl(inc.lu, 63);
+        const float all_hl = __shfl(inc.hl, 63), all_hu = __shfl(inc.hu, 63);
+        Flo = capply(Flo, all_ll, all_lu);
+        Fhi = capply(Fhi, all_hl, all_hu);
+    }
+    return n;
+}
+
+// Box (centre, half diagonal) of blocks [b0, b0+nb) per axis, corners included.
+__device__ __forceinline__ void block_box(const FieldDesc& D, int bi, int bj, int bk, int nb, float& cx, float& cy,
+                                          float& cz, float& rad)
+{
+    const int B = D.B;
+    const float lx = D.x0 + (float)(bi * B) * D.pitch, ux = D.x0 + (float)((bi + nb) * B) * D.pitch;
+    const float ly = D.y0 + (float)(bj * B) * D.pitch, uy = D.y0 + (float)((bj + nb) * B) * D.pitch;
+    const float lz = D.z0 + (float)(bk * B) * D.pitch, uz = D.z0 + (float)((bk + nb) * B) * D.pitch;
+    cx = 0.5f * (lx + ux);
+    cy = 0.5f * (ly + uy);
+    cz = 0.5f * (lz + uz);
+    const float ex = ux - lx, ey = uy - ly, ez = uz - lz;
+    rad = 0.5f * sqrtf(ex * ex + ey * ey + ez * ez) * 1.0001f;
+}
+
 // ---------------------------------------------------------------- classify
-// K = (block,corner) pairs per thread = SB^3*8/256.
-template <int K>
+// One workgroup per superblock: the workgroup culls the recipe for the
+// superblock box (stored for the tile pass), then each wave takes groups of
+// 2x2x2 blocks, refines the list for the group's box and evaluates the
+// group's 64 (block, corner) points, one per lane.
 __global__ __launch_bounds__(256) void k_classify(const FieldDesc* __restrict__ descs, const int32_t* __restrict__ sb_prefix,
                                                   int R, int total, uint32_t grid)
 {
     __shared__ uint16_t list[FE_MAX_PRIMS];
+    __shared__ uint16_t wlist[4][FE_MAX_PRIMS / 2];
     __shared__ Clamp2 wtot[4];
     __shared__ int wcnt[4];
     const uint32_t item = xcd_tile(blockIdx.x, grid);
@@ -237,39 +307,38 @@ __global__ __launch_bounds__(256) void k_classify(const FieldDesc* __restrict__ 
     const int SB = D.SB, B = D.B;
     const int sbz = sb % D.nsbz, sby = (sb / D.nsbz) % D.nsby, sbx = sb / (D.nsbz * D.nsby);
     const int bx0 = sbx * SB, by0 = sby * SB, bz0 = sbz * SB;
-    // superblock box: every corner and voxel of its blocks lies inside
-    const float lx = D.x0 + (float)(bx0 * B) * D.pitch, ux = D.x0 + (float)((bx0 + SB) * B) * D.pitch;
-    const float ly = D.y0 + (float)(by0 * B) * D.pitch, uy = D.y0 + (float)((by0 + SB) * B) * D.pitch;
-    const float lz = D.z0 + (float)(bz0 * B) * D.pitch, uz = D.z0 + (float)((bz0 + SB) * B) * D.pitch;
-    const float cx = 0.5f * (lx + ux), cy = 0.5f * (ly + uy), cz = 0.5f * (lz + uz);
-    const float ex = ux - lx, ey = uy - ly, ez = uz - lz;
-    const float rad = 0.5f * sqrtf(ex * ex + ey * ey + ez * ez) * 1.0001f;
+    float cx, cy, cz, rad;
+    block_box(D, bx0, by0, bz0, SB, cx, cy, cz, rad);
     const int n = cull_box(D.prims, D.nprims, cx, cy, cz, rad, D.slack, list, wtot, wcnt);
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     if (tid == 0) D.list_len[sb] = n <= D.cap ? n : -1;
     if (n <= D.cap)
         for (int j = tid; j < n; j += 256) D.lists[(int64_t)sb * D.cap + j] = list[j];
 
-    float x[K], y[K], z[K], f[K];
-    int blk[K];
+    const int G = SB / 2, ngroups = G * G * G;
     const float side = (float)B * D.pitch;
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-        const int q = tid + k * 256, bl = q >> 3, c = q & 7;
-        const int bi = bx0 + bl / (SB * SB), bj = by0 + (bl / SB) % SB, bk = bz0 + bl % SB;
-        blk[k] = (bi < D.nbx && bj < D.nby && bk < D.nbz) ? (bi * D.nby + bj) * D.nbz + bk : -1;
+    uint16_t* wl = wlist[wid];
+    for (int g = wid; g < ngroups; g += 4) {
+        const int gx = bx0 + 2 * (g / (G * G)), gy = by0 + 2 * ((g / G) % G), gz = bz0 + 2 * (g % G);
+        block_box(D, gx, gy, gz, 2, cx, cy, cz, rad);
+        // a group list never exceeds the superblock list; fall back to it if it
+        // would not fit the per-wave buffer
+        const uint16_t* gl = list;
+        int gn = n;
+        if (n <= FE_MAX_PRIMS / 2) {
+            gn = cull_wave(D.prims, list, n, cx, cy, cz, rad, D.slack, wl);
+            gl = wl;
+        }
+        const int bl = lane >> 3, c = lane & 7;
+        const int bi = gx + ((bl >> 2) & 1), bj = gy + ((bl >> 1) & 1), bk = gz + (bl & 1);
         const float ox = D.x0 + (float)(bi * B) * D.pitch;
         const float oy = D.y0 + (float)(bj * B) * D.pitch;
         const float oz = D.z0 + (float)(bk * B) * D.pitch;
-        x[k] = ox + (float)(c & 1) * side;
-        y[k] = oy + (float)((c >> 1) & 1) * side;
-        z[k] = oz + (float)((c >> 2) & 1) * side;
-    }
-    eval_points<K>(D.prims, list, n, x, y, z, f);
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-        float mn = fabsf(f[k]);
-        int pos = f[k] >= 0.0f, neg = !pos;
+        float x = ox + (float)(c & 1) * side, y = oy + (float)((c >> 1) & 1) * side, z = oz + (float)((c >> 2) & 1) * side;
+        float v;
+        eval_points<1>(D.prims, gl, gn, &x, &y, &z, &v);
+        float mn = fabsf(v);
+        int pos = v >= 0.0f, neg = !pos;
 #pragma unroll
         for (int s = 1; s < 8; s <<= 1) {
             mn = sel_min(mn, __shfl_xor(mn, s));
@@ -277,9 +346,10 @@ __global__ __launch_bounds__(256) void k_classify(const FieldDesc* __restrict__ 
             neg |= __shfl_xor(neg, s);
         }
         mn = sel_min(1.0e30f, mn);
-        if ((tid & 7) == 0 && blk[k] >= 0) {
-            D.cma[blk[k]] = mn;
-            D.kind[blk[k]] = (pos && neg) ? 0 : (mn <= D.margin ? 0 : (neg ? -1 : 1));
+        if (c == 0 && bi < D.nbx && bj < D.nby && bk < D.nbz) {
+            const int64_t b = ((int64_t)bi * D.nby + bj) * D.nbz + bk;
+            D.cma[b] = mn;
+            D.kind[b] = (pos && neg) ? 0 : (mn <= D.margin ? 0 : (neg ? -1 : 1));
         }
     }
 }
@@ -349,67 +419,35 @@ __global__ __launch_bounds__(256) void k_compact_write(const FieldDesc* __restri
 }
 
 // ---------------------------------------------------------------- tiles
-// Persistent: every wave walks all recipes' active blocks with a global
-// stride; one wave evaluates one B^3 tile, KV voxels per lane per pass.
+// One wave evaluates one B^3 tile: it refines the superblock list for the
+// block's own box (exact, as above) and evaluates KV voxels per lane per pass.
+// `lin` is the block whose voxels are evaluated (the planted fault shifts it,
+// and then evaluates the whole recipe), `bl` the listed block.
 template <int KV>
-__global__ __launch_bounds__(256) void k_tiles(const FieldDesc* __restrict__ descs, int R)
+__device__ __forceinline__ void tile_eval(const FieldDesc& D, int64_t a, int bl, uint16_t* wl)
 {
     const int lane = threadIdx.x & 63;
-    const int64_t wave = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    const int64_t nwaves = (int64_t)gridDim.x * 4;
-    for (int r = 0; r < R; ++r) {
-        const FieldDesc& D = descs[r];
-        const int64_t na = min((int64_t)D.counts[0], D.tile_cap);
-        const int B = D.B, b3 = B * B * B;
-        for (int64_t a = wave; a < na; a += nwaves) {
-            int lin = D.ids[a] + (D.corrupt ? 1 : 0);
-            const int bk = lin % D.nbz, bj = (lin / D.nbz) % D.nby, bi = lin / (D.nbz * D.nby);
-            const int bl = D.ids[a];
-            const int sb = ((bl / (D.nbz * D.nby)) / D.SB * D.nsby + ((bl / D.nbz) % D.nby) / D.SB) * D.nsbz +
-                           (bl % D.nbz) / D.SB;
-            // the planted fault evaluates a shifted block, so it skips the culled list
-            const int nl = D.corrupt ? -1 : D.list_len[sb];
-            const uint16_t* list = nl >= 0 ? D.lists + (int64_t)sb * D.cap : nullptr;
-            const int n = nl >= 0 ? nl : D.nprims;
-            float* out = D.tiles + a * b3;
-            for (int v0 = 0; v0 < b3; v0 += 64 * KV) {
-                float x[KV], y[KV], z[KV], f[KV];
-#pragma unroll
-                for (int k = 0; k < KV; ++k) {
-                    const int l = min(v0 + k * 64 + lane, b3 - 1);
-                    const int li = l / (B * B), lj = (l / B) % B, lk = l % B;
-                    x[k] = D.x0 + (float)(bi * B + li) * D.pitch;
-                    y[k] = D.y0 + (float)(bj * B + lj) * D.pitch;
-                    z[k] = D.z0 + (float)(bk * B + lk) * D.pitch;
-                }
-                eval_points<KV>(D.prims, list, n, x, y, z, f);
-#pragma unroll
-                for (int k = 0; k < KV; ++k) {
-                    const int l = v0 + k * 64 + lane;
-                    if (l < b3) out[l] = f[k];
-                }
-            }
+    const int B = D.B, b3 = B * B * B;
+    const int lin = bl + (D.corrupt ? 1 : 0);
+    const int bk = lin % D.nbz, bj = (lin / D.nbz) % D.nby, bi = lin / (D.nbz * D.nby);
+    const uint16_t* list = nullptr;
+    int n = D.nprims;
+    if (!D.corrupt) {
+        const int sb = ((bl / (D.nbz * D.nby)) / D.SB * D.nsby + ((bl / D.nbz) % D.nby) / D.SB) * D.nsbz +
+                       (bl % D.nbz) / D.SB;
+        const int nl = D.list_len[sb];
+        const uint16_t* src = nl >= 0 ? D.lists + (int64_t)sb * D.cap : nullptr;
+        const int nsrc = nl >= 0 ? nl : D.nprims;
+        if (nsrc <= FE_MAX_PRIMS / 2) {
+            float cx, cy, cz, rad;
+            block_box(D, bi, bj, bk, 1, cx, cy, cz, rad);
+            n = cull_wave(D.prims, src, nsrc, cx, cy, cz, rad, D.slack, wl);
+            list = wl;
+        } else {
+            list = src;
+            n = nsrc;
         }
     }
-}
-
-// Tiles for an explicit host-provided id list (the plain per-call path); uses
-// the culled lists that the classify launch stored.
-template <int KV>
-__global__ __launch_bounds__(256) void k_tiles_ids(FieldDesc D, const int32_t* __restrict__ ids, int64_t na)
-{
-    const int lane = threadIdx.x & 63;
-    const int64_t a = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (a >= na) return;
-    const int B = D.B, b3 = B * B * B;
-    const int lin = ids[a] + (D.corrupt ? 1 : 0);
-    const int bk = lin % D.nbz, bj = (lin / D.nbz) % D.nby, bi = lin / (D.nbz * D.nby);
-    const int bl = ids[a];
-    const int sb = ((bl / (D.nbz * D.nby)) / D.SB * D.nsby + ((bl / D.nbz) % D.nby) / D.SB) * D.nsbz +
-                   (bl % D.nbz) / D.SB;
-    const int nl = D.corrupt ? -1 : D.list_len[sb];
-    const uint16_t* list = nl >= 0 ? D.lists + (int64_t)sb * D.cap : nullptr;
-    const int n = nl >= 0 ? nl : D.nprims;
     float* out = D.tiles + a * b3;
     for (int v0 = 0; v0 < b3; v0 += 64 * KV) {
         float x[KV], y[KV], z[KV], f[KV];
@@ -430,26 +468,73 @@ __global__ __launch_bounds__(256) void k_tiles_ids(FieldDesc D, const int32_t* _
     }
 }
 
+// Persistent: every wave walks all recipes' active blocks with a global stride.
+template <int KV>
+__global__ __launch_bounds__(256) void k_tiles(const FieldDesc* __restrict__ descs, int R)
+{
+    __shared__ uint16_t wlist[4][FE_MAX_PRIMS / 2];
+    const int64_t wave = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int64_t nwaves = (int64_t)gridDim.x * 4;
+    for (int r = 0; r < R; ++r) {
+        const FieldDesc& D = descs[r];
+        const int64_t na = min((int64_t)D.counts[0], D.tile_cap);
+        for (int64_t a = wave; a < na; a += nwaves) tile_eval<KV>(D, a, D.ids[a], wlist[threadIdx.x >> 6]);
+    }
+}
+
+// Tiles for an explicit host-provided id list (the plain per-call path); uses
+// the culled lists that the classify launch stored.
+template <int KV>
+__global__ __launch_bounds__(256) void k_tiles_ids(FieldDesc D, const int32_t* __restrict__ ids, int64_t na)
+{
+    __shared__ uint16_t wlist[4][FE_MAX_PRIMS / 2];
+    const int64_t a = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (a >= na) return;
+    tile_eval<KV>(D, a, ids[a], wlist[threadIdx.x >> 6]);
+}
+
 // ---------------------------------------------------------------- reconstruct / checksum
 // Cropped dense field (FE/faltkarna_v1.py:302-325) and, when occ != nullptr,
-// the occupancy dense < 0 (FE/faltkarna_v1_mesh_to_sdf.py:695-710).
+// the occupancy dense < 0 (FE/faltkarna_v1_mesh_to_sdf.py:695-710).  One
+// thread writes 4 consecutive z (one block, since B is a multiple of 4) with
+// one 16-byte store (4-byte for the occupancy) when rows are 4-aligned.
 __global__ __launch_bounds__(256) void k_reconstruct(const FieldDesc* __restrict__ descs, int r, float* __restrict__ dense,
                                                      uint8_t* __restrict__ occ)
 {
     const FieldDesc& D = descs[r];
-    const int64_t n = (int64_t)D.nx * D.ny * D.nz;
+    const int nz4 = (D.nz + 3) >> 2;
+    const int64_t n4 = (int64_t)D.nx * D.ny * nz4;
     const int B = D.B;
-    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
-        const int z = (int)(i % D.nz);
-        const int64_t xy = i / D.nz;
+    const bool vec = (D.nz & 3) == 0;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n4; i += (int64_t)gridDim.x * 256) {
+        const int z0 = (int)(i % nz4) * 4;
+        const int64_t xy = i / nz4;
         const int y = (int)(xy % D.ny), x = (int)(xy / D.ny);
-        const int64_t b = ((int64_t)(x / B) * D.nby + y / B) * D.nbz + z / B;
+        const int64_t b = ((int64_t)(x / B) * D.nby + y / B) * D.nbz + z0 / B;
         const int s = D.slot[b];
-        float v;
-        if (s >= 0 && s < D.tile_cap) v = D.tiles[(int64_t)s * B * B * B + ((x % B) * B + (y % B)) * B + (z % B)];
-        else v = D.kind[b] < 0 ? -FE_BIG : FE_BIG;
-        dense[i] = v;
-        if (occ) occ[i] = v < 0.0f ? 1 : 0;
+        float v[4];
+        if (s >= 0 && s < D.tile_cap) {
+            const float* t = D.tiles + (int64_t)s * B * B * B + ((x % B) * B + (y % B)) * B + (z0 % B);
+            const float4 q = *reinterpret_cast<const float4*>(t);
+            v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
+        } else {
+            const float bg = D.kind[b] < 0 ? -FE_BIG : FE_BIG;
+            v[0] = v[1] = v[2] = v[3] = bg;
+        }
+        const int64_t o = xy * D.nz + z0;
+        if (vec) {
+            *reinterpret_cast<float4*>(dense + o) = make_float4(v[0], v[1], v[2], v[3]);
+            if (occ) {
+                const uint32_t m = (v[0] < 0.0f ? 1u : 0u) | (v[1] < 0.0f ? 1u << 8 : 0u) |
+                                   (v[2] < 0.0f ? 1u << 16 : 0u) | (v[3] < 0.0f ? 1u << 24 : 0u);
+                *reinterpret_cast<uint32_t*>(occ + o) = m;
+            }
+        } else {
+            for (int k = 0; k < 4 && z0 + k < D.nz; ++k) {
+                dense[o + k] = v[k];
+                if (occ) occ[o + k] = v[k] < 0.0f ? 1 : 0;
+            }
+        }
     }
 }
 
@@ -506,8 +591,6 @@ __global__ __launch_bounds__(256) void k_tiles_dense(const float* __restrict__ d
 }
 
 // ---------------------------------------------------------------- C ABI
-static int classify_k(int B) { return B == 4 ? 16 : 2; }
-
 // Launches classify -> compaction -> tiles for R recipes described by the
 // device descriptor table.  sb_prefix / ch_prefix: device int32[R+1]
 // exclusive prefixes of superblock and 1024-block chunk counts; chunk_count:
@@ -521,10 +604,7 @@ extern "C" int fe_field_build(const void* descs, int R, int B, const int32_t* sb
     hipStream_t st = static_cast<hipStream_t>(stream);
     const FieldDesc* d = static_cast<const FieldDesc*>(descs);
     const uint32_t g = round8(total_sb);
-    if (classify_k(B) == 16)
-        hipLaunchKernelGGL(k_classify<16>, dim3(g), dim3(256), 0, st, d, sb_prefix, R, total_sb, g);
-    else
-        hipLaunchKernelGGL(k_classify<2>, dim3(g), dim3(256), 0, st, d, sb_prefix, R, total_sb, g);
+    hipLaunchKernelGGL(k_classify, dim3(g), dim3(256), 0, st, d, sb_prefix, R, total_sb, g);
     FE_TRY(hipGetLastError());
     hipLaunchKernelGGL(k_compact_count, dim3(total_chunks), dim3(256), 0, st, d, ch_prefix, R, total_chunks, chunk_count);
     FE_TRY(hipGetLastError());
@@ -545,10 +625,7 @@ extern "C" int fe_field_classify(const void* descs, int R, int B, const int32_t*
     hipStream_t st = static_cast<hipStream_t>(stream);
     const FieldDesc* d = static_cast<const FieldDesc*>(descs);
     const uint32_t g = round8(total_sb);
-    if (classify_k(B) == 16)
-        hipLaunchKernelGGL(k_classify<16>, dim3(g), dim3(256), 0, st, d, sb_prefix, R, total_sb, g);
-    else
-        hipLaunchKernelGGL(k_classify<2>, dim3(g), dim3(256), 0, st, d, sb_prefix, R, total_sb, g);
+    hipLaunchKernelGGL(k_classify, dim3(g), dim3(256), 0, st, d, sb_prefix, R, total_sb, g);
     FE_TRY(hipGetLastError());
     return FE_OK;
 }
@@ -575,7 +652,7 @@ extern "C" int fe_field_reconstruct(const void* descs, int r, int64_t n_voxels, 
 {
     if (!descs || r < 0 || !dense || n_voxels < 1) return FE_INVALID;
     hipStream_t st = static_cast<hipStream_t>(stream);
-    const unsigned blocks = (unsigned)std::min<int64_t>((n_voxels + 255) / 256, 65536);
+    const unsigned blocks = (unsigned)std::min<int64_t>((n_voxels / 4 + 255) / 256 + 1, 65536);
     hipLaunchKernelGGL(k_reconstruct, dim3(blocks), dim3(256), 0, st, static_cast<const FieldDesc*>(descs), r, dense, occ);
     FE_TRY(hipGetLastError());
     return FE_OK;

@@ -129,7 +129,9 @@ class FieldPlan:
         lcap = [min(len(r), LIST_CAP) for _, r in items]
         # pooled allocations: int32, float32, uint16, prims
         i32_sizes = [3 * m.n_blocks + nsb[i] + 4 for i, (m, _) in enumerate(items)]
-        f32_sizes = [m.n_blocks + caps[i] * b3 for i, (m, _) in enumerate(items)]
+        # tiles start 256-byte aligned (the reconstruction reads them as float4)
+        cma_len = [-(-m.n_blocks // 64) * 64 for m, _ in items]
+        f32_sizes = [cma_len[i] + caps[i] * b3 for i in range(n)]
         u16_sizes = [nsb[i] * lcap[i] for i in range(n)]
         prim_counts = [len(r) for _, r in items]
         self.i32 = torch.empty(sum(i32_sizes), dtype=torch.int32, device=self.device)
@@ -153,7 +155,7 @@ class FieldPlan:
             d["corrupt"] = 1 if corrupt else 0
             nb = meta.n_blocks
             v = dict(kind=(oi, nb), ids=(oi + nb, nb), slot=(oi + 2 * nb, nb), list_len=(oi + 3 * nb, nsb[i]),
-                     counts=(oi + 3 * nb + nsb[i], 4), cma=(of, nb), tiles=(of + nb, caps[i] * b3),
+                     counts=(oi + 3 * nb + nsb[i], 4), cma=(of, nb), tiles=(of + cma_len[i], caps[i] * b3),
                      lists=(ou, nsb[i] * lcap[i]))
             d["prims"] = bpr + op * 32
             for k in ("kind", "ids", "slot", "list_len", "counts"):
