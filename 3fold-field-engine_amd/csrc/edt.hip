@@ -108,14 +108,17 @@ __device__ __forceinline__ bool keep_mid(const int32_t* f, int a, int b, int c)
 
 // One exact lower-envelope pass over W lines x L positions.  Thread (c, line)
 // owns chunk c (C positions) of a line.
-//  1. every thread builds the envelope stack of its chunk's sites;
-//  2. one thread per line merges the chunk stacks left to right: at each
-//     chunk boundary it pops the tail of the running envelope and the head of
-//     the next stack until the bridge between them is found (as when merging
-//     the lower hulls of x-separated point sets (s, f(s)+s^2)); a line whose
-//     values are all equal is flagged and copied through unchanged;
+//  1. every thread builds the lower hull of the points (s, f(s)+s^2) of its
+//     chunk (the parabola envelope), keeping the two top entries in registers;
+//  2. the chunk hulls of a line are merged in log2(T) rounds: in round h the
+//     thread at the boundary of two adjacent groups pops the tail of the left
+//     group and the head of the right group until the bridge between them is
+//     found (merging lower hulls of x-separated point sets);
 //  3. every thread walks the merged envelope over its positions (the minimum
-//     is unimodal along the envelope and its site moves right with p).
+//     is unimodal along the envelope and its site moves right with p);
+//     results go back through LDS so that loads and stores are 16-byte vectors
+//     along the contiguous z axis.
+// A line whose values are all equal is its own result (g = f).
 template <int W, int T, int CMAX>
 __global__ __launch_bounds__(W* T) void k_edt_lines(const int32_t* __restrict__ src, int32_t* __restrict__ dst, int L,
                                                     int64_t axis_stride, int nz, int ztiles, int64_t outer_stride,
@@ -132,21 +135,33 @@ __global__ __launch_bounds__(W* T) void k_edt_lines(const int32_t* __restrict__ 
     const int nlines = min(W, nz - zt * W);
     const int LP = L + 1;
     constexpr int NT = W * T;
-    int32_t* f = lds;                                   // [W][LP]
-    int32_t* cmin = f + W * LP;                         // [NT] chunk minimum
-    int32_t* cmax = cmin + NT;                          // [NT] chunk maximum
-    uint16_t* stk = reinterpret_cast<uint16_t*>(cmax + NT);  // [CMAX][NT]
-    uint16_t* blo = stk + CMAX * NT;                    // [NT] surviving range of a chunk stack
-    uint16_t* bhi = blo + NT;                           // [NT]
-    uint8_t* lsb = reinterpret_cast<uint8_t*>(bhi + NT);  // [NT] nearest nonempty chunk <= c
-    uint8_t* rsb = lsb + NT;                            // [NT] nearest nonempty chunk >= c
-    uint8_t* bsl = rsb + NT;                            // [W][T] merge stack of chunk ids
-    uint8_t* uni = bsl + NT;                            // [W] constant-line flag
+    int32_t* f = lds;                                          // [W][LP]
+    uint16_t* stk = reinterpret_cast<uint16_t*>(f + W * LP);  // [CMAX][NT]
+    uint16_t* blo = stk + CMAX * NT;                           // [NT] surviving range of a chunk hull
+    uint16_t* bhi = blo + NT;                                  // [NT]
+    uint8_t* lsb = reinterpret_cast<uint8_t*>(bhi + NT);       // [NT] nearest nonempty chunk <= c
+    uint8_t* rsb = lsb + NT;                                   // [NT] nearest nonempty chunk >= c
+    int32_t* lval = reinterpret_cast<int32_t*>(rsb + NT);      // [W] value of chunk 0's first sample
+    int32_t* lflat = lval + W;                                 // [W] line is constant
     const int tid = threadIdx.x;
+    const bool vec = (nz & 3) == 0;  // then rows of the tile are 16-byte aligned
 
-    for (int i = tid; i < W * L; i += NT) {
-        int line = i % W, q = i / W;
-        if (line < nlines) f[line * LP + q] = src[base + (int64_t)q * axis_stride + line];
+    if (vec) {
+        for (int i = tid; i < (W / 4) * L; i += NT) {
+            const int l0 = (i % (W / 4)) * 4, q = i / (W / 4);
+            if (l0 < nlines) {
+                const int4 v = *reinterpret_cast<const int4*>(src + base + (int64_t)q * axis_stride + l0);
+                f[l0 * LP + q] = v.x;
+                f[(l0 + 1) * LP + q] = v.y;
+                f[(l0 + 2) * LP + q] = v.z;
+                f[(l0 + 3) * LP + q] = v.w;
+            }
+        }
+    } else {
+        for (int i = tid; i < W * L; i += NT) {
+            const int line = i % W, q = i / W;
+            if (line < nlines) f[line * LP + q] = src[base + (int64_t)q * axis_stride + line];
+        }
     }
     __syncthreads();
 
@@ -156,151 +171,201 @@ __global__ __launch_bounds__(W* T) void k_edt_lines(const int32_t* __restrict__ 
     const bool active = line < nlines;
     const int32_t* fl = f + line * LP;
 
-    // 1. envelope stack of this chunk
-    int k = -1;
+    // 1. hull of this chunk
+    int k = -1, s_top = 0, s_sec = 0;
+    int64_t F_top = 0, F_sec = 0;
     int32_t mn = INT_MAX, mx = INT_MIN;
     if (active) {
         for (int q = p0; q < p1; ++q) {
             const int32_t fq = fl[q];
             mn = min(mn, fq);
             mx = max(mx, fq);
-            while (k >= 1 && !keep_mid(fl, stk[(k - 1) * NT + tid], stk[k * NT + tid], q)) --k;
-            stk[++k * NT + tid] = (uint16_t)q;
+            const int64_t Fq = (int64_t)fq + (int64_t)q * q;
+            while (k >= 1) {
+                // top stays iff it is strictly lowest somewhere between sec and q
+                if ((Fq - F_top) * (int64_t)(s_top - s_sec) > (F_top - F_sec) * (int64_t)(q - s_top)) break;
+                --k;
+                s_top = s_sec;
+                F_top = F_sec;
+                if (k >= 1) {
+                    s_sec = stk[(k - 1) * NT + tid];
+                    F_sec = (int64_t)fl[s_sec] + (int64_t)s_sec * s_sec;
+                }
+            }
+            ++k;
+            stk[k * NT + tid] = (uint16_t)q;
+            s_sec = s_top;
+            F_sec = F_top;
+            s_top = q;
+            F_top = Fq;
         }
     }
-    cmin[tid] = mn;
-    cmax[tid] = mx;
     blo[tid] = 0;
     bhi[tid] = (uint16_t)(k + 1);
+    if (c == 0) {
+        lval[line] = mn;
+        lflat[line] = 1;
+    }
     __syncthreads();
+    if (active && p0 < p1 && !(mn == mx && mn == lval[line])) lflat[line] = 0;
 
-    // 2. merge the chunk stacks of each line (one thread per line)
-    if (tid < W && tid < nlines) {
-        const int ln = tid;
-        const int32_t* fm = f + ln * LP;
-        const int32_t v0 = cmin[ln];
-        bool flat = true;
-        for (int b = 0; b < T; ++b) {
-            const int t = b * W + ln;
-            if (bhi[t] == 0) continue;
-            flat = flat && cmin[t] == v0 && cmax[t] == v0;
-        }
-        uni[ln] = flat;
-        if (!flat) {
-            uint8_t* bs = bsl + ln * T;
-            int nbs = 0;
-            for (int b = 0; b < T; ++b) {
-                const int t = b * W + ln;
-                int lo = 0;
-                const int hi = bhi[t];
-                if (hi == 0) continue;
-                bool changed = nbs > 0;
+    // 2. tree merge of the chunk hulls
+    for (int h = 1; h < T; h <<= 1) {
+        if (active && (c & (2 * h - 1)) == h) {
+            const int a0 = c - h, a2 = min(c + h, T);
+            int ta = c - 1;
+            while (ta >= a0 && bhi[ta * W + line] == blo[ta * W + line]) --ta;
+            int hb = c;
+            while (hb < a2 && bhi[hb * W + line] == blo[hb * W + line]) ++hb;
+            if (ta >= a0 && hb < a2) {
+                bool changed = true;
                 while (changed) {
                     changed = false;
-                    const int s3 = stk[lo * NT + t];
-                    while (nbs > 0) {  // tail of the running envelope
-                        const int gt = bs[nbs - 1] * W + ln;
-                        const int gh = bhi[gt], gl = blo[gt];
-                        const int s1 = stk[(gh - 1) * NT + gt];
-                        int s2;
-                        if (gh - gl >= 2) s2 = stk[(gh - 2) * NT + gt];
-                        else if (nbs >= 2) {
-                            const int pt = bs[nbs - 2] * W + ln;
-                            s2 = stk[(bhi[pt] - 1) * NT + pt];
-                        } else break;
-                        if (keep_mid(fm, s2, s1, s3)) break;
-                        bhi[gt] = (uint16_t)(gh - 1);
-                        if (gh - 1 == gl) --nbs;
+                    for (;;) {  // tail of the left group
+                        const int tt = ta * W + line;
+                        const int th = bhi[tt];
+                        const int s1 = stk[(th - 1) * NT + tt];
+                        int s2, pa = ta;
+                        if (th - blo[tt] >= 2) s2 = stk[(th - 2) * NT + tt];
+                        else {
+                            pa = ta - 1;
+                            while (pa >= a0 && bhi[pa * W + line] == blo[pa * W + line]) --pa;
+                            if (pa < a0) break;
+                            s2 = stk[(bhi[pa * W + line] - 1) * NT + pa * W + line];
+                        }
+                        const int hbt = hb * W + line;
+                        const int s3 = stk[blo[hbt] * NT + hbt];
+                        if (keep_mid(fl, s2, s1, s3)) break;
+                        bhi[tt] = (uint16_t)(th - 1);
+                        if (th - 1 == blo[tt]) ta = pa;
                     }
-                    while (hi - lo >= 2 && nbs > 0) {  // head of the next stack
-                        const int gt = bs[nbs - 1] * W + ln;
-                        const int s1 = stk[(bhi[gt] - 1) * NT + gt];
-                        if (keep_mid(fm, s1, stk[lo * NT + t], stk[(lo + 1) * NT + t])) break;
-                        ++lo;
+                    for (;;) {  // head of the right group
+                        const int ht = hb * W + line;
+                        const int hl = blo[ht];
+                        const int s1 = stk[hl * NT + ht];
+                        int s2, nb = hb;
+                        if (bhi[ht] - hl >= 2) s2 = stk[(hl + 1) * NT + ht];
+                        else {
+                            nb = hb + 1;
+                            while (nb < a2 && bhi[nb * W + line] == blo[nb * W + line]) ++nb;
+                            if (nb >= a2) break;
+                            s2 = stk[blo[nb * W + line] * NT + nb * W + line];
+                        }
+                        const int tt = ta * W + line;
+                        const int s0 = stk[(bhi[tt] - 1) * NT + tt];
+                        if (keep_mid(fl, s0, s1, s2)) break;
+                        blo[ht] = (uint16_t)(hl + 1);
                         changed = true;
+                        if (hl + 1 == bhi[ht]) hb = nb;
                     }
                 }
-                blo[t] = (uint16_t)lo;
-                bs[nbs++] = (uint8_t)b;
             }
-            int last = 255;
-            for (int b = 0; b < T; ++b) {
-                const int t = b * W + ln;
-                if (bhi[t] > blo[t]) last = b;
-                lsb[t] = (uint8_t)last;
+        }
+        __syncthreads();
+    }
+    {
+        int b = c;
+        while (b >= 0 && bhi[b * W + line] == blo[b * W + line]) --b;
+        lsb[tid] = (uint8_t)(b < 0 ? 255 : b);
+        b = c;
+        while (b < T && bhi[b * W + line] == blo[b * W + line]) ++b;
+        rsb[tid] = (uint8_t)(b >= T ? 255 : b);
+    }
+    __syncthreads();
+
+    // 3. walk the merged envelope
+    int32_t out[CMAX];
+    if (active && p0 < p1) {
+        if (lflat[line]) {
+#pragma unroll
+            for (int i = 0; i < CMAX; ++i)
+                if (p0 + i < p1) out[i] = fl[p0 + i];
+        } else {
+            int cb = (bhi[tid] > blo[tid]) ? c : (lsb[tid] != 255 ? lsb[tid] : rsb[tid]);
+            int ct = cb * W + line;
+            int ci = cb == c ? blo[ct] : (cb < c ? bhi[ct] - 1 : blo[ct]);
+            int cs = stk[ci * NT + ct];
+            int32_t cf = fl[cs];
+            {
+                int32_t cv = cf + (p0 - cs) * (p0 - cs);
+                bool moved = false;
+                for (;;) {  // right while not worse
+                    int nb = cb, ni = ci + 1;
+                    if (ni >= bhi[ct]) {
+                        if (cb + 1 >= T) break;
+                        nb = rsb[(cb + 1) * W + line];
+                        if (nb == 255) break;
+                        ni = blo[nb * W + line];
+                    }
+                    const int nt = nb * W + line;
+                    const int ns = stk[ni * NT + nt];
+                    const int32_t nf = fl[ns];
+                    const int32_t nv = nf + (p0 - ns) * (p0 - ns);
+                    if (nv > cv) break;
+                    cb = nb; ct = nt; ci = ni; cs = ns; cf = nf; cv = nv; moved = true;
+                }
+                if (!moved) {
+                    for (;;) {  // left while strictly better
+                        int pb = cb, pi = ci - 1;
+                        if (pi < blo[ct]) {
+                            if (cb == 0) break;
+                            pb = lsb[(cb - 1) * W + line];
+                            if (pb == 255) break;
+                            pi = bhi[pb * W + line] - 1;
+                        }
+                        const int pt = pb * W + line;
+                        const int ps = stk[pi * NT + pt];
+                        const int32_t pf = fl[ps];
+                        const int32_t pv = pf + (p0 - ps) * (p0 - ps);
+                        if (pv >= cv) break;
+                        cb = pb; ct = pt; ci = pi; cs = ps; cf = pf; cv = pv;
+                    }
+                }
             }
-            int nxt = 255;
-            for (int b = T - 1; b >= 0; --b) {
-                const int t = b * W + ln;
-                if (bhi[t] > blo[t]) nxt = b;
-                rsb[t] = (uint8_t)nxt;
+#pragma unroll
+            for (int i = 0; i < CMAX; ++i) {
+                const int p = p0 + i;
+                if (p < p1) {
+                    int32_t cv = cf + (p - cs) * (p - cs);
+                    for (;;) {
+                        int nb = cb, ni = ci + 1;
+                        if (ni >= bhi[ct]) {
+                            if (cb + 1 >= T) break;
+                            nb = rsb[(cb + 1) * W + line];
+                            if (nb == 255) break;
+                            ni = blo[nb * W + line];
+                        }
+                        const int nt = nb * W + line;
+                        const int ns = stk[ni * NT + nt];
+                        const int32_t nf = fl[ns];
+                        const int32_t nv = nf + (p - ns) * (p - ns);
+                        if (nv > cv) break;
+                        cb = nb; ct = nt; ci = ni; cs = ns; cf = nf; cv = nv;
+                    }
+                    out[i] = cv;
+                }
             }
         }
     }
     __syncthreads();
-    if (!active || p0 >= p1) return;
-
-    int64_t out = base + (int64_t)p0 * axis_stride + line;
-    if (uni[line]) {
-        for (int p = p0; p < p1; ++p, out += axis_stride) dst[out] = fl[p];
-        return;
+    if (active) {
+#pragma unroll
+        for (int i = 0; i < CMAX; ++i)
+            if (p0 + i < p1) f[line * LP + p0 + i] = out[i];
     }
-    // 3. walk the merged envelope
-    int cb = (bhi[tid] > blo[tid]) ? c : (lsb[tid] != 255 ? lsb[tid] : rsb[tid]);
-    int ct = cb * W + line;
-    int ci = cb == c ? blo[ct] : (cb < c ? bhi[ct] - 1 : blo[ct]);
-    int cs = stk[ci * NT + ct];
-    int32_t cv = pval(fl, cs, p0);
-    bool moved = false;
-    for (;;) {  // right while not worse
-        int nb = cb, ni = ci + 1;
-        if (ni >= bhi[ct]) {
-            if (cb + 1 >= T) break;
-            nb = rsb[(cb + 1) * W + line];
-            if (nb == 255) break;
-            ni = blo[nb * W + line];
+    __syncthreads();
+    if (vec) {
+        for (int i = tid; i < (W / 4) * L; i += NT) {
+            const int l0 = (i % (W / 4)) * 4, q = i / (W / 4);
+            if (l0 < nlines)
+                *reinterpret_cast<int4*>(dst + base + (int64_t)q * axis_stride + l0) =
+                    make_int4(f[l0 * LP + q], f[(l0 + 1) * LP + q], f[(l0 + 2) * LP + q], f[(l0 + 3) * LP + q]);
         }
-        const int nt = nb * W + line;
-        const int ns = stk[ni * NT + nt];
-        const int32_t nv = pval(fl, ns, p0);
-        if (nv > cv) break;
-        cb = nb; ct = nt; ci = ni; cs = ns; cv = nv; moved = true;
-    }
-    if (!moved) {
-        for (;;) {  // left while strictly better
-            int pb = cb, pi = ci - 1;
-            if (pi < blo[ct]) {
-                if (cb == 0) break;
-                pb = lsb[(cb - 1) * W + line];
-                if (pb == 255) break;
-                pi = bhi[pb * W + line] - 1;
-            }
-            const int pt = pb * W + line;
-            const int ps = stk[pi * NT + pt];
-            const int32_t pv = pval(fl, ps, p0);
-            if (pv >= cv) break;
-            cb = pb; ct = pt; ci = pi; cs = ps; cv = pv;
+    } else {
+        for (int i = tid; i < W * L; i += NT) {
+            const int ln = i % W, q = i / W;
+            if (ln < nlines) dst[base + (int64_t)q * axis_stride + ln] = f[ln * LP + q];
         }
-    }
-    dst[out] = cv;
-    for (int p = p0 + 1; p < p1; ++p) {
-        out += axis_stride;
-        cv = pval(fl, cs, p);
-        for (;;) {
-            int nb = cb, ni = ci + 1;
-            if (ni >= bhi[ct]) {
-                if (cb + 1 >= T) break;
-                nb = rsb[(cb + 1) * W + line];
-                if (nb == 255) break;
-                ni = blo[nb * W + line];
-            }
-            const int nt = nb * W + line;
-            const int ns = stk[ni * NT + nt];
-            const int32_t nv = pval(fl, ns, p);
-            if (nv > cv) break;
-            cb = nb; ct = nt; ci = ni; cs = ns; cv = nv;
-        }
-        dst[out] = cv;
     }
 }
 
@@ -339,7 +404,7 @@ hipError_t launch_lines_t(const int32_t* src, int32_t* dst, int L, int64_t axis_
     const int ztiles = (nz + W - 1) / W;
     const uint64_t tiles = (uint64_t)ztiles * outer_count * planes;
     const uint32_t grid = round8(tiles);
-    const size_t lds = (size_t)W * (L + 1) * 4 + (size_t)W * T * 8 + (size_t)CMAX * W * T * 2 + (size_t)W * T * 7 + W;
+    const size_t lds = (size_t)W * (L + 1) * 4 + (size_t)CMAX * W * T * 2 + (size_t)W * T * 6 + (size_t)W * 8;
     if (lds > 65536) {
         hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_edt_lines<W, T, CMAX>),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
