@@ -118,12 +118,15 @@ __device__ __forceinline__ bool keep_mid(const int32_t* f, int a, int b, int c)
 //     is unimodal along the envelope and its site moves right with p);
 //     results go back through LDS so that loads and stores are 16-byte vectors
 //     along the contiguous z axis.
-// A line whose values are all equal is its own result (g = f).
+// A line whose values are all equal is its own result (g = f).  Sites holding
+// the sentinel (no site reached yet) are left out of the hulls: such a site
+// contributes sentinel + (p-s)^2 >= sentinel, with equality only at its own
+// position, so the result is min(envelope of the other sites, sentinel).
 template <int W, int T, int CMAX>
 __global__ __launch_bounds__(W* T) void k_edt_lines(const int32_t* __restrict__ src, int32_t* __restrict__ dst, int L,
                                                     int64_t axis_stride, int nz, int ztiles, int64_t outer_stride,
                                                     int outer_count, int64_t plane_stride, uint32_t tiles,
-                                                    uint32_t grid)
+                                                    uint32_t grid, int32_t sentinel)
 {
     extern __shared__ int32_t lds[];
     const uint32_t tile = xcd_tile(blockIdx.x, grid);
@@ -180,6 +183,7 @@ __global__ __launch_bounds__(W* T) void k_edt_lines(const int32_t* __restrict__ 
             const int32_t fq = fl[q];
             mn = min(mn, fq);
             mx = max(mx, fq);
+            if (fq >= sentinel) continue;
             const int64_t Fq = (int64_t)fq + (int64_t)q * q;
             while (k >= 1) {
                 // top stays iff it is strictly lowest somewhere between sec and q
@@ -342,7 +346,7 @@ __global__ __launch_bounds__(W* T) void k_edt_lines(const int32_t* __restrict__ 
                         if (nv > cv) break;
                         cb = nb; ct = nt; ci = ni; cs = ns; cf = nf; cv = nv;
                     }
-                    out[i] = cv;
+                    out[i] = min(cv, sentinel);
                 }
             }
         }
@@ -387,19 +391,21 @@ struct LineCfg {
     int W, T, CMAX;
 };
 
-// Tile shape by line length: LDS holds W*(L+1) int32 values plus CMAX*W*T
-// uint16 stack slots; the choice keeps it near or under 64 KB.
+// Tile shape by line length (W lines, T threads per line, CMAX >= L/T): LDS
+// holds W*(L+1) int32 values plus CMAX*W*T uint16 hull slots.  Measured on
+// MI355X at L = 512: 16 lines x 32 threads beats 16 x 16 and 16 x 64 (fewer
+// merge rounds vs. shorter serial chunks) and 32-line tiles (LDS occupancy).
 LineCfg pick_cfg(int L)
 {
     if (L <= 128) return {32, 8, 16};
-    if (L <= 512) return {16, 16, 32};
+    if (L <= 512) return {16, 32, 16};
     if (L <= 1024) return {8, 32, 32};
     return {4, 64, 64};
 }
 
 template <int W, int T, int CMAX>
 hipError_t launch_lines_t(const int32_t* src, int32_t* dst, int L, int64_t axis_stride, int nz, int64_t outer_stride,
-                          int outer_count, int planes, int64_t plane_stride, hipStream_t st)
+                          int outer_count, int planes, int64_t plane_stride, int32_t sentinel, hipStream_t st)
 {
     const int ztiles = (nz + W - 1) / W;
     const uint64_t tiles = (uint64_t)ztiles * outer_count * planes;
@@ -411,18 +417,18 @@ hipError_t launch_lines_t(const int32_t* src, int32_t* dst, int L, int64_t axis_
         if (e != hipSuccess) return e;
     }
     hipLaunchKernelGGL((k_edt_lines<W, T, CMAX>), dim3(grid), dim3(W * T), lds, st, src, dst, L, axis_stride, nz,
-                       ztiles, outer_stride, outer_count, plane_stride, (uint32_t)tiles, grid);
+                       ztiles, outer_stride, outer_count, plane_stride, (uint32_t)tiles, grid, sentinel);
     return hipGetLastError();
 }
 
 hipError_t launch_lines(const int32_t* src, int32_t* dst, int L, int64_t axis_stride, int nz, int64_t outer_stride,
-                        int outer_count, int planes, int64_t plane_stride, hipStream_t st)
+                        int outer_count, int planes, int64_t plane_stride, int32_t sentinel, hipStream_t st)
 {
     LineCfg c = pick_cfg(L);
-    if (c.W == 32) return launch_lines_t<32, 8, 16>(src, dst, L, axis_stride, nz, outer_stride, outer_count, planes, plane_stride, st);
-    if (c.W == 16) return launch_lines_t<16, 16, 32>(src, dst, L, axis_stride, nz, outer_stride, outer_count, planes, plane_stride, st);
-    if (c.W == 8) return launch_lines_t<8, 32, 32>(src, dst, L, axis_stride, nz, outer_stride, outer_count, planes, plane_stride, st);
-    return launch_lines_t<4, 64, 64>(src, dst, L, axis_stride, nz, outer_stride, outer_count, planes, plane_stride, st);
+    if (c.W == 32) return launch_lines_t<32, 8, 16>(src, dst, L, axis_stride, nz, outer_stride, outer_count, planes, plane_stride, sentinel, st);
+    if (c.W == 16) return launch_lines_t<16, 32, 16>(src, dst, L, axis_stride, nz, outer_stride, outer_count, planes, plane_stride, sentinel, st);
+    if (c.W == 8) return launch_lines_t<8, 32, 32>(src, dst, L, axis_stride, nz, outer_stride, outer_count, planes, plane_stride, sentinel, st);
+    return launch_lines_t<4, 64, 64>(src, dst, L, axis_stride, nz, outer_stride, outer_count, planes, plane_stride, sentinel, st);
 }
 
 }  // namespace
@@ -458,9 +464,9 @@ extern "C" int fe_edt_device(const uint8_t* mask, int nx, int ny, int nz, int pl
         hipLaunchKernelGGL(k_edt_zpass<false>, dim3(zgrid), dim3(256), 0, st, mask, out, rows, nz, n, sentinel, site_one);
     FE_TRY(hipGetLastError());
     // y lines: outer = x (stride ny*nz), axis stride nz
-    FE_TRY(launch_lines(out, work, ny, nz, nz, (int64_t)ny * nz, nx, planes, n, st));
+    FE_TRY(launch_lines(out, work, ny, nz, nz, (int64_t)ny * nz, nx, planes, n, sentinel, st));
     // x lines: outer = y (stride nz), axis stride ny*nz
-    FE_TRY(launch_lines(work, out, nx, (int64_t)ny * nz, nz, nz, ny, planes, n, st));
+    FE_TRY(launch_lines(work, out, nx, (int64_t)ny * nz, nz, nz, ny, planes, n, sentinel, st));
     return FE_OK;
 }
 
@@ -488,8 +494,10 @@ extern "C" int fe_edt_axis_pass(const void* src, int32_t* dst, int nx, int ny, i
         return FE_OK;
     }
     const int32_t* a = static_cast<const int32_t*>(src);
-    if (axis == 1) FE_TRY(launch_lines(a, dst, ny, nz, nz, (int64_t)ny * nz, nx, planes, n, st));
-    else FE_TRY(launch_lines(a, dst, nx, (int64_t)ny * nz, nz, nz, ny, planes, n, st));
+    const int32_t sent =
+        (int32_t)((int64_t)(nx - 1) * (nx - 1) + (int64_t)(ny - 1) * (ny - 1) + (int64_t)(nz - 1) * (nz - 1) + 1);
+    if (axis == 1) FE_TRY(launch_lines(a, dst, ny, nz, nz, (int64_t)ny * nz, nx, planes, n, sent, st));
+    else FE_TRY(launch_lines(a, dst, nx, (int64_t)ny * nz, nz, nz, ny, planes, n, sent, st));
     return FE_OK;
 }
 
