@@ -35,6 +35,7 @@ SIGNATURES = {
     "fe_field_checksum": (ct.c_int, [P, ct.c_int, F64, P, P]),
     "fe_dense_classify": (ct.c_int, [P, ct.c_int, ct.c_int, ct.c_int, ct.c_int, ct.c_float, P, P]),
     "fe_dense_tiles": (ct.c_int, [P, ct.c_int, ct.c_int, ct.c_int, ct.c_int, P, I64, P, P]),
+    "fe_query_points": (ct.c_int, [P, P, I64, P, P]),
 }
 
 _lib = None

@@ -553,6 +553,44 @@ __global__ __launch_bounds__(256) void k_checksum(const FieldDesc* __restrict__ 
     if ((threadIdx.x & 63) == 0) atomicAdd(out, (unsigned long long)s);
 }
 
+// ---------------------------------------------------------------- point queries
+// Point SDF of a resident field: the recipe's fp32 fold at arbitrary points
+// (the same arithmetic as the stored tiles), evaluated over the culled list
+// of the superblock containing the point; points outside every superblock
+// box evaluate the whole recipe.  One thread per point.
+__global__ __launch_bounds__(256) void k_query_points(const FieldDesc* __restrict__ desc, const float* __restrict__ pts,
+                                                      int64_t n, float* __restrict__ out)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const FieldDesc& D = *desc;
+    const float x = pts[3 * i], y = pts[3 * i + 1], z = pts[3 * i + 2];
+    const float span = (float)(D.SB * D.B) * D.pitch;
+    const int sx = (int)floorf((x - D.x0) / span), sy = (int)floorf((y - D.y0) / span), sz = (int)floorf((z - D.z0) / span);
+    const uint16_t* list = nullptr;
+    int nl = D.nprims;
+    if (sx >= 0 && sy >= 0 && sz >= 0 && sx < D.nsbx && sy < D.nsby && sz < D.nsbz) {
+        // the point must lie in the box the list was culled for
+        const int bx0 = sx * D.SB, by0 = sy * D.SB, bz0 = sz * D.SB, B = D.B;
+        const float lx = D.x0 + (float)(bx0 * B) * D.pitch, ux = D.x0 + (float)((bx0 + D.SB) * B) * D.pitch;
+        const float ly = D.y0 + (float)(by0 * B) * D.pitch, uy = D.y0 + (float)((by0 + D.SB) * B) * D.pitch;
+        const float lz = D.z0 + (float)(bz0 * B) * D.pitch, uz = D.z0 + (float)((bz0 + D.SB) * B) * D.pitch;
+        const int sb = (sx * D.nsby + sy) * D.nsbz + sz;
+        const int len = D.list_len[sb];
+        if (len >= 0 && x >= lx && x <= ux && y >= ly && y <= uy && z >= lz && z <= uz) {
+            list = D.lists + (int64_t)sb * D.cap;
+            nl = len;
+        }
+    }
+    float f = 0.0f;
+    for (int j = 0; j < nl; ++j) {
+        const Prim p = D.prims[list ? list[j] : j];
+        const float v = prim_eval(p.kind, p.cx, p.cy, p.cz, p.a, p.b, p.c, x, y, z);
+        f = j == 0 ? v : fold(p.op, f, v);
+    }
+    out[i] = f;
+}
+
 // ---------------------------------------------------------------- dense-source classify
 // FE/faltkarna_v1_mesh_to_sdf.py:602-632 (corners clamped to n-1).
 __global__ __launch_bounds__(256) void k_classify_dense(const float* __restrict__ dense, int nx, int ny, int nz, int B,
@@ -691,6 +729,18 @@ extern "C" int fe_dense_tiles(const float* dense, int nx, int ny, int nz, int B,
     const int64_t n = n_active * B * B * B;
     hipLaunchKernelGGL(k_tiles_dense, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, static_cast<hipStream_t>(stream),
                        dense, nx, ny, nz, B, nby, nbz, ids, n_active, tiles);
+    FE_TRY(hipGetLastError());
+    return FE_OK;
+}
+
+// Point SDF queries on one resident field (desc: device pointer to its
+// descriptor; pts: device float32 (n,3); out: device float32 (n)).
+extern "C" int fe_query_points(const void* desc, const float* pts, int64_t n, float* out, void* stream) noexcept
+{
+    if (!desc || !pts || !out || n < 0) return FE_INVALID;
+    if (n == 0) return FE_OK;
+    hipLaunchKernelGGL(k_query_points, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, static_cast<hipStream_t>(stream),
+                       static_cast<const FieldDesc*>(desc), pts, n, out);
     FE_TRY(hipGetLastError());
     return FE_OK;
 }

@@ -82,6 +82,43 @@ def cpu_baseline(fe, rec, occ_host, threads):
             "field_mvox_s": field_vox / t_field / 1e6, "edt_mvox_s": edt_vox / t_edt / 1e6}
 
 
+def bench_batch(fe, torch, count, plain_sample):
+    """Uneven recipes (seed 3) in one graph-captured launch vs sequential plain calls."""
+    items = fe.batch.batch_workload(seed=3, count=count)
+    idx = list(range(0, count, max(1, count // plain_sample)))[:plain_sample]
+    res, b = fe.batch.time_batch_vs_plain(items, reps=3, plain_items=idx)
+    vox = sum(s[4] * s[5] * s[6] for s, _ in items)
+    res.update(recipes=count, voxels=vox, batch_gvox_s=vox / (res["batch_ms"] * 1e-3) / 1e9,
+               plain_sample=len(idx), note="plain rate extrapolated from an evenly spaced sample of the batch")
+    del b
+    torch.cuda.empty_cache()
+    return res
+
+
+def bench_serve(fe, torch, rates, count):
+    """Poisson arrivals of 512^3 / 1024-primitive recipes into the field server."""
+    rng = np.random.default_rng(5)
+    metas = [(fe.GridMeta(0.0, 0.0, 0.0, 1.0, 512, 512, 512, block=8), fe.recipes.mixed_recipe(rng, 1024, 512))
+             for _ in range(count)]
+    out = {}
+    for rate in rates:
+        srv = fe.server.FieldServer(hbm_budget=200e9, slots=2, max_batch=8)
+        try:
+            fps, st, _ = fe.server.poisson_run(srv, metas, rate=rate, queries_per_field=100000)
+        finally:
+            srv.close()
+        out[f"rate_{int(rate)}"] = dict(fields_s=fps, **{k: st[k] for k in ("build_p50_ms", "build_p99_ms",
+                                                                               "query_p50_ms", "query_p99_ms")})
+        torch.cuda.empty_cache()
+    t0 = time.perf_counter()
+    for m, r in metas[:8]:
+        fe.klassificera_och_evaluera_recept(m, r)
+    torch.cuda.synchronize()
+    out["plain_fields_s"] = 8 / (time.perf_counter() - t0)
+    out["requests_per_rate"] = count
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -89,6 +126,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--quick", action="store_true", help="single-field step only")
+    ap.add_argument("--batch-count", type=int, default=256)
+    ap.add_argument("--serve-count", type=int, default=48)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU: the block-sparse build and the EDT run on one "
@@ -172,6 +212,11 @@ def main():
     }
     occ_host = occ.cpu().numpy()
     out["config"]["occupancy_fraction"] = float(occ_host.mean())
+    del plan, dense, pair, work
+    torch.cuda.empty_cache()
+    if not args.quick:
+        out["batched"] = bench_batch(fe, torch, args.batch_count, plain_sample=32)
+        out["served"] = bench_serve(fe, torch, (200.0, 2000.0), args.serve_count)
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", "16"))
         out["cpu_baseline"] = cpu_baseline(fe, rec, occ_host, threads)

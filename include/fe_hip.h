@@ -100,6 +100,12 @@ int fe_dense_classify(const float *dense, int nx, int ny, int nz, int B, float m
 int fe_dense_tiles(const float *dense, int nx, int ny, int nz, int B, const int32_t *ids, int64_t n_active,
                    float *tiles, void *stream);
 
+/* Point SDF queries on a resident field: the recipe's fp32 fold at device
+ * points (n,3), with the culled list of the containing superblock.  Replaces
+ * the per-batch eval_batch launch of ikarus_v1/eval_warp.py:478-489 for a
+ * built field. */
+int fe_query_points(const void *desc, const float *points, int64_t n, float *out, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
