@@ -36,6 +36,23 @@ SIGNATURES = {
     "fe_dense_classify": (ct.c_int, [P, ct.c_int, ct.c_int, ct.c_int, ct.c_int, ct.c_float, P, P]),
     "fe_dense_tiles": (ct.c_int, [P, ct.c_int, ct.c_int, ct.c_int, ct.c_int, P, I64, P, P]),
     "fe_query_points": (ct.c_int, [P, P, I64, P, P]),
+    # leaf-box pack, lookups, CSG, tile reader
+    "adaptive_pack": (P, [P, P, ct.c_int]),
+    "adaptive_nodes": (I64, [P]),
+    "adaptive_samples": (I64, [P]),
+    "adaptive_copy": (ct.c_int, [P, P, I64, P, I64]),
+    "adaptive_free": (None, [P]),
+    "fe_pack_device": (P, [P, ct.c_int, ct.c_int, ct.c_int, ct.c_int, ct.POINTER(ct.c_int), P]),
+    "fe_pack_nodes": (I64, [P]),
+    "fe_pack_samples": (I64, [P]),
+    "fe_pack_copy": (ct.c_int, [P, P, I64, P, I64, P]),
+    "fe_pack_free": (None, [P]),
+    "fe_samples_at": (ct.c_int, [P, P, P, I64, P, P]),
+    "fe_query_packed": (ct.c_int, [P, P, ct.c_int, ct.c_int, ct.c_int, ct.POINTER(ct.c_double), F64, P, I64, P, P]),
+    "fe_unpack": (ct.c_int, [P, I64, P, ct.c_int, ct.c_int, ct.c_int, P, P]),
+    "csg_gather": (ct.c_int, [P, I64, P, I64, P, P, P]),
+    "csg_execute": (ct.c_int, [P, P, P, P, I32, P, P, I32, P]),
+    "fe_csg_device": (ct.c_int, [P, P, P, P, I32, P, P, I32, P, P]),
 }
 
 _lib = None

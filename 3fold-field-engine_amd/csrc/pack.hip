@@ -1,0 +1,548 @@
+// Lossless leaf-box packing of a dense fp32 field on gfx950, plus leaf
+// unpacking, point lookups and the trilinear query on packed fields.
+//
+// Reference semantics (FE/adaptive_sample_blocks_v1.py:19-58 and its native
+// twin FE/adaptive_pack_native_v1/pack.cpp:18-55): a node is a box; along
+// each axis it is "invariant" when every sample's bit pattern equals the one
+// in the node's first slab, and then stores one slab; its direct cost is
+// 52 + 4*prod(stored).  A node with prod(stored) <= 64 or at most
+// `leaf_samples` samples is a leaf; otherwise it is bisected at shape//2 along
+// the largest non-invariant axis (first on ties) and kept split only when
+// 52 + cost(left) + cost(right) < direct cost.  The output is the pre-order
+// (left first) node table of 52-byte records and the payload of every leaf's
+// stored samples in C order.
+//
+// GPU formulation: the candidate split tree is built top-down one level per
+// pass (invariance flags by wave-ballot AND reductions over each candidate's
+// samples, split decisions and child creation per node), the keep-or-collapse
+// decisions and subtree sizes bottom-up, the pre-order indices and payload
+// offsets top-down, and a final pass writes records and copies payloads.
+// Every decision depends only on exact bit comparisons and integer costs, so
+// the table is identical to the reference's.
+#include "fe_common.hpp"
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <vector>
+
+namespace {
+
+struct PNode {
+    int32_t lo[3], shape[3];
+    int32_t inv;    // bit a set: axis a invariant
+    int32_t child;  // left child index in the next level (right = child + 1), -1: none
+    int32_t stored[3];
+    int32_t split;  // kept split
+    int64_t reduced, direct, cost, nn, pay, idx, off;
+    int32_t reach, pad;
+};
+
+__device__ __forceinline__ int64_t node_count(const PNode& p) { return (int64_t)p.shape[0] * p.shape[1] * p.shape[2]; }
+
+// AND-reduction of the three invariance flags over one part of a node.
+__global__ __launch_bounds__(256) void k_pack_inv(PNode* __restrict__ nodes, int n, int S, const uint32_t* __restrict__ d,
+                                                  int64_t sx, int64_t sy)
+{
+    const int node = blockIdx.x / S, part = blockIdx.x % S;
+    if (node >= n) return;
+    PNode& P = nodes[node];
+    const int64_t cnt = node_count(P);
+    const int64_t b = cnt * part / S, e = cnt * (part + 1) / S;
+    const int hy = P.shape[1], hz = P.shape[2];
+    const int64_t o = (int64_t)P.lo[0] * sx + (int64_t)P.lo[1] * sy + P.lo[2];
+    bool ex = true, ey = true, ez = true;
+    for (int64_t l = b + threadIdx.x; l < e; l += 256) {
+        const int z = (int)(l % hz);
+        const int64_t r = l / hz;
+        const int y = (int)(r % hy), x = (int)(r / hy);
+        const uint32_t v = d[o + x * sx + y * sy + z];
+        ex = ex && v == d[o + y * sy + z];
+        ey = ey && v == d[o + x * sx + z];
+        ez = ez && v == d[o + x * sx + y * sy];
+    }
+    const int m = (__ballot(!ex) ? 0 : 1) | (__ballot(!ey) ? 0 : 2) | (__ballot(!ez) ? 0 : 4);
+    if ((threadIdx.x & 63) == 0 && m != 7) atomicAnd(&P.inv, m);
+}
+
+__device__ void init_node(PNode& c, const int* lo, const int* shape)
+{
+    for (int a = 0; a < 3; ++a) {
+        c.lo[a] = lo[a];
+        c.shape[a] = shape[a];
+        c.stored[a] = 0;
+    }
+    c.inv = 7;
+    c.child = -1;
+    c.split = 0;
+    c.reach = 0;
+    c.reduced = c.direct = c.cost = c.nn = c.pay = c.idx = c.off = 0;
+}
+
+__global__ __launch_bounds__(256) void k_pack_decide(PNode* __restrict__ nodes, int n, int leaf, PNode* __restrict__ next,
+                                                     int* __restrict__ next_count)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    PNode& P = nodes[i];
+    int64_t reduced = 1;
+    for (int a = 0; a < 3; ++a) {
+        P.stored[a] = ((P.inv >> a) & 1) ? 1 : P.shape[a];
+        reduced *= P.stored[a];
+    }
+    P.reduced = reduced;
+    P.direct = 52 + 4 * reduced;
+    P.child = -1;
+    if (reduced <= 64 || node_count(P) <= leaf) return;
+    int axis = -1;
+    for (int a = 0; a < 3; ++a)
+        if (P.stored[a] > 1 && (axis < 0 || P.shape[a] > P.shape[axis])) axis = a;
+    if (axis < 0) return;
+    const int c = atomicAdd(next_count, 2);
+    int ls[3], rs[3], rl[3];
+    for (int a = 0; a < 3; ++a) {
+        ls[a] = rs[a] = P.shape[a];
+        rl[a] = P.lo[a];
+    }
+    ls[axis] = P.shape[axis] / 2;
+    rs[axis] -= ls[axis];
+    rl[axis] += ls[axis];
+    init_node(next[c], P.lo, ls);
+    init_node(next[c + 1], rl, rs);
+    P.child = c;
+}
+
+__global__ __launch_bounds__(256) void k_pack_bottom(PNode* __restrict__ nodes, int n, const PNode* __restrict__ next)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    PNode& P = nodes[i];
+    P.split = 0;
+    P.cost = P.direct;
+    P.nn = 1;
+    P.pay = P.reduced;
+    if (P.child < 0) return;
+    const PNode& L = next[P.child];
+    const PNode& R = next[P.child + 1];
+    const int64_t s = 52 + L.cost + R.cost;
+    if (s < P.direct) {
+        P.split = 1;
+        P.cost = s;
+        P.nn = 1 + L.nn + R.nn;
+        P.pay = L.pay + R.pay;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_pack_top(const PNode* __restrict__ nodes, int n, PNode* __restrict__ next)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const PNode& P = nodes[i];
+    if (!P.reach || !P.split) return;
+    PNode& L = next[P.child];
+    PNode& R = next[P.child + 1];
+    L.reach = R.reach = 1;
+    L.idx = P.idx + 1;
+    L.off = P.off;
+    R.idx = P.idx + 1 + L.nn;
+    R.off = P.off + L.pay;
+}
+
+// Writes the 52-byte record (as 13 words) of every node in the final tree and
+// copies each final leaf's stored samples into the payload.
+__global__ __launch_bounds__(256) void k_pack_emit(const PNode* __restrict__ nodes, int n, const PNode* __restrict__ next,
+                                                   uint32_t* __restrict__ rec, uint32_t* __restrict__ payload,
+                                                   const uint32_t* __restrict__ d, int64_t sx, int64_t sy)
+{
+    const int i = blockIdx.x;
+    if (i >= n) return;
+    const PNode& P = nodes[i];
+    if (!P.reach) return;
+    if (threadIdx.x == 0) {
+        uint32_t* r = rec + P.idx * 13;
+        for (int a = 0; a < 3; ++a) {
+            r[a] = (uint32_t)P.lo[a];
+            r[3 + a] = (uint32_t)P.shape[a];
+            r[6 + a] = P.split ? 0u : (uint32_t)P.stored[a];
+        }
+        int64_t off = -1;
+        if (P.split) {
+            const PNode& L = next[P.child];
+            r[9] = (uint32_t)(P.idx + 1);
+            r[10] = (uint32_t)(P.idx + 1 + L.nn);
+        } else {
+            r[9] = r[10] = 0xffffffffu;
+            off = P.off;
+        }
+        r[11] = (uint32_t)((uint64_t)off & 0xffffffffu);
+        r[12] = (uint32_t)((uint64_t)off >> 32);
+    }
+    if (P.split) return;
+    const int s1 = P.stored[1], s2 = P.stored[2];
+    const int64_t o = (int64_t)P.lo[0] * sx + (int64_t)P.lo[1] * sy + P.lo[2];
+    for (int64_t j = threadIdx.x; j < P.reduced; j += 256) {
+        const int z = (int)(j % s2);
+        const int64_t r = j / s2;
+        const int y = (int)(r % s1), x = (int)(r / s1);
+        payload[P.off + j] = d[o + x * sx + y * sy + z];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_nonfinite(const uint32_t* __restrict__ d, int64_t n, int* __restrict__ flag)
+{
+    bool bad = false;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+        bad = bad || (d[i] & 0x7f800000u) == 0x7f800000u;
+    if (__ballot(bad) && (threadIdx.x & 63) == 0) atomicOr(flag, 1);
+}
+
+struct Packed {
+    int64_t n_nodes = 0, n_samples = 0;
+    uint32_t* nodes = nullptr;    // device, 13 words per record
+    uint32_t* payload = nullptr;  // device
+    ~Packed()
+    {
+        if (nodes) (void)hipFree(nodes);
+        if (payload) (void)hipFree(payload);
+    }
+};
+
+// Device pack of `data` (device, C order, nx*ny*nz fp32).  Returns nullptr on
+// a non-finite sample or a runtime failure (fe_last_error tells which).
+Packed* pack_device(const float* data, int nx, int ny, int nz, int leaf, hipStream_t st, int* status)
+{
+    *status = FE_RUNTIME;
+    const int64_t N = (int64_t)nx * ny * nz;
+    const uint32_t* d = reinterpret_cast<const uint32_t*>(data);
+    const int64_t sx = (int64_t)ny * nz, sy = nz;
+    std::vector<PNode*> lv;
+    std::vector<int> cnt;
+    int* counter = nullptr;
+    int* hcounter = nullptr;
+    Packed* out = nullptr;
+    bool ok = false;
+    auto cleanup = [&]() {
+        for (PNode* p : lv) (void)hipFree(p);
+        if (counter) (void)hipFree(counter);
+        if (hcounter) (void)hipHostFree(hcounter);
+    };
+    do {
+        if (hipMalloc(&counter, 16) != hipSuccess) break;
+        if (hipHostMalloc(&hcounter, 16, hipHostMallocDefault) != hipSuccess) break;
+        if (hipMemsetAsync(counter, 0, 16, st) != hipSuccess) break;
+        {
+            const unsigned g = (unsigned)std::min<int64_t>((N + 255) / 256, 4096);
+            hipLaunchKernelGGL(k_nonfinite, dim3(g), dim3(256), 0, st, d, N, counter);
+            if (hipMemcpyAsync(hcounter, counter, 4, hipMemcpyDeviceToHost, st) != hipSuccess) break;
+            if (hipStreamSynchronize(st) != hipSuccess) break;
+            if (hcounter[0]) {
+                *status = FE_INVALID;
+                break;
+            }
+        }
+        PNode root;
+        std::memset(&root, 0, sizeof root);
+        root.shape[0] = nx;
+        root.shape[1] = ny;
+        root.shape[2] = nz;
+        root.inv = 7;
+        root.child = -1;
+        root.reach = 1;
+        PNode* r0 = nullptr;
+        if (hipMalloc(&r0, sizeof(PNode)) != hipSuccess) break;
+        lv.push_back(r0);
+        cnt.push_back(1);
+        if (hipMemcpyAsync(r0, &root, sizeof root, hipMemcpyHostToDevice, st) != hipSuccess) break;
+        bool fail = false;
+        while (cnt.back() > 0) {
+            const int n = cnt.back();
+            const int64_t per = std::max<int64_t>(1, N / n);
+            const int S = (int)std::max<int64_t>(1, std::min<int64_t>((per + 16383) / 16384, (1 << 30) / n));
+            hipLaunchKernelGGL(k_pack_inv, dim3((unsigned)((int64_t)n * S)), dim3(256), 0, st, lv.back(), n, S, d, sx,
+                               sy);
+            PNode* next = nullptr;
+            if (hipMalloc(&next, sizeof(PNode) * (size_t)2 * n) != hipSuccess) { fail = true; break; }
+            lv.push_back(next);
+            if (hipMemsetAsync(counter, 0, 4, st) != hipSuccess) { fail = true; break; }
+            hipLaunchKernelGGL(k_pack_decide, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, lv[lv.size() - 2], n,
+                               leaf, next, counter);
+            if (hipMemcpyAsync(hcounter, counter, 4, hipMemcpyDeviceToHost, st) != hipSuccess) { fail = true; break; }
+            if (hipStreamSynchronize(st) != hipSuccess) { fail = true; break; }
+            cnt.push_back(hcounter[0]);
+        }
+        if (fail) break;
+        const int L = (int)cnt.size();
+        for (int l = L - 1; l >= 0; --l) {
+            if (cnt[l] == 0) continue;
+            hipLaunchKernelGGL(k_pack_bottom, dim3((unsigned)((cnt[l] + 255) / 256)), dim3(256), 0, st, lv[l], cnt[l],
+                               l + 1 < L ? lv[l + 1] : nullptr);
+        }
+        int64_t sizes[2];
+        if (hipMemcpyAsync(sizes, &lv[0]->nn, 16, hipMemcpyDeviceToHost, st) != hipSuccess) break;
+        if (hipStreamSynchronize(st) != hipSuccess) break;
+        out = new (std::nothrow) Packed();
+        if (!out) break;
+        out->n_nodes = sizes[0];
+        out->n_samples = sizes[1];
+        if (hipMalloc(&out->nodes, (size_t)out->n_nodes * 52) != hipSuccess) break;
+        if (hipMalloc(&out->payload, (size_t)std::max<int64_t>(1, out->n_samples) * 4) != hipSuccess) break;
+        for (int l = 0; l + 1 < L; ++l) {
+            if (cnt[l] == 0) continue;
+            hipLaunchKernelGGL(k_pack_top, dim3((unsigned)((cnt[l] + 255) / 256)), dim3(256), 0, st, lv[l], cnt[l],
+                               lv[l + 1]);
+        }
+        for (int l = 0; l < L; ++l) {
+            if (cnt[l] == 0) continue;
+            hipLaunchKernelGGL(k_pack_emit, dim3((unsigned)cnt[l]), dim3(256), 0, st, lv[l], cnt[l],
+                               l + 1 < L ? lv[l + 1] : nullptr, out->nodes, out->payload, d, sx, sy);
+        }
+        if (hipGetLastError() != hipSuccess) break;
+        if (hipStreamSynchronize(st) != hipSuccess) break;
+        ok = true;
+        *status = FE_OK;
+    } while (0);
+    cleanup();
+    if (!ok) {
+        delete out;
+        out = nullptr;
+    }
+    return out;
+}
+
+// ---------------------------------------------------------------- lookups
+struct Node52 {
+    int32_t lo[3], shape[3], stored[3], children[2];
+    int32_t off_lo, off_hi;
+};
+static_assert(sizeof(Node52) == 52, "node record");
+
+__device__ __forceinline__ int64_t node_offset(const Node52& n)
+{
+    return (int64_t)(((uint64_t)(uint32_t)n.off_hi << 32) | (uint32_t)n.off_lo);
+}
+
+// Leaf lookup by tree descent (FE/adaptive_sample_blocks_v1.py:103-128).
+__device__ __forceinline__ uint32_t lookup(const Node52* __restrict__ nodes, const uint32_t* __restrict__ payload, int x,
+                                           int y, int z)
+{
+    int id = 0;
+    for (;;) {
+        const Node52& nd = nodes[id];
+        if (nd.children[0] < 0) {
+            const int64_t lx = nd.stored[0] == 1 ? 0 : x - nd.lo[0];
+            const int64_t ly = nd.stored[1] == 1 ? 0 : y - nd.lo[1];
+            const int64_t lz = nd.stored[2] == 1 ? 0 : z - nd.lo[2];
+            return payload[node_offset(nd) + (lx * nd.stored[1] + ly) * nd.stored[2] + lz];
+        }
+        const Node52& l = nodes[nd.children[0]];
+        const bool in = x >= l.lo[0] && x < l.lo[0] + l.shape[0] && y >= l.lo[1] && y < l.lo[1] + l.shape[1] &&
+                        z >= l.lo[2] && z < l.lo[2] + l.shape[2];
+        id = in ? nd.children[0] : nd.children[1];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_samples_at(const Node52* __restrict__ nodes, const uint32_t* __restrict__ payload,
+                                                    const int64_t* __restrict__ idx, int64_t n, uint32_t* __restrict__ out)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    out[i] = lookup(nodes, payload, (int)idx[3 * i], (int)idx[3 * i + 1], (int)idx[3 * i + 2]);
+}
+
+// Trilinear query (FE/adaptive_sample_blocks_v1.py:130-151): f64 grid
+// coordinates clipped to [0, n-1], eight clamped neighbours, and the order-1
+// spline sum of scipy.ndimage.map_coordinates (w0 = 1-t, w1 = 1-w0; each term
+// value*w_x*w_y*w_z, summed with the last axis fastest), rounded to fp32.
+__global__ __launch_bounds__(256) void k_query(const Node52* __restrict__ nodes, const uint32_t* __restrict__ payload,
+                                               int nx, int ny, int nz, double ox, double oy, double oz, double pitch,
+                                               const double* __restrict__ pts, int64_t n, float* __restrict__ out)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const int shape[3] = {nx, ny, nz};
+    const double org[3] = {ox, oy, oz};
+    int lower[3];
+    double w[3][2];
+    for (int a = 0; a < 3; ++a) {
+        double c = (pts[3 * i + a] - org[a]) / pitch;
+        const double hi = (double)(shape[a] - 1);
+        if (c < 0.0) c = 0.0;
+        if (c > hi) c = hi;
+        const double fl = floor(c);
+        lower[a] = (int)fl;
+        double t = c - fl;
+        t -= floor(t);
+        w[a][0] = 1.0 - t;
+        w[a][1] = 1.0 - w[a][0];
+    }
+    double acc = 0.0;
+    for (int h = 0; h < 8; ++h) {
+        const int bx = (h >> 2) & 1, by = (h >> 1) & 1, bz = h & 1;
+        const int qx = min(lower[0] + bx, nx - 1), qy = min(lower[1] + by, ny - 1), qz = min(lower[2] + bz, nz - 1);
+        const uint32_t bits = lookup(nodes, payload, qx, qy, qz);
+        double coeff = (double)__uint_as_float(bits);
+        coeff *= 1.0;
+        coeff *= w[0][bx];
+        coeff *= w[1][by];
+        coeff *= w[2][bz];
+        acc += coeff;
+    }
+    out[i] = (float)acc;
+}
+
+// Dense reconstruction of a packed field: one workgroup per leaf writes its box.
+__global__ __launch_bounds__(256) void k_unpack(const Node52* __restrict__ nodes, int64_t count,
+                                                const uint32_t* __restrict__ payload, int nx, int ny, int nz,
+                                                uint32_t* __restrict__ dense)
+{
+    const int64_t id = blockIdx.x;
+    if (id >= count) return;
+    const Node52& nd = nodes[id];
+    if (nd.children[0] >= 0) return;
+    const int hx = min(nd.shape[0], nx - nd.lo[0]), hy = min(nd.shape[1], ny - nd.lo[1]), hz = min(nd.shape[2], nz - nd.lo[2]);
+    if (hx <= 0 || hy <= 0 || hz <= 0) return;
+    const int64_t off = node_offset(nd);
+    const int64_t cnt = (int64_t)hx * hy * hz;
+    for (int64_t j = threadIdx.x; j < cnt; j += 256) {
+        const int z = (int)(j % hz);
+        const int64_t r = j / hz;
+        const int y = (int)(r % hy), x = (int)(r / hy);
+        const int64_t sx = nd.stored[0] == 1 ? 0 : x, sy = nd.stored[1] == 1 ? 0 : y, sz = nd.stored[2] == 1 ? 0 : z;
+        dense[((int64_t)(nd.lo[0] + x) * ny + nd.lo[1] + y) * nz + nd.lo[2] + z] =
+            payload[off + (sx * nd.stored[1] + sy) * nd.stored[2] + sz];
+    }
+}
+
+struct HostPacked {
+    std::vector<uint8_t> nodes;
+    std::vector<uint32_t> payload;
+};
+
+}  // namespace
+
+// ---------------------------------------------------------------- device ABI
+extern "C" void* fe_pack_device(const float* data, int nx, int ny, int nz, int leaf, int* status, void* stream) noexcept
+{
+    int st_local = FE_INVALID;
+    int* s = status ? status : &st_local;
+    *s = FE_INVALID;
+    if (!data || nx < 1 || ny < 1 || nz < 1 || leaf < 8 || leaf > 4096) return nullptr;
+    try {
+        return pack_device(data, nx, ny, nz, leaf, static_cast<hipStream_t>(stream), s);
+    } catch (...) {
+        *s = FE_RUNTIME;
+        return nullptr;
+    }
+}
+
+extern "C" int64_t fe_pack_nodes(const void* h) noexcept { return h ? static_cast<const Packed*>(h)->n_nodes : 0; }
+extern "C" int64_t fe_pack_samples(const void* h) noexcept { return h ? static_cast<const Packed*>(h)->n_samples : 0; }
+
+// Copies a device pack into caller buffers (device or host; hipMemcpyDefault).
+extern "C" int fe_pack_copy(const void* h, void* nodes, int64_t count, void* payload, int64_t samples, void* stream) noexcept
+{
+    const Packed* p = static_cast<const Packed*>(h);
+    if (!p || !nodes || !payload || count != p->n_nodes || samples != p->n_samples) return FE_INVALID;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    FE_TRY(hipMemcpyAsync(nodes, p->nodes, (size_t)count * 52, hipMemcpyDefault, st));
+    if (samples) FE_TRY(hipMemcpyAsync(payload, p->payload, (size_t)samples * 4, hipMemcpyDefault, st));
+    FE_TRY(hipStreamSynchronize(st));
+    return FE_OK;
+}
+
+extern "C" void fe_pack_free(void* h) noexcept { delete static_cast<Packed*>(h); }
+
+// Packed-field lookups on the device: nodes (count x 52 bytes) and payload are device pointers.
+extern "C" int fe_samples_at(const void* nodes, const uint32_t* payload, const int64_t* idx, int64_t n, uint32_t* out,
+                             void* stream) noexcept
+{
+    if (!nodes || !payload || !idx || !out || n < 0) return FE_INVALID;
+    if (n == 0) return FE_OK;
+    hipLaunchKernelGGL(k_samples_at, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, static_cast<hipStream_t>(stream),
+                       static_cast<const Node52*>(nodes), payload, idx, n, out);
+    FE_TRY(hipGetLastError());
+    return FE_OK;
+}
+
+extern "C" int fe_query_packed(const void* nodes, const uint32_t* payload, int nx, int ny, int nz, const double* origin,
+                               double pitch, const double* pts, int64_t n, float* out, void* stream) noexcept
+{
+    if (!nodes || !payload || !origin || !pts || !out || n < 0 || nx < 1 || ny < 1 || nz < 1) return FE_INVALID;
+    if (n == 0) return FE_OK;
+    hipLaunchKernelGGL(k_query, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, static_cast<hipStream_t>(stream),
+                       static_cast<const Node52*>(nodes), payload, nx, ny, nz, origin[0], origin[1], origin[2], pitch,
+                       pts, n, out);
+    FE_TRY(hipGetLastError());
+    return FE_OK;
+}
+
+extern "C" int fe_unpack(const void* nodes, int64_t count, const uint32_t* payload, int nx, int ny, int nz, float* dense,
+                         void* stream) noexcept
+{
+    if (!nodes || !payload || !dense || count < 1 || nx < 1 || ny < 1 || nz < 1) return FE_INVALID;
+    hipLaunchKernelGGL(k_unpack, dim3((unsigned)count), dim3(256), 0, static_cast<hipStream_t>(stream),
+                       static_cast<const Node52*>(nodes), count, payload, nx, ny, nz, reinterpret_cast<uint32_t*>(dense));
+    FE_TRY(hipGetLastError());
+    return FE_OK;
+}
+
+// ---------------------------------------------------------------- reference-shaped host ABI
+// FE/adaptive_pack_native_v1/pack.cpp:57-79: pack a host field, keep the
+// result on the host until adaptive_copy, free with adaptive_free.
+extern "C" void* adaptive_pack(const void* data, const int32_t* shape, int threshold) noexcept
+{
+    try {
+        if (!data || !shape || threshold < 8 || threshold > 4096) return nullptr;
+        int64_t total = 1;
+        for (int a = 0; a < 3; ++a) {
+            if (shape[a] < 1 || shape[a] > 4096) return nullptr;
+            total *= shape[a];
+        }
+        if (total > 16777216) return nullptr;
+        float* d = nullptr;
+        if (hipMalloc(&d, (size_t)total * 4) != hipSuccess) return nullptr;
+        void* result = nullptr;
+        if (hipMemcpy(d, data, (size_t)total * 4, hipMemcpyHostToDevice) == hipSuccess) {
+            int status = 0;
+            Packed* p = pack_device(d, shape[0], shape[1], shape[2], threshold, nullptr, &status);
+            if (p) {
+                HostPacked* h = new HostPacked();
+                h->nodes.resize((size_t)p->n_nodes * 52);
+                h->payload.resize((size_t)p->n_samples);
+                if (hipMemcpy(h->nodes.data(), p->nodes, h->nodes.size(), hipMemcpyDeviceToHost) == hipSuccess &&
+                    (h->payload.empty() ||
+                     hipMemcpy(h->payload.data(), p->payload, h->payload.size() * 4, hipMemcpyDeviceToHost) == hipSuccess))
+                    result = h;
+                else
+                    delete h;
+                delete p;
+            }
+        }
+        (void)hipFree(d);
+        return result;
+    } catch (...) {
+        return nullptr;
+    }
+}
+
+extern "C" int64_t adaptive_nodes(const void* pointer) noexcept
+{
+    return pointer ? (int64_t)static_cast<const HostPacked*>(pointer)->nodes.size() / 52 : 0;
+}
+
+extern "C" int64_t adaptive_samples(const void* pointer) noexcept
+{
+    return pointer ? (int64_t)static_cast<const HostPacked*>(pointer)->payload.size() : 0;
+}
+
+extern "C" int adaptive_copy(const void* pointer, void* nodes, int64_t count, void* payload, int64_t samples) noexcept
+{
+    if (!pointer || !nodes || !payload) return 1;
+    const HostPacked& h = *static_cast<const HostPacked*>(pointer);
+    if (count != (int64_t)h.nodes.size() / 52 || samples != (int64_t)h.payload.size()) return 1;
+    std::memcpy(nodes, h.nodes.data(), h.nodes.size());
+    std::memcpy(payload, h.payload.data(), h.payload.size() * 4);
+    return 0;
+}
+
+extern "C" void adaptive_free(void* pointer) noexcept { delete static_cast<HostPacked*>(pointer); }

@@ -106,6 +106,57 @@ int fe_dense_tiles(const float *dense, int nx, int ny, int nz, int B, const int3
  * built field. */
 int fe_query_points(const void *desc, const float *points, int64_t n, float *out, void *stream);
 
+/* ---------------------------------------------------------------- leaf-box packing */
+/* Node records are 52 bytes {int32 lo[3], shape[3], stored[3], children[2];
+ * int64 offset} in pre-order, as adaptive_sample_blocks_v1.py:14-16. */
+
+/* Replace adaptive_pack / adaptive_nodes / adaptive_samples / adaptive_copy /
+ * adaptive_free, adaptive_pack_native_v1/pack.cpp:57-79: pack a host fp32
+ * (shape[0],shape[1],shape[2]) field (NULL on invalid or non-finite input);
+ * the split decisions run on the device, the result is held on the host. */
+void *adaptive_pack(const void *data, const int32_t *shape, int threshold);
+int64_t adaptive_nodes(const void *packed);
+int64_t adaptive_samples(const void *packed);
+int adaptive_copy(const void *packed, void *nodes, int64_t count, void *payload, int64_t samples);
+void adaptive_free(void *packed);
+
+/* Device-resident packing of a device fp32 field (same split decisions as
+ * adaptive_sample_blocks_v1.py:19-58); *status 1 on non-finite input, 2 on a
+ * runtime failure.  fe_pack_copy copies into device or host buffers. */
+void *fe_pack_device(const float *data, int nx, int ny, int nz, int leaf_samples, int *status, void *stream);
+int64_t fe_pack_nodes(const void *packed);
+int64_t fe_pack_samples(const void *packed);
+int fe_pack_copy(const void *packed, void *nodes, int64_t count, void *payload, int64_t samples, void *stream);
+void fe_pack_free(void *packed);
+
+/* samples_at and query of adaptive_sample_blocks_v1.py:103-151 on device tables:
+ * exact leaf lookups and the order-1 trilinear query (fp64, rounded to fp32). */
+int fe_samples_at(const void *nodes, const uint32_t *payload, const int64_t *indices, int64_t n, uint32_t *out,
+                  void *stream);
+int fe_query_packed(const void *nodes, const uint32_t *payload, int nx, int ny, int nz, const double *origin,
+                    double pitch, const double *points, int64_t n, float *out, void *stream);
+
+/* Dense device reconstruction of a packed field (one workgroup per leaf). */
+int fe_unpack(const void *nodes, int64_t count, const uint32_t *payload, int nx, int ny, int nz, float *dense,
+              void *stream);
+
+/* ---------------------------------------------------------------- tiled CSG */
+/* Replaces csg_gather, csg_tile_native_v1/gather.cpp:11 (codes 0-5): exact
+ * bits of every leaf intersecting [lower, upper) into a tile of <= 8192 samples. */
+int csg_gather(const void *nodes, int64_t count, const void *payload, int64_t samples, const int32_t *lower,
+               const int32_t *upper, void *output);
+
+/* Replaces csg_execute, csg_batch_native_v1/execute.cpp:10 (codes 0-5):
+ * ordered fp32 union/difference/offset program over packed operands; program
+ * records are 24 bytes {int32 op, left, right, pad; double offset}. */
+int csg_execute(const void *const *nodes, const int64_t *counts, const void *const *payloads, const int64_t *lengths,
+                int32_t fields, const int32_t *shape, const void *program, int32_t instructions, float *output);
+
+/* The same program over device-resident operands into a device dense output. */
+int fe_csg_device(const void *const *nodes, const int64_t *counts, const void *const *payloads,
+                  const int64_t *lengths, int32_t fields, const int32_t *shape, const void *program,
+                  int32_t instructions, float *output, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
