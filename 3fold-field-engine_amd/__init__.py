@@ -8,7 +8,7 @@ from ._lib import LIB_PATH, NativeLibraryMissing, exported_symbols
 
 _lib.lib()  # fail loudly at import when the HIP library is missing
 
-from .edt import (NativeEDT, NativeEDTLarge, NativeEDTVolume, PersistentPairEDT,  # noqa: E402
+from .edt import (EDTBatch, NativeEDT, NativeEDTLarge, NativeEDTVolume, PersistentPairEDT,  # noqa: E402
                   edt_device, signed_distance, signed_from_pair_device, squared_distance)
 
 from . import recipes, field, batch, server, blocks, csg  # noqa: E402
@@ -27,5 +27,5 @@ __all__ = ["recipes", "field", "batch", "server", "blocks", "csg", "NODE", "Adap
            "reconstruct_solid_from_sparse", "volym_via_gpu_reduktion", "determinism_cert", "konservering_check",
            "klassificera_och_evaluera_fran_tatt_falt",
            "LIB_PATH", "NativeLibraryMissing", "exported_symbols", "NativeEDT", "NativeEDTLarge",
-           "NativeEDTVolume", "PersistentPairEDT", "edt_device", "signed_distance", "signed_from_pair_device",
+           "NativeEDTVolume", "PersistentPairEDT", "EDTBatch", "edt_device", "signed_distance", "signed_from_pair_device",
            "squared_distance"]

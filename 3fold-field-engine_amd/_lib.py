@@ -27,6 +27,9 @@ SIGNATURES = {
     "fe_edt_device": (ct.c_int, [P, ct.c_int, ct.c_int, ct.c_int, ct.c_int, P, P, P]),
     "fe_edt_signed_device": (ct.c_int, [P, I64, F64, P, P]),
     "fe_edt_axis_pass": (ct.c_int, [P, P, ct.c_int, ct.c_int, ct.c_int, ct.c_int, ct.c_int, P]),
+    "fe_edt_batch_create": (P, [P, P, P, P, ct.c_int, ct.c_int, I64, I64, P]),
+    "fe_edt_batch_run": (ct.c_int, [P, P, P, P, P]),
+    "fe_edt_batch_destroy": (None, [P]),
     # field build
     "fe_field_build": (ct.c_int, [P, ct.c_int, ct.c_int, P, ct.c_int, P, ct.c_int, P, ct.c_int, P]),
     "fe_field_classify": (ct.c_int, [P, ct.c_int, ct.c_int, P, ct.c_int, P]),

@@ -20,10 +20,12 @@
 //   signed  FE/edt_persistent_pair_v1/edt.cu:36-42 semantics with IEEE
 //           correctly rounded f64 sqrt/mul and f32 sub/mul.
 #include "fe_common.hpp"
+#include <algorithm>
 #include <climits>
 #include <cstdio>
 #include <cmath>
 #include <cstring>
+#include <utility>
 #include <vector>
 
 static thread_local char fe_err_buf[256];
@@ -36,26 +38,66 @@ void fe_note_error(hipError_t e, const char* where)
 // Text of the last runtime failure on this host thread ("" if none).
 extern "C" const char* fe_last_error() noexcept { return fe_err_buf; }
 
+// One volume of a launch.  A launch covers a table of volumes in two arenas:
+// a byte arena of masks and int32 arenas of planes (out and work share the
+// layout).  Tiles of all volumes are numbered consecutively per pass; the
+// volume of a tile is the last one whose first tile is <= it.
+struct EdtVol {
+    int64_t mask;       // byte offset of the nx*ny*nz mask
+    int64_t data;       // element offset of plane 0
+    int64_t plane;      // elements from one plane to the next (>= nx*ny*nz)
+    int32_t nx, ny, nz, sentinel;
+    uint32_t first[3];  // first tile of the volume in the z, y and x pass
+    uint32_t pad;
+};
+static_assert(sizeof(EdtVol) == 56, "EdtVol layout");
+
 namespace {
 
-// ---------------------------------------------------------------- z pass
-// rows: number of (x,y) rows; sites of plane 0 are voxels whose mask equals
-// site_one (1) or is zero (0); plane 1 (PAIR) takes the complement.
-template <bool PAIR>
-__global__ __launch_bounds__(256) void k_edt_zpass(const uint8_t* __restrict__ mask, int32_t* __restrict__ out,
-                                                   int64_t rows, int nz, int64_t plane, int32_t sentinel, int site_one)
+// Volume of `tile` in pass `pass` (wave-uniform): the ascending first tiles
+// are read 64 at a time and counted with one ballot.
+__device__ __forceinline__ int find_vol(const EdtVol* __restrict__ table, int nvol, int pass, uint32_t tile)
 {
     const int lane = threadIdx.x & 63;
-    const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    for (int base = 0;; base += 64) {
+        const int v = base + lane;
+        const bool le = v < nvol && table[v].first[pass] <= tile;
+        const int cnt = __popcll(__ballot(le));
+        if (cnt < 64 || base + 64 >= nvol) return __builtin_amdgcn_readfirstlane(base + cnt - 1);
+    }
+}
+
+__device__ __forceinline__ EdtVol load_vol(const EdtVol* __restrict__ table, int nvol, const EdtVol& one, int pass,
+                                           uint32_t tile)
+{
+    return nvol ? table[find_vol(table, nvol, pass, tile)] : one;
+}
+
+// ---------------------------------------------------------------- z pass
+// Four rows (one per wave) per block; sites of plane 0 are voxels whose mask
+// equals site_one (1) or is zero (0); plane 1 (PAIR) takes the complement.
+template <bool PAIR>
+__global__ __launch_bounds__(256) void k_edt_zpass(const uint8_t* __restrict__ masks, int32_t* __restrict__ outs,
+                                                   const EdtVol* __restrict__ table, int nvol, EdtVol one,
+                                                   int site_one)
+{
+    const int lane = threadIdx.x & 63;
+    const EdtVol V = load_vol(table, nvol, one, 0, blockIdx.x);
+    const int64_t rows = (int64_t)V.nx * V.ny;
+    const int64_t row = (int64_t)(blockIdx.x - V.first[0]) * 4 + (threadIdx.x >> 6);
     if (row >= rows) return;  // uniform per wave
-    const uint8_t* m = mask + row * nz;
+    const int nz = V.nz;
+    const int64_t plane = V.plane;
+    const int32_t sentinel = V.sentinel;
+    int32_t* out = outs + V.data;
+    const uint8_t* m = masks + V.mask + row * nz;
     const int nseg = (nz + 63) >> 6;
     uint64_t w0 = 0, valid = 0;
     for (int s = 0; s < nseg; ++s) {
         int p = (s << 6) + lane;
         bool in = p < nz;
-        bool one = in && m[p] != 0;
-        uint64_t b = __ballot(site_one ? one : (in && !one));
+        bool hit = in && m[p] != 0;
+        uint64_t b = __ballot(site_one ? hit : (in && !hit));
         uint64_t v = __ballot(in);
         if (lane == s) { w0 = b; valid = v; }
     }
@@ -122,19 +164,30 @@ __device__ __forceinline__ bool keep_mid(const int32_t* f, int a, int b, int c)
 // the sentinel (no site reached yet) are left out of the hulls: such a site
 // contributes sentinel + (p-s)^2 >= sentinel, with equality only at its own
 // position, so the result is min(envelope of the other sites, sentinel).
+// axis 1: lines along y (outer index x); axis 0: lines along x (outer index y).
 template <int W, int T, int CMAX>
-__global__ __launch_bounds__(W* T) void k_edt_lines(const int32_t* __restrict__ src, int32_t* __restrict__ dst, int L,
-                                                    int64_t axis_stride, int nz, int ztiles, int64_t outer_stride,
-                                                    int outer_count, int64_t plane_stride, uint32_t tiles,
-                                                    uint32_t grid, int32_t sentinel)
+__global__ __launch_bounds__(W* T) void k_edt_lines(const int32_t* __restrict__ src, int32_t* __restrict__ dst,
+                                                    const EdtVol* __restrict__ table, int nvol, EdtVol one, int axis,
+                                                    uint32_t tiles, uint32_t grid)
 {
     extern __shared__ int32_t lds[];
     const uint32_t tile = xcd_tile(blockIdx.x, grid);
     if (tile >= tiles) return;  // uniform per block
-    const int zt = tile % ztiles;
-    const int rest = tile / ztiles;
+    const int pass = axis == 1 ? 1 : 2;
+    const EdtVol V = load_vol(table, nvol, one, pass, tile);
+    const uint32_t local = tile - V.first[pass];
+    const int nz = V.nz;
+    const int32_t sentinel = V.sentinel;
+    const int64_t yz = (int64_t)V.ny * nz;
+    const int L = axis == 1 ? V.ny : V.nx;
+    const int64_t axis_stride = axis == 1 ? (int64_t)nz : yz;
+    const int64_t outer_stride = axis == 1 ? yz : (int64_t)nz;
+    const int outer_count = axis == 1 ? V.nx : V.ny;
+    const int ztiles = (nz + W - 1) / W;
+    const int zt = local % ztiles;
+    const int rest = local / ztiles;
     const int outer = rest % outer_count, plane = rest / outer_count;
-    const int64_t base = (int64_t)plane * plane_stride + (int64_t)outer * outer_stride + (int64_t)zt * W;
+    const int64_t base = V.data + (int64_t)plane * V.plane + (int64_t)outer * outer_stride + (int64_t)zt * W;
     const int nlines = min(W, nz - zt * W);
     const int LP = L + 1;
     constexpr int NT = W * T;
@@ -147,7 +200,8 @@ __global__ __launch_bounds__(W* T) void k_edt_lines(const int32_t* __restrict__ 
     int32_t* lval = reinterpret_cast<int32_t*>(rsb + NT);      // [W] value of chunk 0's first sample
     int32_t* lflat = lval + W;                                 // [W] line is constant
     const int tid = threadIdx.x;
-    const bool vec = (nz & 3) == 0;  // then rows of the tile are 16-byte aligned
+    // rows of the tile are 16-byte aligned when nz and the volume's offsets are multiples of 4
+    const bool vec = ((nz | V.data | V.plane) & 3) == 0;
 
     if (vec) {
         for (int i = tid; i < (W / 4) * L; i += NT) {
@@ -404,31 +458,76 @@ LineCfg pick_cfg(int L)
 }
 
 template <int W, int T, int CMAX>
-hipError_t launch_lines_t(const int32_t* src, int32_t* dst, int L, int64_t axis_stride, int nz, int64_t outer_stride,
-                          int outer_count, int planes, int64_t plane_stride, int32_t sentinel, hipStream_t st)
+hipError_t launch_lines_t(const int32_t* src, int32_t* dst, const EdtVol* table, int nvol, const EdtVol& one,
+                          int axis, int Lmax, uint64_t tiles, hipStream_t st)
 {
-    const int ztiles = (nz + W - 1) / W;
-    const uint64_t tiles = (uint64_t)ztiles * outer_count * planes;
     const uint32_t grid = round8(tiles);
-    const size_t lds = (size_t)W * (L + 1) * 4 + (size_t)CMAX * W * T * 2 + (size_t)W * T * 6 + (size_t)W * 8;
+    const size_t lds = (size_t)W * (Lmax + 1) * 4 + (size_t)CMAX * W * T * 2 + (size_t)W * T * 6 + (size_t)W * 8;
     if (lds > 65536) {
         hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_edt_lines<W, T, CMAX>),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL((k_edt_lines<W, T, CMAX>), dim3(grid), dim3(W * T), lds, st, src, dst, L, axis_stride, nz,
-                       ztiles, outer_stride, outer_count, plane_stride, (uint32_t)tiles, grid, sentinel);
+    hipLaunchKernelGGL((k_edt_lines<W, T, CMAX>), dim3(grid), dim3(W * T), lds, st, src, dst, table, nvol, one, axis,
+                       (uint32_t)tiles, grid);
     return hipGetLastError();
 }
 
-hipError_t launch_lines(const int32_t* src, int32_t* dst, int L, int64_t axis_stride, int nz, int64_t outer_stride,
-                        int outer_count, int planes, int64_t plane_stride, int32_t sentinel, hipStream_t st)
+// One line pass over every volume: Lmax (the longest line of the pass over
+// all volumes) picks the tile shape, `tiles` counts the tiles of that shape.
+hipError_t launch_lines(const int32_t* src, int32_t* dst, const EdtVol* table, int nvol, const EdtVol& one, int axis,
+                        int Lmax, uint64_t tiles, hipStream_t st)
 {
-    LineCfg c = pick_cfg(L);
-    if (c.W == 32) return launch_lines_t<32, 8, 16>(src, dst, L, axis_stride, nz, outer_stride, outer_count, planes, plane_stride, sentinel, st);
-    if (c.W == 16) return launch_lines_t<16, 32, 16>(src, dst, L, axis_stride, nz, outer_stride, outer_count, planes, plane_stride, sentinel, st);
-    if (c.W == 8) return launch_lines_t<8, 32, 32>(src, dst, L, axis_stride, nz, outer_stride, outer_count, planes, plane_stride, sentinel, st);
-    return launch_lines_t<4, 64, 64>(src, dst, L, axis_stride, nz, outer_stride, outer_count, planes, plane_stride, sentinel, st);
+    LineCfg c = pick_cfg(Lmax);
+    if (c.W == 32) return launch_lines_t<32, 8, 16>(src, dst, table, nvol, one, axis, Lmax, tiles, st);
+    if (c.W == 16) return launch_lines_t<16, 32, 16>(src, dst, table, nvol, one, axis, Lmax, tiles, st);
+    if (c.W == 8) return launch_lines_t<8, 32, 32>(src, dst, table, nvol, one, axis, Lmax, tiles, st);
+    return launch_lines_t<4, 64, 64>(src, dst, table, nvol, one, axis, Lmax, tiles, st);
+}
+
+int64_t edt_sentinel(int nx, int ny, int nz)
+{
+    return (int64_t)(nx - 1) * (nx - 1) + (int64_t)(ny - 1) * (ny - 1) + (int64_t)(nz - 1) * (nz - 1) + 1;
+}
+
+// Tile counts of one volume in the three passes for line tiles of Wy and Wx lines.
+void vol_tiles(int nx, int ny, int nz, int planes, int Wy, int Wx, uint64_t t[3])
+{
+    t[0] = ((uint64_t)nx * ny + 3) / 4;
+    t[1] = (uint64_t)((nz + Wy - 1) / Wy) * nx * planes;
+    t[2] = (uint64_t)((nz + Wx - 1) / Wx) * ny * planes;
+}
+
+// The three passes over a table (nvol > 0, device table) or one volume (nvol == 0).
+// z pass into `out`, y pass into `work`, x pass back into `out`.  planes == 1
+// sites: mask == 0; planes == 2, plane 0 sites: mask != 0.
+hipError_t run_passes(const uint8_t* masks, int32_t* work, int32_t* out, const EdtVol* table, int nvol,
+                      const EdtVol& one, int planes, int Lmax_y, int Lmax_x, const uint64_t tiles[3], hipStream_t st)
+{
+    const int site_one = planes == 2 ? 1 : 0;
+    if (planes == 2)
+        hipLaunchKernelGGL(k_edt_zpass<true>, dim3((uint32_t)tiles[0]), dim3(256), 0, st, masks, out, table, nvol, one,
+                           site_one);
+    else
+        hipLaunchKernelGGL(k_edt_zpass<false>, dim3((uint32_t)tiles[0]), dim3(256), 0, st, masks, out, table, nvol,
+                           one, site_one);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    if ((e = launch_lines(out, work, table, nvol, one, 1, Lmax_y, tiles[1], st)) != hipSuccess) return e;
+    return launch_lines(work, out, table, nvol, one, 0, Lmax_x, tiles[2], st);
+}
+
+EdtVol single_vol(int nx, int ny, int nz, int64_t plane)
+{
+    EdtVol v{};
+    v.mask = 0;
+    v.data = 0;
+    v.plane = plane;
+    v.nx = nx;
+    v.ny = ny;
+    v.nz = nz;
+    v.sentinel = (int32_t)edt_sentinel(nx, ny, nz);
+    return v;
 }
 
 }  // namespace
@@ -449,24 +548,10 @@ extern "C" int fe_edt_device(const uint8_t* mask, int nx, int ny, int nz, int pl
     if (!mask || !work || !out || nx < 1 || ny < 1 || nz < 1 || nx > FE_EDT_MAX_AXIS || ny > FE_EDT_MAX_AXIS ||
         nz > FE_EDT_MAX_AXIS || (planes != 1 && planes != 2))
         return FE_INVALID;
-    hipStream_t st = static_cast<hipStream_t>(stream);
-    const int64_t n = (int64_t)nx * ny * nz;
-    const int64_t sentinel64 = (int64_t)(nx - 1) * (nx - 1) + (int64_t)(ny - 1) * (ny - 1) + (int64_t)(nz - 1) * (nz - 1) + 1;
-    const int32_t sentinel = (int32_t)sentinel64;
-    const int64_t rows = (int64_t)nx * ny;
-    // z pass into `out`, y pass into `work`, x pass back into `out`.
-    // planes == 1 sites: mask == 0; planes == 2, plane 0 sites: mask != 0.
-    const int site_one = planes == 2 ? 1 : 0;
-    const unsigned zgrid = (unsigned)((rows + 3) / 4);
-    if (planes == 2)
-        hipLaunchKernelGGL(k_edt_zpass<true>, dim3(zgrid), dim3(256), 0, st, mask, out, rows, nz, n, sentinel, site_one);
-    else
-        hipLaunchKernelGGL(k_edt_zpass<false>, dim3(zgrid), dim3(256), 0, st, mask, out, rows, nz, n, sentinel, site_one);
-    FE_TRY(hipGetLastError());
-    // y lines: outer = x (stride ny*nz), axis stride nz
-    FE_TRY(launch_lines(out, work, ny, nz, nz, (int64_t)ny * nz, nx, planes, n, sentinel, st));
-    // x lines: outer = y (stride nz), axis stride ny*nz
-    FE_TRY(launch_lines(work, out, nx, (int64_t)ny * nz, nz, nz, ny, planes, n, sentinel, st));
+    const EdtVol one = single_vol(nx, ny, nz, (int64_t)nx * ny * nz);
+    uint64_t tiles[3];
+    vol_tiles(nx, ny, nz, planes, pick_cfg(ny).W, pick_cfg(nx).W, tiles);
+    FE_TRY(run_passes(mask, work, out, nullptr, 0, one, planes, ny, nx, tiles, static_cast<hipStream_t>(stream)));
     return FE_OK;
 }
 
@@ -479,26 +564,131 @@ extern "C" int fe_edt_axis_pass(const void* src, int32_t* dst, int nx, int ny, i
         nz > FE_EDT_MAX_AXIS || (planes != 1 && planes != 2) || axis < 0 || axis > 2)
         return FE_INVALID;
     hipStream_t st = static_cast<hipStream_t>(stream);
-    const int64_t n = (int64_t)nx * ny * nz;
+    const EdtVol one = single_vol(nx, ny, nz, (int64_t)nx * ny * nz);
+    uint64_t tiles[3];
+    vol_tiles(nx, ny, nz, planes, pick_cfg(ny).W, pick_cfg(nx).W, tiles);
     if (axis == 2) {
-        const int32_t sentinel =
-            (int32_t)((int64_t)(nx - 1) * (nx - 1) + (int64_t)(ny - 1) * (ny - 1) + (int64_t)(nz - 1) * (nz - 1) + 1);
-        const int64_t rows = (int64_t)nx * ny;
-        const unsigned zgrid = (unsigned)((rows + 3) / 4);
         const uint8_t* m = static_cast<const uint8_t*>(src);
         if (planes == 2)
-            hipLaunchKernelGGL(k_edt_zpass<true>, dim3(zgrid), dim3(256), 0, st, m, dst, rows, nz, n, sentinel, 1);
+            hipLaunchKernelGGL(k_edt_zpass<true>, dim3((uint32_t)tiles[0]), dim3(256), 0, st, m, dst, nullptr, 0, one, 1);
         else
-            hipLaunchKernelGGL(k_edt_zpass<false>, dim3(zgrid), dim3(256), 0, st, m, dst, rows, nz, n, sentinel, 0);
+            hipLaunchKernelGGL(k_edt_zpass<false>, dim3((uint32_t)tiles[0]), dim3(256), 0, st, m, dst, nullptr, 0, one, 0);
         FE_TRY(hipGetLastError());
         return FE_OK;
     }
     const int32_t* a = static_cast<const int32_t*>(src);
-    const int32_t sent =
-        (int32_t)((int64_t)(nx - 1) * (nx - 1) + (int64_t)(ny - 1) * (ny - 1) + (int64_t)(nz - 1) * (nz - 1) + 1);
-    if (axis == 1) FE_TRY(launch_lines(a, dst, ny, nz, nz, (int64_t)ny * nz, nx, planes, n, sent, st));
-    else FE_TRY(launch_lines(a, dst, nx, (int64_t)ny * nz, nz, nz, ny, planes, n, sent, st));
+    if (axis == 1) FE_TRY(launch_lines(a, dst, nullptr, 0, one, 1, ny, tiles[1], st));
+    else FE_TRY(launch_lines(a, dst, nullptr, 0, one, 0, nx, tiles[2], st));
     return FE_OK;
+}
+
+// ---------------------------------------------------------------- batched volumes
+namespace {
+struct EdtBatch {
+    EdtVol* table = nullptr;  // device
+    int nvol = 0, planes = 0, Lmax_y = 0, Lmax_x = 0;
+    uint64_t tiles[3] = {0, 0, 0};
+};
+}  // namespace
+
+// A batch of nvol volumes sharing one launch per axis pass.  dims: host int32
+// [nvol][3]; mask_off: byte offset of each mask in the mask arena (mask_bytes
+// long); data_off / plane_stride: element offset of each volume's plane 0 and
+// the distance between its planes in the int32 arenas (data_elems long; out
+// and work use the same layout).  Volumes' plane ranges must not overlap.
+// Offsets that are multiples of 4 (and nz % 4 == 0) take the 16-byte vector
+// path.  Returns NULL with *status 1 (invalid) or 2 (runtime).
+extern "C" void* fe_edt_batch_create(const int32_t* dims, const int64_t* mask_off, const int64_t* data_off,
+                                     const int64_t* plane_stride, int nvol, int planes, int64_t mask_bytes,
+                                     int64_t data_elems, int* status) noexcept
+{
+    int dummy = 0;
+    int& st = status ? *status : dummy;
+    st = FE_INVALID;
+    if (!dims || !mask_off || !data_off || !plane_stride || nvol < 1 || nvol > (1 << 20) ||
+        (planes != 1 && planes != 2) || mask_bytes < 1 || data_elems < 1)
+        return nullptr;
+    EdtBatch* b = nullptr;
+    try {
+        std::vector<EdtVol> host((size_t)nvol);
+        std::vector<std::pair<int64_t, int64_t>> ranges((size_t)nvol);
+        int Ly = 1, Lx = 1;
+        for (int v = 0; v < nvol; ++v) {
+            const int nx = dims[3 * v], ny = dims[3 * v + 1], nz = dims[3 * v + 2];
+            if (nx < 1 || ny < 1 || nz < 1 || nx > FE_EDT_MAX_AXIS || ny > FE_EDT_MAX_AXIS || nz > FE_EDT_MAX_AXIS)
+                return nullptr;
+            const int64_t n = (int64_t)nx * ny * nz;
+            if (mask_off[v] < 0 || mask_off[v] > mask_bytes - n || data_off[v] < 0 || plane_stride[v] < n ||
+                data_off[v] + (planes - 1) * plane_stride[v] > data_elems - n)
+                return nullptr;
+            host[v] = single_vol(nx, ny, nz, plane_stride[v]);
+            host[v].mask = mask_off[v];
+            host[v].data = data_off[v];
+            ranges[v] = {data_off[v], data_off[v] + (planes - 1) * plane_stride[v] + n};
+            Ly = std::max(Ly, ny);
+            Lx = std::max(Lx, nx);
+        }
+        std::sort(ranges.begin(), ranges.end());
+        for (size_t i = 1; i < ranges.size(); ++i)
+            if (ranges[i].first < ranges[i - 1].second) return nullptr;
+        const int Wy = pick_cfg(Ly).W, Wx = pick_cfg(Lx).W;
+        uint64_t tot[3] = {0, 0, 0};
+        for (int v = 0; v < nvol; ++v) {
+            uint64_t t[3];
+            vol_tiles(host[v].nx, host[v].ny, host[v].nz, planes, Wy, Wx, t);
+            for (int p = 0; p < 3; ++p) {
+                host[v].first[p] = (uint32_t)tot[p];
+                tot[p] += t[p];
+            }
+        }
+        for (int p = 0; p < 3; ++p)
+            if (tot[p] >= 0x7fffffffull) return nullptr;
+        st = FE_RUNTIME;
+        b = new EdtBatch();
+        b->nvol = nvol;
+        b->planes = planes;
+        b->Lmax_y = Ly;
+        b->Lmax_x = Lx;
+        for (int p = 0; p < 3; ++p) b->tiles[p] = tot[p];
+        hipError_t e = hipMalloc(&b->table, sizeof(EdtVol) * (size_t)nvol);
+        if (e == hipSuccess) e = hipMemcpy(b->table, host.data(), sizeof(EdtVol) * (size_t)nvol, hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            fe_note_error(e, "fe_edt_batch_create");
+            if (b->table) (void)hipFree(b->table);
+            delete b;
+            return nullptr;
+        }
+        st = FE_OK;
+        return b;
+    } catch (...) {
+        if (b) {
+            if (b->table) (void)hipFree(b->table);
+            delete b;
+        }
+        st = FE_RUNTIME;
+        return nullptr;
+    }
+}
+
+// The three passes of every volume of the batch: three kernel launches on
+// `stream`, no allocation, copy or synchronisation (capturable in a graph).
+extern "C" int fe_edt_batch_run(const void* handle, const uint8_t* masks, int32_t* work, int32_t* out,
+                                void* stream) noexcept
+{
+    const auto* b = static_cast<const EdtBatch*>(handle);
+    if (!b || !masks || !work || !out) return FE_INVALID;
+    const EdtVol none{};
+    FE_TRY(run_passes(masks, work, out, b->table, b->nvol, none, b->planes, b->Lmax_y, b->Lmax_x, b->tiles,
+                      static_cast<hipStream_t>(stream)));
+    return FE_OK;
+}
+
+extern "C" void fe_edt_batch_destroy(void* handle) noexcept
+{
+    auto* b = static_cast<EdtBatch*>(handle);
+    if (!b) return;
+    if (b->table) (void)hipFree(b->table);
+    delete b;
 }
 
 // Signed distance from a device pair (2N int32) into N float32.

@@ -116,3 +116,87 @@ def batch_recipes(seed=3, count=256):
         n = int(rng.choice(np.arange(64, 385, 32)))
         out.append((mixed_recipe(rng, p, n), n))
     return out
+
+
+# ---------------------------------------------------------------- obstacle volumes (EDT workloads)
+def obstacle_volume(shape, rng, count, scale=100.0):
+    """uint8 occupancy of `count` seeded axis-aligned obstacles, in voxel units.
+
+    Kinds (probabilities): solid box 0.35 (edges log-uniform in [2, scale]);
+    1-voxel-thick wall 0.25 (the other two edges log-uniform in [8, 2*scale]);
+    capped cylinder along a random axis 0.25 (radius log-uniform in
+    [1, scale/3], length log-uniform in [4, 2*scale]; voxel (i,j) of the
+    cross-section is inside when (i-ci)^2 + (j-cj)^2 <= r^2); split box 0.15,
+    a solid box with a 1-voxel gap cut through its middle along a random axis.
+    Obstacles are painted in order (later gaps clear earlier obstacles).  At
+    scale 100 the occupancy is about 12% for any volume when `count` is
+    proportional to the voxel count (4000 at 768^3)."""
+    n = np.asarray(shape, dtype=np.int64)
+    vol = np.zeros(tuple(int(x) for x in shape), np.uint8)
+    kinds = rng.choice(4, size=count, p=[0.35, 0.25, 0.25, 0.15])
+    log = np.log
+    for k in kinds:
+        if k in (0, 3):
+            e = np.rint(np.exp(rng.uniform(log(2), log(scale), 3))).astype(np.int64)
+            lo = (rng.uniform(0, 1, 3) * n).astype(np.int64) - e // 2
+            hi = np.minimum(lo + e, n)
+            lo = np.maximum(lo, 0)
+            vol[lo[0]:hi[0], lo[1]:hi[1], lo[2]:hi[2]] = 1
+            if k == 3:
+                ax = int(rng.integers(3))
+                if hi[ax] - lo[ax] >= 3:
+                    g = (lo[ax] + hi[ax]) // 2
+                    sl = [slice(lo[0], hi[0]), slice(lo[1], hi[1]), slice(lo[2], hi[2])]
+                    sl[ax] = slice(g, g + 1)
+                    vol[tuple(sl)] = 0
+        elif k == 1:
+            ax = int(rng.integers(3))
+            e = np.rint(np.exp(rng.uniform(log(8), log(2 * scale), 3))).astype(np.int64)
+            e[ax] = 1
+            lo = (rng.uniform(0, 1, 3) * n).astype(np.int64) - e // 2
+            hi = np.minimum(lo + e, n)
+            lo = np.maximum(lo, 0)
+            vol[lo[0]:hi[0], lo[1]:hi[1], lo[2]:hi[2]] = 1
+        else:
+            ax = int(rng.integers(3))
+            r = float(np.exp(rng.uniform(0.0, log(scale / 3))))
+            length = int(np.rint(np.exp(rng.uniform(log(4), log(2 * scale)))))
+            c = (rng.uniform(0, 1, 3) * n).astype(np.int64)
+            ri = int(np.ceil(r))
+            lo, hi = c - ri, c + ri + 1
+            lo[ax] = c[ax] - length // 2
+            hi[ax] = lo[ax] + length
+            lo, hi = np.maximum(lo, 0), np.minimum(hi, n)
+            if np.any(hi <= lo):
+                continue
+            idx = [np.arange(lo[a], hi[a]) - c[a] for a in range(3)]
+            idx[ax] = np.zeros(1, np.int64)
+            d2 = idx[0][:, None, None] ** 2 + idx[1][None, :, None] ** 2 + idx[2][None, None, :] ** 2
+            disk = (d2 <= r * r).astype(np.uint8)
+            sl = tuple(slice(lo[a], hi[a]) for a in range(3))
+            vol[sl] |= np.broadcast_to(disk, tuple(int(x) for x in hi - lo))
+    return vol
+
+
+def obstacle_count(shape, per_768=4000):
+    """Obstacle count proportional to the voxel count (4000 at 768^3)."""
+    return max(1, int(round(per_768 * float(np.prod(shape)) / 768.0 ** 3)))
+
+
+def thin_feature_volume_768(seed=4):
+    """768^3 from 4000 seeded obstacles (about 12% occupied)."""
+    return obstacle_volume((768, 768, 768), np.random.default_rng(seed), 4000)
+
+
+def thin_feature_shapes(seed=4, count=64, lo=96, hi=256):
+    """Edges of `count` uneven volumes, each axis uniform in [lo, hi]."""
+    rng = np.random.default_rng([seed, 1])
+    return [tuple(int(x) for x in rng.integers(lo, hi + 1, 3)) for _ in range(count)]
+
+
+def thin_feature_batch(seed=4, count=64, lo=96, hi=256):
+    """`count` volumes of the 768^3 construction with edges in [lo, hi]."""
+    out = []
+    for i, shape in enumerate(thin_feature_shapes(seed, count, lo, hi)):
+        out.append(obstacle_volume(shape, np.random.default_rng([seed, 2, i]), obstacle_count(shape)))
+    return out

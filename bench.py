@@ -95,6 +95,73 @@ def bench_batch(fe, torch, count, plain_sample):
     return res
 
 
+def bench_edt_heavy(fe, torch, reps=5, plain_sample=16):
+    """Thin-feature obstacle volumes (seed 4): the 768^3 paired EDT, then 64
+    uneven volumes (edges 96..256) in one graph launch (three kernels for the
+    whole batch) against sequential per-volume calls."""
+    dev = torch.device("cuda:0")
+    vol = fe.recipes.thin_feature_volume_768(4)
+    occ = float(np.count_nonzero(vol)) / vol.size
+    mask = torch.from_numpy(vol).to(dev)
+    del vol
+    n = mask.numel()
+    pair = torch.empty((2,) + tuple(mask.shape), dtype=torch.int32, device=dev)
+    work = torch.empty_like(pair)
+    L = fe._lib.lib()
+    st = fe._lib.stream_handle(None)
+    edt_ms = event_ms(torch, lambda: fe.edt_device(mask, 2, pair, work), reps)
+    s = tuple(mask.shape)
+    pz = event_ms(torch, lambda: L.fe_edt_axis_pass(fe._lib.ptr(mask), fe._lib.ptr(work), *s, 2, 2, st), reps)
+    py = event_ms(torch, lambda: L.fe_edt_axis_pass(fe._lib.ptr(work), fe._lib.ptr(pair), *s, 2, 1, st), reps)
+    px = event_ms(torch, lambda: L.fe_edt_axis_pass(fe._lib.ptr(pair), fe._lib.ptr(work), *s, 2, 0, st), reps)
+    out = {"volume_768": {"occupancy_fraction": occ, "pair_ms": edt_ms, "edt_mvox_s": n / (edt_ms * 1e-3) / 1e6,
+                          "pass_ms": {"z": pz, "y": py, "x": px},
+                          "pair_gb_s": 53 * n / (edt_ms * 1e-3) / 1e9}}
+    del mask, pair, work
+    torch.cuda.empty_cache()
+
+    vols = fe.recipes.thin_feature_batch(4, 64)
+    b = fe.EDTBatch([v.shape for v in vols], planes=2)
+    b.set_masks(vols)
+    graph = b.capture()
+    batch_ms = event_ms(torch, graph.replay, reps)
+    vox = b.voxels
+    # plain: one device-resident call per volume, outputs preallocated
+    bufs = [(b.mask(i).contiguous(), torch.empty((2,) + v.shape, dtype=torch.int32, device=dev),
+             torch.empty((2,) + v.shape, dtype=torch.int32, device=dev)) for i, v in enumerate(vols)]
+
+    def plain():
+        for m, o, w in bufs:
+            fe.edt_device(m, 2, o, w)
+    plain_ms = event_ms(torch, plain, reps)
+    del bufs
+    # reference-shaped per-call path (host mask in, signed field and squares out) on a sample
+    idx = list(range(0, len(vols), max(1, len(vols) // plain_sample)))[:plain_sample]
+    handles = [fe.PersistentPairEDT(vols[i].shape, capped=False) for i in idx]
+    masks = [vols[i].astype(bool) for i in idx]
+    for h, m in zip(handles, masks):
+        h.query(m, 1.0, with_squared=True)
+    t0 = time.perf_counter()
+    for h, m in zip(handles, masks):
+        h.query(m, 1.0, with_squared=True)
+    host_ms = (time.perf_counter() - t0) * 1e3 * len(vols) / len(idx)
+    for h in handles:
+        h.close()
+    out["batch_64"] = {"volumes": len(vols), "voxels": vox,
+                       "occupancy_fraction": float(sum(np.count_nonzero(v) for v in vols)) / vox,
+                       "batch_ms": batch_ms, "batch_volumes_s": len(vols) / (batch_ms * 1e-3),
+                       "batch_mvox_s": vox / (batch_ms * 1e-3) / 1e6,
+                       "plain_device_ms": plain_ms, "plain_device_volumes_s": len(vols) / (plain_ms * 1e-3),
+                       "plain_host_ms": host_ms, "plain_host_volumes_s": len(vols) / (host_ms * 1e-3),
+                       "speedup_vs_plain_device": plain_ms / batch_ms, "speedup_vs_plain_host": host_ms / batch_ms,
+                       "plain_host_note": f"PersistentPairEDT.query per volume (host copies included), "
+                                          f"sample of {len(idx)} volumes scaled to 64"}
+    b.close()
+    del graph
+    torch.cuda.empty_cache()
+    return out
+
+
 def bench_serve(fe, torch, rates, count):
     """Poisson arrivals of 512^3 / 1024-primitive recipes into the field server."""
     rng = np.random.default_rng(5)
@@ -129,6 +196,7 @@ def main():
     ap.add_argument("--quick", action="store_true", help="single-field step only")
     ap.add_argument("--batch-count", type=int, default=256)
     ap.add_argument("--serve-count", type=int, default=48)
+    ap.add_argument("--only", choices=["edt-heavy"], default=None, help="run one section only")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU: the block-sparse build and the EDT run on one "
@@ -137,6 +205,9 @@ def main():
     import torch
     fe = load_package()
     dev = torch.device("cuda:0")
+    if args.only == "edt-heavy":
+        print(json.dumps({"edt_heavy": bench_edt_heavy(fe, torch)}))
+        return 0
     n = args.n
     rec = fe.recipes.large_field_512(seed=2) if n == 512 else fe.recipes.mixed_recipe(np.random.default_rng(2), 1024, n)
     meta = fe.GridMeta(0.0, 0.0, 0.0, 1.0, n, n, n, block=8)
@@ -217,6 +288,7 @@ def main():
     if not args.quick:
         out["batched"] = bench_batch(fe, torch, args.batch_count, plain_sample=32)
         out["served"] = bench_serve(fe, torch, (200.0, 2000.0), args.serve_count)
+        out["edt_heavy"] = bench_edt_heavy(fe, torch)
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", "16"))
         out["cpu_baseline"] = cpu_baseline(fe, rec, occ_host, threads)

@@ -59,6 +59,22 @@ int fe_edt_device(const uint8_t *mask, int nx, int ny, int nz, int planes, int32
  * edt_native_linear_v1/edt.cu:13-29 over int32 planes. */
 int fe_edt_axis_pass(const void *src, int32_t *dst, int nx, int ny, int nz, int planes, int axis, void *stream);
 
+/* Batched EDT of many uneven volumes (no reference counterpart; it replaces
+ * one edt_pair_query / edt_squared_large call per volume,
+ * edt_persistent_pair_v1/edt.cu:59-78, with one launch per axis pass over all
+ * volumes).  dims: host int32 [nvol][3]; mask_off: byte offset of each
+ * volume's mask in a device byte arena of mask_bytes; data_off/plane_stride:
+ * element offset of each volume's plane 0 and the distance between its planes
+ * in device int32 arenas of data_elems (out and work share the layout; plane
+ * ranges must not overlap).  planes as in fe_edt_device.  create returns NULL
+ * with *status 1 (invalid) or 2 (runtime); run launches three kernels on
+ * `stream` and allocates nothing (capturable in a graph). */
+void *fe_edt_batch_create(const int32_t *dims, const int64_t *mask_off, const int64_t *data_off,
+                          const int64_t *plane_stride, int nvol, int planes, int64_t mask_bytes, int64_t data_elems,
+                          int *status);
+int fe_edt_batch_run(const void *handle, const uint8_t *masks, int32_t *work, int32_t *out, void *stream);
+void fe_edt_batch_destroy(void *handle);
+
 /* Signed output of edt_persistent_pair_v1/edt.cu:36-42 from a device pair. */
 int fe_edt_signed_device(const int32_t *pair, int64_t n, double pitch, float *out, void *stream);
 
