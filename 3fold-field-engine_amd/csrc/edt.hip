@@ -135,42 +135,51 @@ __global__ __launch_bounds__(256) void k_edt_zpass(const uint8_t* __restrict__ m
 }
 
 // ---------------------------------------------------------------- line passes
-// Parabola value of site s at position p over the LDS line f.
-__device__ __forceinline__ int32_t pval(const int32_t* f, int s, int p) { return f[s] + (p - s) * (p - s); }
+// Hull entries are packed into one 32-bit word: (F << 6) | (s - chunk start),
+// with F = f(s) + s^2.  Only sites with f < sentinel enter a hull, and for
+// axes <= 4096 every such F is below 3*4095^2 + 4095^2 < 2^26, so the packing
+// is exact; the chunk offset needs 6 bits (chunks hold at most 64 positions).
+__device__ __forceinline__ uint32_t pack_entry(int32_t F, int rel) { return ((uint32_t)F << 6) | (uint32_t)rel; }
 
-// Site b stays on the lower envelope of sites a < b < c iff the parabola of b
-// is strictly lowest somewhere, i.e. x(b,c) > x(a,b) for the real
-// intersections x; evaluated exactly in int64 without division.
-__device__ __forceinline__ bool keep_mid(const int32_t* f, int a, int b, int c)
+// Site b stays on the lower envelope of sites a < b < c iff its parabola is
+// strictly lowest somewhere, i.e. the point (b, Fb) lies strictly below the
+// segment from (a, Fa) to (c, Fc); exact in int64, no division.
+__device__ __forceinline__ bool keep_mid(int sa, int32_t Fa, int sb, int32_t Fb, int sc, int32_t Fc)
 {
-    const int64_t Fa = (int64_t)f[a] + (int64_t)a * a, Fb = (int64_t)f[b] + (int64_t)b * b,
-                  Fc = (int64_t)f[c] + (int64_t)c * c;
-    return (Fc - Fb) * (int64_t)(b - a) > (Fb - Fa) * (int64_t)(c - b);
+    return (int64_t)(Fc - Fb) * (sb - sa) > (int64_t)(Fb - Fa) * (sc - sb);
 }
 
-// One exact lower-envelope pass over W lines x L positions.  Thread (c, line)
-// owns chunk c (C positions) of a line.
-//  1. every thread builds the lower hull of the points (s, f(s)+s^2) of its
-//     chunk (the parabola envelope), keeping the two top entries in registers;
+// One exact lower-envelope pass over W lines x L positions.  Thread (line, c)
+// owns chunk c (C consecutive positions) of one line; the W lines of a tile are
+// adjacent along z, so every load and store of a warp-wide step touches W
+// consecutive int32 words per position.
+//  1. the thread loads its C values into registers and builds the lower hull
+//     of the points (s, f(s)+s^2) of its chunk (the parabola envelope) as a
+//     stack of packed entries in LDS, keeping the top two in registers;
 //  2. the chunk hulls of a line are merged in log2(T) rounds: in round h the
 //     thread at the boundary of two adjacent groups pops the tail of the left
 //     group and the head of the right group until the bridge between them is
-//     found (merging lower hulls of x-separated point sets);
+//     found (merging lower hulls of x-separated point sets), with the entries
+//     next to the bridge cached in registers;
 //  3. every thread walks the merged envelope over its positions (the minimum
-//     is unimodal along the envelope and its site moves right with p);
-//     results go back through LDS so that loads and stores are 16-byte vectors
-//     along the contiguous z axis.
+//     is unimodal along the envelope and its site moves right with p) and
+//     stores its results from registers.  When every axis is <= 1024 the
+//     surviving entries of a line are first compacted into one ascending
+//     array, so the walk is a plain index increment with the next entry
+//     cached; otherwise it steps through the chunk stacks.
 // A line whose values are all equal is its own result (g = f).  Sites holding
 // the sentinel (no site reached yet) are left out of the hulls: such a site
 // contributes sentinel + (p-s)^2 >= sentinel, with equality only at its own
 // position, so the result is min(envelope of the other sites, sentinel).
-// axis 1: lines along y (outer index x); axis 0: lines along x (outer index y).
-template <int W, int T, int CMAX>
+// LDS: kmax*W*T packed entries (kmax = the largest chunk length of the launch)
+// plus four bytes per thread and twelve bytes per line.
+template <int W, int T, int CMAX, bool COMPACT>
 __global__ __launch_bounds__(W* T) void k_edt_lines(const int32_t* __restrict__ src, int32_t* __restrict__ dst,
                                                     const EdtVol* __restrict__ table, int nvol, EdtVol one, int axis,
-                                                    uint32_t tiles, uint32_t grid)
+                                                    uint32_t tiles, uint32_t grid, int kmax)
 {
-    extern __shared__ int32_t lds[];
+    extern __shared__ uint32_t lds[];
+    constexpr int NT = W * T;
     const uint32_t tile = xcd_tile(blockIdx.x, grid);
     if (tile >= tiles) return;  // uniform per block
     const int pass = axis == 1 ? 1 : 2;
@@ -189,77 +198,69 @@ __global__ __launch_bounds__(W* T) void k_edt_lines(const int32_t* __restrict__ 
     const int outer = rest % outer_count, plane = rest / outer_count;
     const int64_t base = V.data + (int64_t)plane * V.plane + (int64_t)outer * outer_stride + (int64_t)zt * W;
     const int nlines = min(W, nz - zt * W);
-    const int LP = L + 1;
-    constexpr int NT = W * T;
-    int32_t* f = lds;                                          // [W][LP]
-    uint16_t* stk = reinterpret_cast<uint16_t*>(f + W * LP);  // [CMAX][NT]
-    uint16_t* blo = stk + CMAX * NT;                           // [NT] surviving range of a chunk hull
-    uint16_t* bhi = blo + NT;                                  // [NT]
-    uint8_t* lsb = reinterpret_cast<uint8_t*>(bhi + NT);       // [NT] nearest nonempty chunk <= c
-    uint8_t* rsb = lsb + NT;                                   // [NT] nearest nonempty chunk >= c
-    int32_t* lval = reinterpret_cast<int32_t*>(rsb + NT);      // [W] value of chunk 0's first sample
-    int32_t* lflat = lval + W;                                 // [W] line is constant
+
+    uint32_t* stk = lds;                                      // [kmax][NT] packed hull entries
+    uint8_t* blo = reinterpret_cast<uint8_t*>(stk + kmax * NT);  // [NT] surviving range of a chunk hull
+    uint8_t* bhi = blo + NT;
+    uint8_t* lsb = bhi + NT;                                  // [NT] nearest nonempty chunk <= c
+    uint8_t* rsb = lsb + NT;                                  // [NT] nearest nonempty chunk >= c
+    int32_t* lval = reinterpret_cast<int32_t*>(rsb + NT);     // [W] first value of the line
+    int32_t* lflat = lval + W;                                // [W] line is constant
+
     const int tid = threadIdx.x;
-    // rows of the tile are 16-byte aligned when nz and the volume's offsets are multiples of 4
-    const bool vec = ((nz | V.data | V.plane) & 3) == 0;
-
-    if (vec) {
-        for (int i = tid; i < (W / 4) * L; i += NT) {
-            const int l0 = (i % (W / 4)) * 4, q = i / (W / 4);
-            if (l0 < nlines) {
-                const int4 v = *reinterpret_cast<const int4*>(src + base + (int64_t)q * axis_stride + l0);
-                f[l0 * LP + q] = v.x;
-                f[(l0 + 1) * LP + q] = v.y;
-                f[(l0 + 2) * LP + q] = v.z;
-                f[(l0 + 3) * LP + q] = v.w;
-            }
-        }
-    } else {
-        for (int i = tid; i < W * L; i += NT) {
-            const int line = i % W, q = i / W;
-            if (line < nlines) f[line * LP + q] = src[base + (int64_t)q * axis_stride + line];
-        }
-    }
-    __syncthreads();
-
     const int line = tid % W, c = tid / W;
     const int C = (L + T - 1) / T;
     const int p0 = min(c * C, L), p1 = min(p0 + C, L);
     const bool active = line < nlines;
-    const int32_t* fl = f + line * LP;
+    // slot of chunk b of this line: rotated within the W-slot row so that the
+    // threads of one line (merge) and of one chunk row (hull, walk) spread over banks
+    auto slot = [&](int b) { return b * W + (line ^ (b & (W - 1))); };
+    auto entry = [&](int k, int b, int& s, int32_t& F) {
+        const uint32_t e = stk[k * NT + slot(b)];
+        s = b * C + (int)(e & 63u);
+        F = (int32_t)(e >> 6);
+    };
+    const int me = slot(c);
+    const int64_t lbase = base + line;
+
+    int32_t v[CMAX];
+#pragma unroll
+    for (int i = 0; i < CMAX; ++i)
+        if (active && p0 + i < p1) v[i] = src[lbase + (int64_t)(p0 + i) * axis_stride];
 
     // 1. hull of this chunk
     int k = -1, s_top = 0, s_sec = 0;
-    int64_t F_top = 0, F_sec = 0;
+    int32_t F_top = 0, F_sec = 0;
     int32_t mn = INT_MAX, mx = INT_MIN;
     if (active) {
-        for (int q = p0; q < p1; ++q) {
-            const int32_t fq = fl[q];
-            mn = min(mn, fq);
-            mx = max(mx, fq);
-            if (fq >= sentinel) continue;
-            const int64_t Fq = (int64_t)fq + (int64_t)q * q;
-            while (k >= 1) {
-                // top stays iff it is strictly lowest somewhere between sec and q
-                if ((Fq - F_top) * (int64_t)(s_top - s_sec) > (F_top - F_sec) * (int64_t)(q - s_top)) break;
-                --k;
-                s_top = s_sec;
-                F_top = F_sec;
-                if (k >= 1) {
-                    s_sec = stk[(k - 1) * NT + tid];
-                    F_sec = (int64_t)fl[s_sec] + (int64_t)s_sec * s_sec;
+#pragma unroll
+        for (int i = 0; i < CMAX; ++i) {
+            const int q = p0 + i;
+            if (q < p1) {
+                const int32_t fq = v[i];
+                mn = min(mn, fq);
+                mx = max(mx, fq);
+                if (fq < sentinel) {
+                    const int32_t Fq = fq + q * q;
+                    while (k >= 1) {
+                        if ((int64_t)(Fq - F_top) * (s_top - s_sec) > (int64_t)(F_top - F_sec) * (q - s_top)) break;
+                        --k;
+                        s_top = s_sec;
+                        F_top = F_sec;
+                        if (k >= 1) entry(k - 1, c, s_sec, F_sec);
+                    }
+                    ++k;
+                    stk[k * NT + me] = pack_entry(Fq, i);
+                    s_sec = s_top;
+                    F_sec = F_top;
+                    s_top = q;
+                    F_top = Fq;
                 }
             }
-            ++k;
-            stk[k * NT + tid] = (uint16_t)q;
-            s_sec = s_top;
-            F_sec = F_top;
-            s_top = q;
-            F_top = Fq;
         }
     }
-    blo[tid] = 0;
-    bhi[tid] = (uint16_t)(k + 1);
+    blo[me] = 0;
+    bhi[me] = (uint8_t)(k + 1);
     if (c == 0) {
         lval[line] = mn;
         lflat[line] = 1;
@@ -272,158 +273,269 @@ __global__ __launch_bounds__(W* T) void k_edt_lines(const int32_t* __restrict__ 
         if (active && (c & (2 * h - 1)) == h) {
             const int a0 = c - h, a2 = min(c + h, T);
             int ta = c - 1;
-            while (ta >= a0 && bhi[ta * W + line] == blo[ta * W + line]) --ta;
+            while (ta >= a0 && bhi[slot(ta)] == blo[slot(ta)]) --ta;
             int hb = c;
-            while (hb < a2 && bhi[hb * W + line] == blo[hb * W + line]) ++hb;
+            while (hb < a2 && bhi[slot(hb)] == blo[slot(hb)]) ++hb;
             if (ta >= a0 && hb < a2) {
+                int th = bhi[slot(ta)], tlo = blo[slot(ta)];
+                int hl = blo[slot(hb)], hhi = bhi[slot(hb)];
+                int s1, s3;
+                int32_t F1, F3;
+                entry(th - 1, ta, s1, F1);
+                entry(hl, hb, s3, F3);
                 bool changed = true;
                 while (changed) {
                     changed = false;
                     for (;;) {  // tail of the left group
-                        const int tt = ta * W + line;
-                        const int th = bhi[tt];
-                        const int s1 = stk[(th - 1) * NT + tt];
                         int s2, pa = ta;
-                        if (th - blo[tt] >= 2) s2 = stk[(th - 2) * NT + tt];
+                        int32_t F2;
+                        if (th - tlo >= 2) entry(th - 2, ta, s2, F2);
                         else {
                             pa = ta - 1;
-                            while (pa >= a0 && bhi[pa * W + line] == blo[pa * W + line]) --pa;
+                            while (pa >= a0 && bhi[slot(pa)] == blo[slot(pa)]) --pa;
                             if (pa < a0) break;
-                            s2 = stk[(bhi[pa * W + line] - 1) * NT + pa * W + line];
+                            entry(bhi[slot(pa)] - 1, pa, s2, F2);
                         }
-                        const int hbt = hb * W + line;
-                        const int s3 = stk[blo[hbt] * NT + hbt];
-                        if (keep_mid(fl, s2, s1, s3)) break;
-                        bhi[tt] = (uint16_t)(th - 1);
-                        if (th - 1 == blo[tt]) ta = pa;
+                        if (keep_mid(s2, F2, s1, F1, s3, F3)) break;
+                        if (--th == tlo) {
+                            bhi[slot(ta)] = (uint8_t)th;
+                            ta = pa;
+                            th = bhi[slot(pa)];
+                            tlo = blo[slot(pa)];
+                        }
+                        s1 = s2;
+                        F1 = F2;
                     }
                     for (;;) {  // head of the right group
-                        const int ht = hb * W + line;
-                        const int hl = blo[ht];
-                        const int s1 = stk[hl * NT + ht];
-                        int s2, nb = hb;
-                        if (bhi[ht] - hl >= 2) s2 = stk[(hl + 1) * NT + ht];
+                        int s4, nb = hb;
+                        int32_t F4;
+                        if (hhi - hl >= 2) entry(hl + 1, hb, s4, F4);
                         else {
                             nb = hb + 1;
-                            while (nb < a2 && bhi[nb * W + line] == blo[nb * W + line]) ++nb;
+                            while (nb < a2 && bhi[slot(nb)] == blo[slot(nb)]) ++nb;
                             if (nb >= a2) break;
-                            s2 = stk[blo[nb * W + line] * NT + nb * W + line];
+                            entry(blo[slot(nb)], nb, s4, F4);
                         }
-                        const int tt = ta * W + line;
-                        const int s0 = stk[(bhi[tt] - 1) * NT + tt];
-                        if (keep_mid(fl, s0, s1, s2)) break;
-                        blo[ht] = (uint16_t)(hl + 1);
+                        if (keep_mid(s1, F1, s3, F3, s4, F4)) break;
                         changed = true;
-                        if (hl + 1 == bhi[ht]) hb = nb;
+                        if (++hl == hhi) {
+                            blo[slot(hb)] = (uint8_t)hl;
+                            hb = nb;
+                            hl = blo[slot(nb)];
+                            hhi = bhi[slot(nb)];
+                        }
+                        s3 = s4;
+                        F3 = F4;
                     }
                 }
+                bhi[slot(ta)] = (uint8_t)th;
+                blo[slot(hb)] = (uint8_t)hl;
             }
         }
         __syncthreads();
     }
-    {
-        int b = c;
-        while (b >= 0 && bhi[b * W + line] == blo[b * W + line]) --b;
-        lsb[tid] = (uint8_t)(b < 0 ? 255 : b);
-        b = c;
-        while (b < T && bhi[b * W + line] == blo[b * W + line]) ++b;
-        rsb[tid] = (uint8_t)(b >= T ? 255 : b);
-    }
-    __syncthreads();
-
-    // 3. walk the merged envelope
-    int32_t out[CMAX];
-    if (active && p0 < p1) {
-        if (lflat[line]) {
+    if constexpr (COMPACT) {
+        // 3. every axis is <= 1024, so an entry repacks exactly into
+        // (F << 10) | s (F < 2^22).  Each thread computes, for the surviving
+        // entries of its chunk, the first integer position where the entry is
+        // the envelope minimum (one exact floor division with the neighbouring
+        // envelope entries) and writes the entry there; every position then
+        // takes the last entry written at or before it.  No data-dependent
+        // loops: three reads/writes per position and a scan over chunk tails.
+        const int lo = blo[me], cnt = bhi[me] - lo;
+        uint32_t pred = 0xffffffffu, succ = 0xffffffffu;
+        if (active) {
 #pragma unroll
             for (int i = 0; i < CMAX; ++i)
-                if (p0 + i < p1) out[i] = fl[p0 + i];
-        } else {
-            int cb = (bhi[tid] > blo[tid]) ? c : (lsb[tid] != 255 ? lsb[tid] : rsb[tid]);
-            int ct = cb * W + line;
-            int ci = cb == c ? blo[ct] : (cb < c ? bhi[ct] - 1 : blo[ct]);
-            int cs = stk[ci * NT + ct];
-            int32_t cf = fl[cs];
-            {
-                int32_t cv = cf + (p0 - cs) * (p0 - cs);
-                bool moved = false;
-                for (;;) {  // right while not worse
-                    int nb = cb, ni = ci + 1;
-                    if (ni >= bhi[ct]) {
-                        if (cb + 1 >= T) break;
-                        nb = rsb[(cb + 1) * W + line];
-                        if (nb == 255) break;
-                        ni = blo[nb * W + line];
+                if (i < cnt) {
+                    int es;
+                    int32_t eF;
+                    entry(lo + i, c, es, eF);
+                    v[i] = (int32_t)(((uint32_t)eF << 10) | (uint32_t)es);
+                }
+            int b = c - 1;
+            while (b >= 0 && bhi[slot(b)] == blo[slot(b)]) --b;
+            if (b >= 0) {
+                int es;
+                int32_t eF;
+                entry(bhi[slot(b)] - 1, b, es, eF);
+                pred = ((uint32_t)eF << 10) | (uint32_t)es;
+            }
+            b = c + 1;
+            while (b < T && bhi[slot(b)] == blo[slot(b)]) ++b;
+            if (b < T) {
+                int es;
+                int32_t eF;
+                entry(blo[slot(b)], b, es, eF);
+                succ = ((uint32_t)eF << 10) | (uint32_t)es;
+            }
+        }
+        __syncthreads();  // the hull stacks are free from here on
+        // at[pos(p)]: entry whose interval starts at position p.  Positions are
+        // permuted within aligned groups of four by the chunk of the group, so
+        // the threads of one wave (four chunks, same offset) hit four bank groups.
+        uint32_t* at = stk;
+        uint32_t* tail = reinterpret_cast<uint32_t*>(blo);  // [NT] last entry at or before a chunk's end
+        const uint32_t mC = ((1u << 20) + (uint32_t)C - 1) / (uint32_t)C;  // (p * mC) >> 20 == p / C for p < 4096
+        auto pos = [&](int p) { return (p ^ (int)((((uint32_t)p & ~3u) * mC >> 20) & 3u)) * W + line; };
+        if (active) {
+#pragma unroll
+            for (int i = 0; i < CMAX; ++i)
+                if (p0 + i < p1) at[pos(p0 + i)] = 0xffffffffu;
+        }
+        __syncthreads();
+        // first integer p where entry b is strictly below entry a (a left of b):
+        // floor((Fb - Fa) / (2 (sb - sa))) + 1, exact: |Fb - Fa| < 2^22 and the
+        // reciprocal estimate is within 0.4 of the quotient, so one integer
+        // correction step gives the floor.
+        auto start = [](uint32_t a, uint32_t b) {
+            const int32_t n = (int32_t)(b >> 10) - (int32_t)(a >> 10);
+            const int32_t d = 2 * ((int32_t)(b & 1023u) - (int32_t)(a & 1023u));
+            int32_t q = (int32_t)floorf((float)n * __builtin_amdgcn_rcpf((float)d));
+            const int32_t r = n - q * d;
+            q += r < 0 ? -1 : (r >= d ? 1 : 0);
+            return q + 1;
+        };
+        if (active && cnt > 0) {
+            int xlo = pred == 0xffffffffu ? 0 : min(max(start(pred, (uint32_t)v[0]), 0), L);
+#pragma unroll
+            for (int i = 0; i < CMAX; ++i)
+                if (i < cnt) {
+                    const uint32_t e = (uint32_t)v[i];
+                    const uint32_t nx = i + 1 < cnt ? (uint32_t)v[min(i + 1, CMAX - 1)] : succ;
+                    const int xhi = nx == 0xffffffffu ? L : min(max(start(e, nx), 0), L);
+                    if (xlo < xhi) at[pos(xlo)] = e;
+                    xlo = max(xlo, xhi);
+                }
+        }
+        __syncthreads();
+        uint32_t last = 0xffffffffu;
+        if (active) {
+#pragma unroll
+            for (int i = 0; i < CMAX; ++i)
+                if (p0 + i < p1) {
+                    const uint32_t e = at[pos(p0 + i)];
+                    last = e != 0xffffffffu ? e : last;
+                    v[i] = (int32_t)last;
+                }
+        }
+        tail[me] = last;
+        __syncthreads();
+        if (active && p0 < p1) {
+            if (lflat[line]) {
+                const int32_t g = lval[line];
+#pragma unroll
+                for (int i = 0; i < CMAX; ++i) v[i] = g;
+            } else {
+                uint32_t carry = 0xffffffffu;
+#pragma unroll
+                for (int b = 0; b < T; ++b)
+                    if (b < c) {
+                        const uint32_t e = tail[slot(b)];
+                        carry = e != 0xffffffffu ? e : carry;
                     }
-                    const int nt = nb * W + line;
-                    const int ns = stk[ni * NT + nt];
-                    const int32_t nf = fl[ns];
-                    const int32_t nv = nf + (p0 - ns) * (p0 - ns);
-                    if (nv > cv) break;
-                    cb = nb; ct = nt; ci = ni; cs = ns; cf = nf; cv = nv; moved = true;
+#pragma unroll
+                for (int i = 0; i < CMAX; ++i) {
+                    const int p = p0 + i;
+                    if (p < p1) {
+                        uint32_t e = (uint32_t)v[i];
+                        if (e == 0xffffffffu) e = carry;  // position 0 always holds an entry
+                        const int es = (int)(e & 1023u);
+                        const int32_t eF = (int32_t)(e >> 10);
+                        const int32_t g = eF - (int32_t)__umul24(2u * p, es) + (int32_t)__umul24(p, p);
+                        v[i] = e == 0xffffffffu ? sentinel : min(g, sentinel);
+                    }
+                }
+            }
+        }
+    } else {
+        {
+            int b = c;
+            while (b >= 0 && bhi[slot(b)] == blo[slot(b)]) --b;
+            lsb[me] = (uint8_t)(b < 0 ? 255 : b);
+            b = c;
+            while (b < T && bhi[slot(b)] == blo[slot(b)]) ++b;
+            rsb[me] = (uint8_t)(b >= T ? 255 : b);
+        }
+        __syncthreads();
+
+        // 3. walk the merged envelope; results replace v
+        if (active && p0 < p1 && !lflat[line]) {
+            int cb = bhi[me] > blo[me] ? c : (lsb[me] != 255 ? lsb[me] : rsb[me]);
+            if (cb == 255) {
+    #pragma unroll
+                for (int i = 0; i < CMAX; ++i) v[i] = sentinel;
+            } else {
+                int clo = blo[slot(cb)], chi = bhi[slot(cb)];
+                int ci = cb >= c ? clo : chi - 1;
+                int cs;
+                int32_t cF;
+                entry(ci, cb, cs, cF);
+                // next entry to the right; false at the end of the envelope
+                auto next = [&](int& nb, int& ni, int& nlo, int& nhi) {
+                    nb = cb;
+                    ni = ci + 1;
+                    nlo = clo;
+                    nhi = chi;
+                    if (ni < chi) return true;
+                    if (cb + 1 >= T) return false;
+                    nb = rsb[slot(cb + 1)];
+                    if (nb == 255) return false;
+                    nlo = blo[slot(nb)];
+                    nhi = bhi[slot(nb)];
+                    ni = nlo;
+                    return true;
+                };
+                bool moved = false;
+                for (;;) {  // at p0: right while not worse
+                    int nb, ni, nlo, nhi, ns;
+                    int32_t nF;
+                    if (!next(nb, ni, nlo, nhi)) break;
+                    entry(ni, nb, ns, nF);
+                    if (nF - 2 * p0 * ns > cF - 2 * p0 * cs) break;
+                    cb = nb; ci = ni; clo = nlo; chi = nhi; cs = ns; cF = nF;
+                    moved = true;
                 }
                 if (!moved) {
                     for (;;) {  // left while strictly better
-                        int pb = cb, pi = ci - 1;
-                        if (pi < blo[ct]) {
+                        int pb = cb, pi = ci - 1, plo = clo, phi = chi;
+                        if (pi < clo) {
                             if (cb == 0) break;
-                            pb = lsb[(cb - 1) * W + line];
+                            pb = lsb[slot(cb - 1)];
                             if (pb == 255) break;
-                            pi = bhi[pb * W + line] - 1;
+                            plo = blo[slot(pb)];
+                            phi = bhi[slot(pb)];
+                            pi = phi - 1;
                         }
-                        const int pt = pb * W + line;
-                        const int ps = stk[pi * NT + pt];
-                        const int32_t pf = fl[ps];
-                        const int32_t pv = pf + (p0 - ps) * (p0 - ps);
-                        if (pv >= cv) break;
-                        cb = pb; ct = pt; ci = pi; cs = ps; cf = pf; cv = pv;
+                        int ps;
+                        int32_t pF;
+                        entry(pi, pb, ps, pF);
+                        if (pF - 2 * p0 * ps >= cF - 2 * p0 * cs) break;
+                        cb = pb; ci = pi; clo = plo; chi = phi; cs = ps; cF = pF;
                     }
                 }
-            }
-#pragma unroll
-            for (int i = 0; i < CMAX; ++i) {
-                const int p = p0 + i;
-                if (p < p1) {
-                    int32_t cv = cf + (p - cs) * (p - cs);
-                    for (;;) {
-                        int nb = cb, ni = ci + 1;
-                        if (ni >= bhi[ct]) {
-                            if (cb + 1 >= T) break;
-                            nb = rsb[(cb + 1) * W + line];
-                            if (nb == 255) break;
-                            ni = blo[nb * W + line];
+    #pragma unroll
+                for (int i = 0; i < CMAX; ++i) {
+                    const int p = p0 + i;
+                    if (p < p1) {
+                        for (;;) {
+                            int nb, ni, nlo, nhi, ns;
+                            int32_t nF;
+                            if (!next(nb, ni, nlo, nhi)) break;
+                            entry(ni, nb, ns, nF);
+                            if (nF - 2 * p * ns > cF - 2 * p * cs) break;
+                            cb = nb; ci = ni; clo = nlo; chi = nhi; cs = ns; cF = nF;
                         }
-                        const int nt = nb * W + line;
-                        const int ns = stk[ni * NT + nt];
-                        const int32_t nf = fl[ns];
-                        const int32_t nv = nf + (p - ns) * (p - ns);
-                        if (nv > cv) break;
-                        cb = nb; ct = nt; ci = ni; cs = ns; cf = nf; cv = nv;
+                        v[i] = min(cF - 2 * p * cs + p * p, sentinel);
                     }
-                    out[i] = min(cv, sentinel);
                 }
             }
         }
     }
-    __syncthreads();
     if (active) {
 #pragma unroll
         for (int i = 0; i < CMAX; ++i)
-            if (p0 + i < p1) f[line * LP + p0 + i] = out[i];
-    }
-    __syncthreads();
-    if (vec) {
-        for (int i = tid; i < (W / 4) * L; i += NT) {
-            const int l0 = (i % (W / 4)) * 4, q = i / (W / 4);
-            if (l0 < nlines)
-                *reinterpret_cast<int4*>(dst + base + (int64_t)q * axis_stride + l0) =
-                    make_int4(f[l0 * LP + q], f[(l0 + 1) * LP + q], f[(l0 + 2) * LP + q], f[(l0 + 3) * LP + q]);
-        }
-    } else {
-        for (int i = tid; i < W * L; i += NT) {
-            const int ln = i % W, q = i / W;
-            if (ln < nlines) dst[base + (int64_t)q * axis_stride + ln] = f[ln * LP + q];
-        }
+            if (p0 + i < p1) dst[lbase + (int64_t)(p0 + i) * axis_stride] = v[i];
     }
 }
 
@@ -445,44 +557,58 @@ struct LineCfg {
     int W, T, CMAX;
 };
 
-// Tile shape by line length (W lines, T threads per line, CMAX >= L/T): LDS
-// holds W*(L+1) int32 values plus CMAX*W*T uint16 hull slots.  Measured on
-// MI355X at L = 512: 16 lines x 32 threads beats 16 x 16 and 16 x 64 (fewer
-// merge rounds vs. shorter serial chunks) and 32-line tiles (LDS occupancy).
+// Tile shape by line length: W lines (W*4 contiguous bytes per position),
+// T threads per line, CMAX >= ceil(L/T) positions per thread (registers).
 LineCfg pick_cfg(int L)
 {
     if (L <= 128) return {32, 8, 16};
+    if (L <= 256) return {32, 16, 16};
     if (L <= 512) return {16, 32, 16};
-    if (L <= 1024) return {8, 32, 32};
-    return {4, 64, 64};
+    if (L <= 768) return {16, 32, 24};
+    if (L <= 1024) return {16, 32, 32};
+    if (L <= 2048) return {8, 64, 32};
+    return {8, 64, 64};
 }
 
-template <int W, int T, int CMAX>
+template <int W, int T, int CMAX, bool COMPACT>
 hipError_t launch_lines_t(const int32_t* src, int32_t* dst, const EdtVol* table, int nvol, const EdtVol& one,
                           int axis, int Lmax, uint64_t tiles, hipStream_t st)
 {
     const uint32_t grid = round8(tiles);
-    const size_t lds = (size_t)W * (Lmax + 1) * 4 + (size_t)CMAX * W * T * 2 + (size_t)W * T * 6 + (size_t)W * 8;
+    const int kmax = (Lmax + T - 1) / T;  // <= CMAX by the choice of shape
+    const size_t lds = (size_t)kmax * W * T * 4 + (size_t)W * T * 4 + (size_t)W * 12;
+    const void* fn = reinterpret_cast<const void*>(&k_edt_lines<W, T, CMAX, COMPACT>);
     if (lds > 65536) {
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_edt_lines<W, T, CMAX>),
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL((k_edt_lines<W, T, CMAX>), dim3(grid), dim3(W * T), lds, st, src, dst, table, nvol, one, axis,
-                       (uint32_t)tiles, grid);
+    hipLaunchKernelGGL((k_edt_lines<W, T, CMAX, COMPACT>), dim3(grid), dim3(W * T), lds, st, src, dst, table, nvol,
+                       one, axis, (uint32_t)tiles, grid, kmax);
     return hipGetLastError();
 }
 
 // One line pass over every volume: Lmax (the longest line of the pass over
 // all volumes) picks the tile shape, `tiles` counts the tiles of that shape.
+// compact: every axis of every volume is <= 1024 (packed 32-bit envelope).
 hipError_t launch_lines(const int32_t* src, int32_t* dst, const EdtVol* table, int nvol, const EdtVol& one, int axis,
-                        int Lmax, uint64_t tiles, hipStream_t st)
+                        int Lmax, uint64_t tiles, bool compact, hipStream_t st)
 {
-    LineCfg c = pick_cfg(Lmax);
-    if (c.W == 32) return launch_lines_t<32, 8, 16>(src, dst, table, nvol, one, axis, Lmax, tiles, st);
-    if (c.W == 16) return launch_lines_t<16, 32, 16>(src, dst, table, nvol, one, axis, Lmax, tiles, st);
-    if (c.W == 8) return launch_lines_t<8, 32, 32>(src, dst, table, nvol, one, axis, Lmax, tiles, st);
-    return launch_lines_t<4, 64, 64>(src, dst, table, nvol, one, axis, Lmax, tiles, st);
+    const LineCfg c = pick_cfg(Lmax);
+#define FE_LINES(w, t, cm)                                                                                   \
+    if (c.W == w && c.T == t && c.CMAX == cm)                                                                \
+        return compact ? launch_lines_t<w, t, cm, true>(src, dst, table, nvol, one, axis, Lmax, tiles, st)   \
+                       : launch_lines_t<w, t, cm, false>(src, dst, table, nvol, one, axis, Lmax, tiles, st);
+    FE_LINES(32, 8, 16)
+    FE_LINES(32, 16, 16)
+    FE_LINES(16, 32, 16)
+    FE_LINES(16, 32, 24)
+    FE_LINES(16, 32, 32)
+#undef FE_LINES
+    if (c.W == 8 && c.T == 64 && c.CMAX == 32)
+        return launch_lines_t<8, 64, 32, false>(src, dst, table, nvol, one, axis, Lmax, tiles, st);
+    if (c.W == 8 && c.T == 64 && c.CMAX == 64)
+        return launch_lines_t<8, 64, 64, false>(src, dst, table, nvol, one, axis, Lmax, tiles, st);
+    return hipErrorInvalidValue;
 }
 
 int64_t edt_sentinel(int nx, int ny, int nz)
@@ -502,7 +628,8 @@ void vol_tiles(int nx, int ny, int nz, int planes, int Wy, int Wx, uint64_t t[3]
 // z pass into `out`, y pass into `work`, x pass back into `out`.  planes == 1
 // sites: mask == 0; planes == 2, plane 0 sites: mask != 0.
 hipError_t run_passes(const uint8_t* masks, int32_t* work, int32_t* out, const EdtVol* table, int nvol,
-                      const EdtVol& one, int planes, int Lmax_y, int Lmax_x, const uint64_t tiles[3], hipStream_t st)
+                      const EdtVol& one, int planes, int Lmax_y, int Lmax_x, const uint64_t tiles[3], bool compact,
+                      hipStream_t st)
 {
     const int site_one = planes == 2 ? 1 : 0;
     if (planes == 2)
@@ -513,8 +640,8 @@ hipError_t run_passes(const uint8_t* masks, int32_t* work, int32_t* out, const E
                            one, site_one);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    if ((e = launch_lines(out, work, table, nvol, one, 1, Lmax_y, tiles[1], st)) != hipSuccess) return e;
-    return launch_lines(work, out, table, nvol, one, 0, Lmax_x, tiles[2], st);
+    if ((e = launch_lines(out, work, table, nvol, one, 1, Lmax_y, tiles[1], compact, st)) != hipSuccess) return e;
+    return launch_lines(work, out, table, nvol, one, 0, Lmax_x, tiles[2], compact, st);
 }
 
 EdtVol single_vol(int nx, int ny, int nz, int64_t plane)
@@ -551,7 +678,9 @@ extern "C" int fe_edt_device(const uint8_t* mask, int nx, int ny, int nz, int pl
     const EdtVol one = single_vol(nx, ny, nz, (int64_t)nx * ny * nz);
     uint64_t tiles[3];
     vol_tiles(nx, ny, nz, planes, pick_cfg(ny).W, pick_cfg(nx).W, tiles);
-    FE_TRY(run_passes(mask, work, out, nullptr, 0, one, planes, ny, nx, tiles, static_cast<hipStream_t>(stream)));
+    const bool compact = nx <= 1024 && ny <= 1024 && nz <= 1024;
+    FE_TRY(run_passes(mask, work, out, nullptr, 0, one, planes, ny, nx, tiles, compact,
+                      static_cast<hipStream_t>(stream)));
     return FE_OK;
 }
 
@@ -577,8 +706,9 @@ extern "C" int fe_edt_axis_pass(const void* src, int32_t* dst, int nx, int ny, i
         return FE_OK;
     }
     const int32_t* a = static_cast<const int32_t*>(src);
-    if (axis == 1) FE_TRY(launch_lines(a, dst, nullptr, 0, one, 1, ny, tiles[1], st));
-    else FE_TRY(launch_lines(a, dst, nullptr, 0, one, 0, nx, tiles[2], st));
+    const bool compact = nx <= 1024 && ny <= 1024 && nz <= 1024;
+    if (axis == 1) FE_TRY(launch_lines(a, dst, nullptr, 0, one, 1, ny, tiles[1], compact, st));
+    else FE_TRY(launch_lines(a, dst, nullptr, 0, one, 0, nx, tiles[2], compact, st));
     return FE_OK;
 }
 
@@ -588,6 +718,7 @@ struct EdtBatch {
     EdtVol* table = nullptr;  // device
     int nvol = 0, planes = 0, Lmax_y = 0, Lmax_x = 0;
     uint64_t tiles[3] = {0, 0, 0};
+    bool compact = true;
 };
 }  // namespace
 
@@ -612,7 +743,7 @@ extern "C" void* fe_edt_batch_create(const int32_t* dims, const int64_t* mask_of
     try {
         std::vector<EdtVol> host((size_t)nvol);
         std::vector<std::pair<int64_t, int64_t>> ranges((size_t)nvol);
-        int Ly = 1, Lx = 1;
+        int Ly = 1, Lx = 1, Lz = 1;
         for (int v = 0; v < nvol; ++v) {
             const int nx = dims[3 * v], ny = dims[3 * v + 1], nz = dims[3 * v + 2];
             if (nx < 1 || ny < 1 || nz < 1 || nx > FE_EDT_MAX_AXIS || ny > FE_EDT_MAX_AXIS || nz > FE_EDT_MAX_AXIS)
@@ -627,6 +758,7 @@ extern "C" void* fe_edt_batch_create(const int32_t* dims, const int64_t* mask_of
             ranges[v] = {data_off[v], data_off[v] + (planes - 1) * plane_stride[v] + n};
             Ly = std::max(Ly, ny);
             Lx = std::max(Lx, nx);
+            Lz = std::max(Lz, nz);
         }
         std::sort(ranges.begin(), ranges.end());
         for (size_t i = 1; i < ranges.size(); ++i)
@@ -649,6 +781,7 @@ extern "C" void* fe_edt_batch_create(const int32_t* dims, const int64_t* mask_of
         b->planes = planes;
         b->Lmax_y = Ly;
         b->Lmax_x = Lx;
+        b->compact = Lx <= 1024 && Ly <= 1024 && Lz <= 1024;
         for (int p = 0; p < 3; ++p) b->tiles[p] = tot[p];
         hipError_t e = hipMalloc(&b->table, sizeof(EdtVol) * (size_t)nvol);
         if (e == hipSuccess) e = hipMemcpy(b->table, host.data(), sizeof(EdtVol) * (size_t)nvol, hipMemcpyHostToDevice);
@@ -679,7 +812,7 @@ extern "C" int fe_edt_batch_run(const void* handle, const uint8_t* masks, int32_
     if (!b || !masks || !work || !out) return FE_INVALID;
     const EdtVol none{};
     FE_TRY(run_passes(masks, work, out, b->table, b->nvol, none, b->planes, b->Lmax_y, b->Lmax_x, b->tiles,
-                      static_cast<hipStream_t>(stream)));
+                      b->compact, static_cast<hipStream_t>(stream)));
     return FE_OK;
 }
 

@@ -53,6 +53,20 @@ def event_ms(torch, fn, reps):
     return s.elapsed_time(e) / reps
 
 
+def profiled_traffic(kernel, n):
+    """HBM bytes per launch of `kernel` at n^3 from the rocprofv3 FETCH_SIZE /
+    WRITE_SIZE runs summarised in profiles/hbm_traffic_<n>_pmc.txt (counters
+    scaled by a 1 GiB copy), or None when no such measurement exists."""
+    path = ROOT / "profiles" / f"hbm_traffic_{n}_pmc.txt"
+    if not path.is_file():
+        return None
+    for line in path.read_text().splitlines():
+        parts = line.split()
+        if parts and parts[0] == kernel and "hbm_bytes_per_launch" in parts:
+            return int(parts[parts.index("hbm_bytes_per_launch") + 2])
+    return None
+
+
 def cpu_baseline(fe, rec, occ_host, threads):
     """The C oracle (oracle/fe_oracle.c) on host cores, on a bounded sample:
     the field build of the central 512x512x16 slab (same recipe, same grid)
@@ -162,6 +176,39 @@ def bench_edt_heavy(fe, torch, reps=5, plain_sample=16):
     return out
 
 
+def kernel_launches(fe, torch, n, reps=3):
+    """Launches each hot kernel of the step `reps` times on resident inputs,
+    then a plain 1 GiB device copy (1 GiB read + 1 GiB written) that
+    calibrates the rocprofv3 FETCH_SIZE / WRITE_SIZE counters."""
+    dev = torch.device("cuda:0")
+    rec = fe.recipes.large_field_512(seed=2) if n == 512 else fe.recipes.mixed_recipe(np.random.default_rng(2), 1024, n)
+    meta = fe.GridMeta(0.0, 0.0, 0.0, 1.0, n, n, n, block=8)
+    plan = fe.FieldPlan([(meta, rec)], device=dev)
+    dense = torch.empty((n, n, n), dtype=torch.float32, device=dev)
+    occ = torch.empty((n, n, n), dtype=torch.uint8, device=dev)
+    pair = torch.empty((2, n, n, n), dtype=torch.int32, device=dev)
+    work = torch.empty_like(pair)
+    L = fe._lib.lib()
+    st = fe._lib.stream_handle(None)
+    P = fe._lib.ptr
+    for _ in range(reps):
+        plan.launch()
+        plan.reconstruct(0, dense, occ)
+        L.fe_edt_axis_pass(P(occ), P(work), n, n, n, 2, 2, st)
+        L.fe_edt_axis_pass(P(work), P(pair), n, n, n, 2, 1, st)
+        L.fe_edt_axis_pass(P(pair), P(work), n, n, n, 2, 0, st)
+    torch.cuda.synchronize()
+    na = plan.n_active(0)
+    del plan, dense, pair, work
+    torch.cuda.empty_cache()
+    src = torch.ones(1 << 28, dtype=torch.int32, device=dev)
+    dst = torch.empty_like(src)
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    return {"n": n, "n_active": na, "n_blocks": meta.n_blocks, "copy_bytes_each_way": 1 << 30, "reps": reps}
+
+
 def bench_serve(fe, torch, rates, count):
     """Poisson arrivals of 512^3 / 1024-primitive recipes into the field server."""
     rng = np.random.default_rng(5)
@@ -196,7 +243,9 @@ def main():
     ap.add_argument("--quick", action="store_true", help="single-field step only")
     ap.add_argument("--batch-count", type=int, default=256)
     ap.add_argument("--serve-count", type=int, default=48)
-    ap.add_argument("--only", choices=["edt-heavy"], default=None, help="run one section only")
+    ap.add_argument("--only", choices=["edt-heavy", "kernels"], default=None,
+                    help="edt-heavy: the thin-feature EDT section only; kernels: each hot kernel of the "
+                         "512^3 step three times plus a 1 GiB copy (for rocprofv3 counter runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU: the block-sparse build and the EDT run on one "
@@ -207,6 +256,9 @@ def main():
     dev = torch.device("cuda:0")
     if args.only == "edt-heavy":
         print(json.dumps({"edt_heavy": bench_edt_heavy(fe, torch)}))
+        return 0
+    if args.only == "kernels":
+        print(json.dumps(kernel_launches(fe, torch, args.n)))
         return 0
     n = args.n
     rec = fe.recipes.large_field_512(seed=2) if n == 512 else fe.recipes.mixed_recipe(np.random.default_rng(2), 1024, n)
@@ -279,7 +331,8 @@ def main():
         "edt_mvox_s": N / (edt_ms * 1e-3) / 1e6,
         "kernel_ms": {k: v[0] for k, v in kernels.items()},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": profiled_traffic(dom, n),
+                     "algorithmic_bytes": dbytes},
     }
     occ_host = occ.cpu().numpy()
     out["config"]["occupancy_fraction"] = float(occ_host.mean())
