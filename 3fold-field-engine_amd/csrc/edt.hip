@@ -144,9 +144,19 @@ __device__ __forceinline__ uint32_t pack_entry(int32_t F, int rel) { return ((ui
 // Site b stays on the lower envelope of sites a < b < c iff its parabola is
 // strictly lowest somewhere, i.e. the point (b, Fb) lies strictly below the
 // segment from (a, Fa) to (c, Fc); exact in int64, no division.
+// With every axis <= 1024 (SMALL), F < 2^22 and site differences < 2^10, so
+// both products are exact 24 x 24-bit products (full-rate multiplies).
+__device__ __forceinline__ int32_t sx24(int32_t x) { return (x << 8) >> 8; }
+template <bool SMALL>
+__device__ __forceinline__ bool above(int32_t dF1, int ds1, int32_t dF2, int ds2)
+{
+    if constexpr (SMALL) return (int64_t)sx24(dF1) * sx24(ds1) > (int64_t)sx24(dF2) * sx24(ds2);
+    else return (int64_t)dF1 * ds1 > (int64_t)dF2 * ds2;
+}
+template <bool SMALL>
 __device__ __forceinline__ bool keep_mid(int sa, int32_t Fa, int sb, int32_t Fb, int sc, int32_t Fc)
 {
-    return (int64_t)(Fc - Fb) * (sb - sa) > (int64_t)(Fb - Fa) * (sc - sb);
+    return above<SMALL>(Fc - Fb, sb - sa, Fb - Fa, sc - sb);
 }
 
 // One exact lower-envelope pass over W lines x L positions.  Thread (line, c)
@@ -224,9 +234,14 @@ __global__ __launch_bounds__(W* T) void k_edt_lines(const int32_t* __restrict__ 
     const int64_t lbase = base + line;
 
     int32_t v[CMAX];
+    if (active) {
+        const int32_t* ps = src + lbase + (int64_t)p0 * axis_stride;
 #pragma unroll
-    for (int i = 0; i < CMAX; ++i)
-        if (active && p0 + i < p1) v[i] = src[lbase + (int64_t)(p0 + i) * axis_stride];
+        for (int i = 0; i < CMAX; ++i) {
+            if (p0 + i < p1) v[i] = *ps;
+            ps += axis_stride;
+        }
+    }
 
     // 1. hull of this chunk
     int k = -1, s_top = 0, s_sec = 0;
@@ -243,7 +258,7 @@ __global__ __launch_bounds__(W* T) void k_edt_lines(const int32_t* __restrict__ 
                 if (fq < sentinel) {
                     const int32_t Fq = fq + q * q;
                     while (k >= 1) {
-                        if ((int64_t)(Fq - F_top) * (s_top - s_sec) > (int64_t)(F_top - F_sec) * (q - s_top)) break;
+                        if (above<COMPACT>(Fq - F_top, s_top - s_sec, F_top - F_sec, q - s_top)) break;
                         --k;
                         s_top = s_sec;
                         F_top = F_sec;
@@ -296,7 +311,7 @@ __global__ __launch_bounds__(W* T) void k_edt_lines(const int32_t* __restrict__ 
                             if (pa < a0) break;
                             entry(bhi[slot(pa)] - 1, pa, s2, F2);
                         }
-                        if (keep_mid(s2, F2, s1, F1, s3, F3)) break;
+                        if (keep_mid<COMPACT>(s2, F2, s1, F1, s3, F3)) break;
                         if (--th == tlo) {
                             bhi[slot(ta)] = (uint8_t)th;
                             ta = pa;
@@ -316,7 +331,7 @@ __global__ __launch_bounds__(W* T) void k_edt_lines(const int32_t* __restrict__ 
                             if (nb >= a2) break;
                             entry(blo[slot(nb)], nb, s4, F4);
                         }
-                        if (keep_mid(s1, F1, s3, F3, s4, F4)) break;
+                        if (keep_mid<COMPACT>(s1, F1, s3, F3, s4, F4)) break;
                         changed = true;
                         if (++hl == hhi) {
                             blo[slot(hb)] = (uint8_t)hl;
@@ -533,9 +548,12 @@ __global__ __launch_bounds__(W* T) void k_edt_lines(const int32_t* __restrict__ 
         }
     }
     if (active) {
+        int32_t* pd = dst + lbase + (int64_t)p0 * axis_stride;
 #pragma unroll
-        for (int i = 0; i < CMAX; ++i)
-            if (p0 + i < p1) dst[lbase + (int64_t)(p0 + i) * axis_stride] = v[i];
+        for (int i = 0; i < CMAX; ++i) {
+            if (p0 + i < p1) *pd = v[i];
+            pd += axis_stride;
+        }
     }
 }
 
