@@ -135,11 +135,16 @@ __global__ __launch_bounds__(256) void k_edt_zpass(const uint8_t* __restrict__ m
 }
 
 // ---------------------------------------------------------------- line passes
-// Hull entries are packed into one 32-bit word: (F << 6) | (s - chunk start),
-// with F = f(s) + s^2.  Only sites with f < sentinel enter a hull, and for
-// axes <= 4096 every such F is below 3*4095^2 + 4095^2 < 2^26, so the packing
-// is exact; the chunk offset needs 6 bits (chunks hold at most 64 positions).
-__device__ __forceinline__ uint32_t pack_entry(int32_t F, int rel) { return ((uint32_t)F << 6) | (uint32_t)rel; }
+// Hull entries are packed into one 32-bit word, F = f(s) + s^2.  Only sites
+// with f < sentinel enter a hull.  When every axis is <= 1024 (SMALL),
+// F < 3*1023^2 + 1023^2 < 2^22 and the word is (F << 10) | s; otherwise
+// (axes <= 4096) F < 3*4095^2 + 4095^2 < 2^26 and the word is
+// (F << 6) | (s - chunk start), chunks holding at most 64 positions.
+template <bool SMALL>
+__device__ __forceinline__ uint32_t pack_entry(int32_t F, int s, int rel)
+{
+    return SMALL ? (((uint32_t)F << 10) | (uint32_t)s) : (((uint32_t)F << 6) | (uint32_t)rel);
+}
 
 // Site b stays on the lower envelope of sites a < b < c iff its parabola is
 // strictly lowest somewhere, i.e. the point (b, Fb) lies strictly below the
@@ -186,7 +191,7 @@ __device__ __forceinline__ bool keep_mid(int sa, int32_t Fa, int sb, int32_t Fb,
 template <int W, int T, int CMAX, bool COMPACT>
 __global__ __launch_bounds__(W* T) void k_edt_lines(const int32_t* __restrict__ src, int32_t* __restrict__ dst,
                                                     const EdtVol* __restrict__ table, int nvol, EdtVol one, int axis,
-                                                    uint32_t tiles, uint32_t grid, int kmax)
+                                                    uint32_t tiles, uint32_t grid, int region)
 {
     extern __shared__ uint32_t lds[];
     constexpr int NT = W * T;
@@ -209,8 +214,8 @@ __global__ __launch_bounds__(W* T) void k_edt_lines(const int32_t* __restrict__ 
     const int64_t base = V.data + (int64_t)plane * V.plane + (int64_t)outer * outer_stride + (int64_t)zt * W;
     const int nlines = min(W, nz - zt * W);
 
-    uint32_t* stk = lds;                                      // [kmax][NT] packed hull entries
-    uint8_t* blo = reinterpret_cast<uint8_t*>(stk + kmax * NT);  // [NT] surviving range of a chunk hull
+    uint32_t* stk = lds;                                        // [kmax][NT] packed hull entries (region words)
+    uint8_t* blo = reinterpret_cast<uint8_t*>(stk + region);  // [NT] surviving range of a chunk hull
     uint8_t* bhi = blo + NT;
     uint8_t* lsb = bhi + NT;                                  // [NT] nearest nonempty chunk <= c
     uint8_t* rsb = lsb + NT;                                  // [NT] nearest nonempty chunk >= c
@@ -227,8 +232,13 @@ __global__ __launch_bounds__(W* T) void k_edt_lines(const int32_t* __restrict__ 
     auto slot = [&](int b) { return b * W + (line ^ (b & (W - 1))); };
     auto entry = [&](int k, int b, int& s, int32_t& F) {
         const uint32_t e = stk[k * NT + slot(b)];
-        s = b * C + (int)(e & 63u);
-        F = (int32_t)(e >> 6);
+        if constexpr (COMPACT) {
+            s = (int)(e & 1023u);
+            F = (int32_t)(e >> 10);
+        } else {
+            s = b * C + (int)(e & 63u);
+            F = (int32_t)(e >> 6);
+        }
     };
     const int me = slot(c);
     const int64_t lbase = base + line;
@@ -265,7 +275,7 @@ __global__ __launch_bounds__(W* T) void k_edt_lines(const int32_t* __restrict__ 
                         if (k >= 1) entry(k - 1, c, s_sec, F_sec);
                     }
                     ++k;
-                    stk[k * NT + me] = pack_entry(Fq, i);
+                    stk[k * NT + me] = pack_entry<COMPACT>(Fq, q, i);
                     s_sec = s_top;
                     F_sec = F_top;
                     s_top = q;
@@ -350,53 +360,32 @@ __global__ __launch_bounds__(W* T) void k_edt_lines(const int32_t* __restrict__ 
         __syncthreads();
     }
     if constexpr (COMPACT) {
-        // 3. every axis is <= 1024, so an entry repacks exactly into
-        // (F << 10) | s (F < 2^22).  Each thread computes, for the surviving
-        // entries of its chunk, the first integer position where the entry is
-        // the envelope minimum (one exact floor division with the neighbouring
-        // envelope entries) and writes the entry there; every position then
-        // takes the last entry written at or before it.  No data-dependent
-        // loops: three reads/writes per position and a scan over chunk tails.
+        // 3. Each thread computes, for the surviving entries of its chunk, the
+        // first integer position where the entry is the envelope minimum (one
+        // exact floor division with the neighbouring envelope entry) and
+        // writes the entry there; every position then takes the last entry
+        // written at or before it.  No data-dependent loops over positions.
         const int lo = blo[me], cnt = bhi[me] - lo;
         uint32_t pred = 0xffffffffu, succ = 0xffffffffu;
         if (active) {
 #pragma unroll
             for (int i = 0; i < CMAX; ++i)
-                if (i < cnt) {
-                    int es;
-                    int32_t eF;
-                    entry(lo + i, c, es, eF);
-                    v[i] = (int32_t)(((uint32_t)eF << 10) | (uint32_t)es);
-                }
+                if (i < cnt) v[i] = (int32_t)stk[(lo + i) * NT + me];
             int b = c - 1;
             while (b >= 0 && bhi[slot(b)] == blo[slot(b)]) --b;
-            if (b >= 0) {
-                int es;
-                int32_t eF;
-                entry(bhi[slot(b)] - 1, b, es, eF);
-                pred = ((uint32_t)eF << 10) | (uint32_t)es;
-            }
+            if (b >= 0) pred = stk[(bhi[slot(b)] - 1) * NT + slot(b)];
             b = c + 1;
             while (b < T && bhi[slot(b)] == blo[slot(b)]) ++b;
-            if (b < T) {
-                int es;
-                int32_t eF;
-                entry(blo[slot(b)], b, es, eF);
-                succ = ((uint32_t)eF << 10) | (uint32_t)es;
-            }
+            if (b < T) succ = stk[blo[slot(b)] * NT + slot(b)];
         }
         __syncthreads();  // the hull stacks are free from here on
-        // at[pos(p)]: entry whose interval starts at position p.  Positions are
-        // permuted within aligned groups of four by the chunk of the group, so
-        // the threads of one wave (four chunks, same offset) hit four bank groups.
+        // at[p*W + line]: entry whose interval starts at position p
         uint32_t* at = stk;
         uint32_t* tail = reinterpret_cast<uint32_t*>(blo);  // [NT] last entry at or before a chunk's end
-        const uint32_t mC = ((1u << 20) + (uint32_t)C - 1) / (uint32_t)C;  // (p * mC) >> 20 == p / C for p < 4096
-        auto pos = [&](int p) { return (p ^ (int)((((uint32_t)p & ~3u) * mC >> 20) & 3u)) * W + line; };
         if (active) {
 #pragma unroll
             for (int i = 0; i < CMAX; ++i)
-                if (p0 + i < p1) at[pos(p0 + i)] = 0xffffffffu;
+                if (p0 + i < p1) at[(p0 + i) * W + line] = 0xffffffffu;
         }
         __syncthreads();
         // first integer p where entry b is strictly below entry a (a left of b):
@@ -419,7 +408,7 @@ __global__ __launch_bounds__(W* T) void k_edt_lines(const int32_t* __restrict__ 
                     const uint32_t e = (uint32_t)v[i];
                     const uint32_t nx = i + 1 < cnt ? (uint32_t)v[min(i + 1, CMAX - 1)] : succ;
                     const int xhi = nx == 0xffffffffu ? L : min(max(start(e, nx), 0), L);
-                    if (xlo < xhi) at[pos(xlo)] = e;
+                    if (xlo < xhi) at[xlo * W + line] = e;
                     xlo = max(xlo, xhi);
                 }
         }
@@ -429,7 +418,7 @@ __global__ __launch_bounds__(W* T) void k_edt_lines(const int32_t* __restrict__ 
 #pragma unroll
             for (int i = 0; i < CMAX; ++i)
                 if (p0 + i < p1) {
-                    const uint32_t e = at[pos(p0 + i)];
+                    const uint32_t e = at[(p0 + i) * W + line];
                     last = e != 0xffffffffu ? e : last;
                     v[i] = (int32_t)last;
                 }
@@ -442,19 +431,15 @@ __global__ __launch_bounds__(W* T) void k_edt_lines(const int32_t* __restrict__ 
 #pragma unroll
                 for (int i = 0; i < CMAX; ++i) v[i] = g;
             } else {
-                uint32_t carry = 0xffffffffu;
-#pragma unroll
-                for (int b = 0; b < T; ++b)
-                    if (b < c) {
-                        const uint32_t e = tail[slot(b)];
-                        carry = e != 0xffffffffu ? e : carry;
-                    }
+                uint32_t carry = 0xffffffffu;  // last entry before p0 (position 0 always holds one)
+                if ((uint32_t)v[0] == 0xffffffffu)
+                    for (int b = c - 1; b >= 0 && carry == 0xffffffffu; --b) carry = tail[slot(b)];
 #pragma unroll
                 for (int i = 0; i < CMAX; ++i) {
                     const int p = p0 + i;
                     if (p < p1) {
                         uint32_t e = (uint32_t)v[i];
-                        if (e == 0xffffffffu) e = carry;  // position 0 always holds an entry
+                        if (e == 0xffffffffu) e = carry;
                         const int es = (int)(e & 1023u);
                         const int32_t eF = (int32_t)(e >> 10);
                         const int32_t g = eF - (int32_t)__umul24(2u * p, es) + (int32_t)__umul24(p, p);
@@ -594,14 +579,16 @@ hipError_t launch_lines_t(const int32_t* src, int32_t* dst, const EdtVol* table,
 {
     const uint32_t grid = round8(tiles);
     const int kmax = (Lmax + T - 1) / T;  // <= CMAX by the choice of shape
-    const size_t lds = (size_t)kmax * W * T * 4 + (size_t)W * T * 4 + (size_t)W * 12;
+    // hull stacks; with COMPACT the same words later hold Lmax rows of W start entries
+    const int region = kmax * W * T;
+    const size_t lds = (size_t)region * 4 + (size_t)W * T * 4 + (size_t)W * 12;
     const void* fn = reinterpret_cast<const void*>(&k_edt_lines<W, T, CMAX, COMPACT>);
     if (lds > 65536) {
         hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
     hipLaunchKernelGGL((k_edt_lines<W, T, CMAX, COMPACT>), dim3(grid), dim3(W * T), lds, st, src, dst, table, nvol,
-                       one, axis, (uint32_t)tiles, grid, kmax);
+                       one, axis, (uint32_t)tiles, grid, region);
     return hipGetLastError();
 }
 
