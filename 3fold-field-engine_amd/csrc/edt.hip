@@ -253,18 +253,35 @@ __global__ __launch_bounds__(W* T) void k_edt_lines(const int32_t* __restrict__ 
         }
     }
 
+    // 0. constant lines (all zero, all at the sentinel, ...) are their own
+    // result: found first, their threads skip the hull, merge and output.
+    int32_t mn = INT_MAX, mx = INT_MIN;
+    if (active) {
+#pragma unroll
+        for (int i = 0; i < CMAX; ++i)
+            if (p0 + i < p1) {
+                mn = min(mn, v[i]);
+                mx = max(mx, v[i]);
+            }
+    }
+    if (c == 0) {
+        lval[line] = mn;
+        lflat[line] = 1;
+    }
+    __syncthreads();
+    if (active && p0 < p1 && !(mn == mx && mn == lval[line])) lflat[line] = 0;
+    __syncthreads();
+    const bool busy = active && !lflat[line];
+
     // 1. hull of this chunk
     int k = -1, s_top = 0, s_sec = 0;
     int32_t F_top = 0, F_sec = 0;
-    int32_t mn = INT_MAX, mx = INT_MIN;
-    if (active) {
+    if (busy) {
 #pragma unroll
         for (int i = 0; i < CMAX; ++i) {
             const int q = p0 + i;
             if (q < p1) {
                 const int32_t fq = v[i];
-                mn = min(mn, fq);
-                mx = max(mx, fq);
                 if (fq < sentinel) {
                     const int32_t Fq = fq + q * q;
                     while (k >= 1) {
@@ -286,16 +303,11 @@ __global__ __launch_bounds__(W* T) void k_edt_lines(const int32_t* __restrict__ 
     }
     blo[me] = 0;
     bhi[me] = (uint8_t)(k + 1);
-    if (c == 0) {
-        lval[line] = mn;
-        lflat[line] = 1;
-    }
     __syncthreads();
-    if (active && p0 < p1 && !(mn == mx && mn == lval[line])) lflat[line] = 0;
 
     // 2. tree merge of the chunk hulls
     for (int h = 1; h < T; h <<= 1) {
-        if (active && (c & (2 * h - 1)) == h) {
+        if (busy && (c & (2 * h - 1)) == h) {
             const int a0 = c - h, a2 = min(c + h, T);
             int ta = c - 1;
             while (ta >= a0 && bhi[slot(ta)] == blo[slot(ta)]) --ta;
@@ -367,7 +379,7 @@ __global__ __launch_bounds__(W* T) void k_edt_lines(const int32_t* __restrict__ 
         // written at or before it.  No data-dependent loops over positions.
         const int lo = blo[me], cnt = bhi[me] - lo;
         uint32_t pred = 0xffffffffu, succ = 0xffffffffu;
-        if (active) {
+        if (busy) {
 #pragma unroll
             for (int i = 0; i < CMAX; ++i)
                 if (i < cnt) v[i] = (int32_t)stk[(lo + i) * NT + me];
@@ -382,7 +394,7 @@ __global__ __launch_bounds__(W* T) void k_edt_lines(const int32_t* __restrict__ 
         // at[p*W + line]: entry whose interval starts at position p
         uint32_t* at = stk;
         uint32_t* tail = reinterpret_cast<uint32_t*>(blo);  // [NT] last entry at or before a chunk's end
-        if (active) {
+        if (busy) {
 #pragma unroll
             for (int i = 0; i < CMAX; ++i)
                 if (p0 + i < p1) at[(p0 + i) * W + line] = 0xffffffffu;
@@ -400,7 +412,7 @@ __global__ __launch_bounds__(W* T) void k_edt_lines(const int32_t* __restrict__ 
             q += r < 0 ? -1 : (r >= d ? 1 : 0);
             return q + 1;
         };
-        if (active && cnt > 0) {
+        if (busy && cnt > 0) {
             int xlo = pred == 0xffffffffu ? 0 : min(max(start(pred, (uint32_t)v[0]), 0), L);
 #pragma unroll
             for (int i = 0; i < CMAX; ++i)
@@ -414,7 +426,7 @@ __global__ __launch_bounds__(W* T) void k_edt_lines(const int32_t* __restrict__ 
         }
         __syncthreads();
         uint32_t last = 0xffffffffu;
-        if (active) {
+        if (busy) {
 #pragma unroll
             for (int i = 0; i < CMAX; ++i)
                 if (p0 + i < p1) {
@@ -425,26 +437,20 @@ __global__ __launch_bounds__(W* T) void k_edt_lines(const int32_t* __restrict__ 
         }
         tail[me] = last;
         __syncthreads();
-        if (active && p0 < p1) {
-            if (lflat[line]) {
-                const int32_t g = lval[line];
+        if (busy && p0 < p1) {
+            uint32_t carry = 0xffffffffu;  // last entry before p0 (position 0 always holds one)
+            if ((uint32_t)v[0] == 0xffffffffu)
+                for (int b = c - 1; b >= 0 && carry == 0xffffffffu; --b) carry = tail[slot(b)];
 #pragma unroll
-                for (int i = 0; i < CMAX; ++i) v[i] = g;
-            } else {
-                uint32_t carry = 0xffffffffu;  // last entry before p0 (position 0 always holds one)
-                if ((uint32_t)v[0] == 0xffffffffu)
-                    for (int b = c - 1; b >= 0 && carry == 0xffffffffu; --b) carry = tail[slot(b)];
-#pragma unroll
-                for (int i = 0; i < CMAX; ++i) {
-                    const int p = p0 + i;
-                    if (p < p1) {
-                        uint32_t e = (uint32_t)v[i];
-                        if (e == 0xffffffffu) e = carry;
-                        const int es = (int)(e & 1023u);
-                        const int32_t eF = (int32_t)(e >> 10);
-                        const int32_t g = eF - (int32_t)__umul24(2u * p, es) + (int32_t)__umul24(p, p);
-                        v[i] = e == 0xffffffffu ? sentinel : min(g, sentinel);
-                    }
+            for (int i = 0; i < CMAX; ++i) {
+                const int p = p0 + i;
+                if (p < p1) {
+                    uint32_t e = (uint32_t)v[i];
+                    if (e == 0xffffffffu) e = carry;
+                    const int es = (int)(e & 1023u);
+                    const int32_t eF = (int32_t)(e >> 10);
+                    const int32_t g = eF - (int32_t)__umul24(2u * p, es) + (int32_t)__umul24(p, p);
+                    v[i] = e == 0xffffffffu ? sentinel : min(g, sentinel);
                 }
             }
         }
