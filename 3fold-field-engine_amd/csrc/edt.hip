@@ -93,13 +93,23 @@ __global__ __launch_bounds__(256) void k_edt_zpass(const uint8_t* __restrict__ m
     const uint8_t* m = masks + V.mask + row * nz;
     const int nseg = (nz + 63) >> 6;
     uint64_t w0 = 0, valid = 0;
-    for (int s = 0; s < nseg; ++s) {
-        int p = (s << 6) + lane;
-        bool in = p < nz;
-        bool hit = in && m[p] != 0;
-        uint64_t b = __ballot(site_one ? hit : (in && !hit));
-        uint64_t v = __ballot(in);
-        if (lane == s) { w0 = b; valid = v; }
+    for (int s0 = 0; s0 < nseg; s0 += 8) {  // eight segments' loads in flight before their ballots
+        uint8_t by[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int p = ((s0 + u) << 6) + lane;
+            by[u] = p < nz ? m[p] : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int s = s0 + u;
+            const int p = (s << 6) + lane;
+            const bool in = p < nz;
+            const bool hit = in && by[u] != 0;
+            const uint64_t b = __ballot(site_one ? hit : (in && !hit));
+            const uint64_t v = __ballot(in);
+            if (lane == s) { w0 = b; valid = v; }
+        }
     }
     const uint64_t w1 = ~w0 & valid;
     const uint64_t seg0 = __ballot(w0 != 0), seg1 = __ballot(w1 != 0);

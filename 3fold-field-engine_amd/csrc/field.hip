@@ -650,7 +650,7 @@ extern "C" int fe_field_build(const void* descs, int R, int B, const int32_t* sb
     FE_TRY(hipGetLastError());
     const int blocks = tile_waves > 0 ? (tile_waves + 3) / 4 : 2048;
     if (B == 4) hipLaunchKernelGGL(k_tiles<1>, dim3(blocks), dim3(256), 0, st, d, R);
-    else hipLaunchKernelGGL(k_tiles<8>, dim3(blocks), dim3(256), 0, st, d, R);
+    else hipLaunchKernelGGL(k_tiles<4>, dim3(blocks), dim3(256), 0, st, d, R);  // 4 voxels per lane: 8 waves/SIMD
     FE_TRY(hipGetLastError());
     return FE_OK;
 }
@@ -679,7 +679,7 @@ extern "C" int fe_field_tiles_ids(const void* host_desc, const int32_t* ids, int
     hipStream_t st = static_cast<hipStream_t>(stream);
     const unsigned blocks = (unsigned)((n_active + 3) / 4);
     if (D.B == 4) hipLaunchKernelGGL(k_tiles_ids<1>, dim3(blocks), dim3(256), 0, st, D, ids, n_active);
-    else hipLaunchKernelGGL(k_tiles_ids<8>, dim3(blocks), dim3(256), 0, st, D, ids, n_active);
+    else hipLaunchKernelGGL(k_tiles_ids<4>, dim3(blocks), dim3(256), 0, st, D, ids, n_active);
     FE_TRY(hipGetLastError());
     return FE_OK;
 }

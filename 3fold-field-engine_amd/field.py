@@ -174,7 +174,7 @@ class FieldPlan:
         self.ch_prefix = torch.tensor(np.concatenate([[0], np.cumsum(nch)]).astype(np.int32), device=self.device)
         self.total_sb, self.total_ch = int(sum(nsb)), int(sum(nch))
         self.chunk_count = torch.empty(self.total_ch, dtype=torch.int32, device=self.device)
-        self.tile_waves = 4096
+        self.tile_waves = 16384  # waves of the persistent tile kernel (measured best for one 512^3 field)
 
     def __len__(self):
         return len(self.items)
