@@ -30,6 +30,11 @@ SIGNATURES = {
     "fe_edt_batch_create": (P, [P, P, P, P, ct.c_int, ct.c_int, I64, I64, P]),
     "fe_edt_batch_run": (ct.c_int, [P, P, P, P, P]),
     "fe_edt_batch_destroy": (None, [P]),
+    "fe_octree_build": (P, [P, ct.c_int, ct.c_int, ct.c_int, P, P, P]),
+    "fe_octree_info": (ct.c_int, [P, P, P, P]),
+    "fe_octree_copy": (ct.c_int, [P, P, P, P, P, P]),
+    "fe_octree_free": (None, [P]),
+    "fe_octree_sample": (ct.c_int, [P, P, P, P, P, I64, F64, P, I64, P, P, P]),
     # field build
     "fe_field_build": (ct.c_int, [P, ct.c_int, ct.c_int, P, ct.c_int, P, ct.c_int, P, ct.c_int, P]),
     "fe_field_classify": (ct.c_int, [P, ct.c_int, ct.c_int, P, ct.c_int, P]),

@@ -78,6 +78,27 @@ void fe_edt_batch_destroy(void *handle);
 /* Signed output of edt_persistent_pair_v1/edt.cu:36-42 from a device pair. */
 int fe_edt_signed_device(const int32_t *pair, int64_t n, double pitch, float *out, void *stream);
 
+/* ---------------------------------------------------------------- 2-bit adaptive block tree */
+/* Replaces the Python builder and lookups of field_adaptive_blocks_v1.py:20-138
+ * (build :97-138, sample :52-82).  dense: device float32 (nx,ny,nz); kinds[4]:
+ * device int32 dense-source kinds at block sides 8, 4, 2, 1 (fe_dense_classify
+ * with margin f32(sqrt(3)*side*pitch)).  build returns a handle or NULL with
+ * *status 1 (invalid) / 2 (runtime).  info: sizes[4] = node count, value count,
+ * packed bytes, checkpoints; counts[16] and first[16] per level (side 8,4,2,1)
+ * and code.  copy: device destinations (any may be NULL).  sample: packed
+ * codes zero-padded to 64-byte groups, device checkpoints, values,
+ * root_shape[3] (device int32), in-domain int64 coordinates [n][3]; *flag (a
+ * device int) is set to 1 if the tree is malformed. */
+void *fe_octree_build(const float *dense, int nx, int ny, int nz, const int32_t *const *kinds, int *status,
+                      void *stream);
+int fe_octree_info(const void *handle, int64_t *sizes, int64_t *counts, int64_t *first);
+int fe_octree_copy(const void *handle, uint8_t *packed, int32_t *branch_prefix, int32_t *value_prefix, float *values,
+                   void *stream);
+void fe_octree_free(void *handle);
+int fe_octree_sample(const uint8_t *packed, const int32_t *branch_prefix, const int32_t *value_prefix,
+                     const float *values, const int32_t *root_shape, int64_t root_count, double pitch,
+                     const int64_t *ijk, int64_t n, float *out, int *flag, void *stream);
+
 /* ---------------------------------------------------------------- field build */
 /*
  * `descs` is a device table of R field descriptors (160 bytes each: grid

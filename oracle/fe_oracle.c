@@ -732,3 +732,112 @@ void or_query(const fe_node *nodes, const uint32_t *payload, const int32_t shape
         out[i] = (float)acc;
     }
 }
+
+/* ---------------------------------------------------------------- 2-bit adaptive block tree */
+/* Node codes in breadth-first order, FE/field_adaptive_blocks_v1.py:97-138:
+ * the roots are the 8-voxel blocks in C order; every node (x, y, z, side) is
+ * 0 when it lies outside the domain, else its dense-source kind at that side
+ * (FE/faltkarna_v1_mesh_to_sdf.py:602-632 with margin f32(sqrt(3)*side*pitch))
+ * decides: exterior 0, interior 1, unresolved side-1 node 3 (a value leaf
+ * holding dense[x,y,z]), unresolved larger node 2 (a branch whose children
+ * (x+i*h, y+j*h, z+k*h, h), h = side/2, are appended in (i,j,k) order).
+ * Returns the node count (codes[0..count)), -1 if cap or vcap is exceeded.
+ * hist[4][4]: per level (side 8,4,2,1) the count of each code. */
+int64_t or_octree_build(const float *dense, int nx, int ny, int nz, double pitch, uint8_t *codes, int64_t cap,
+                        float *values, int64_t vcap, int64_t *nvalues, int64_t *hist)
+{
+    static const int sides[4] = {8, 4, 2, 1};
+    int32_t *lv[4];
+    int nb[4][3];
+    for (int l = 0; l < 4; ++l) {
+        int B = sides[l];
+        nb[l][0] = nblk(nx, B);
+        nb[l][1] = nblk(ny, B);
+        nb[l][2] = nblk(nz, B);
+        lv[l] = (int32_t *)malloc(sizeof(int32_t) * (size_t)nb[l][0] * nb[l][1] * nb[l][2]);
+        or_classify_dense(dense, nx, ny, nz, B, (float)(sqrt(3.0) * B * pitch), lv[l]);
+    }
+    for (int i = 0; i < 16; ++i) hist[i] = 0;
+    int32_t *qx = (int32_t *)malloc(sizeof(int32_t) * (size_t)cap), *qy = (int32_t *)malloc(sizeof(int32_t) * (size_t)cap),
+            *qz = (int32_t *)malloc(sizeof(int32_t) * (size_t)cap);
+    int8_t *ql = (int8_t *)malloc((size_t)cap);
+    int64_t tail = 0, nv = 0, result = -1;
+    for (int i = 0; i < nb[0][0] && tail <= cap; ++i)
+        for (int j = 0; j < nb[0][1]; ++j)
+            for (int k = 0; k < nb[0][2]; ++k) {
+                if (tail >= cap) goto done;
+                qx[tail] = 8 * i; qy[tail] = 8 * j; qz[tail] = 8 * k; ql[tail] = 0; ++tail;
+            }
+    for (int64_t head = 0; head < tail; ++head) {
+        int x = qx[head], y = qy[head], z = qz[head], l = ql[head], s = sides[l];
+        uint8_t code;
+        if (x >= nx || y >= ny || z >= nz) code = 0;
+        else {
+            int32_t kind = lv[l][((int64_t)(x / s) * nb[l][1] + y / s) * nb[l][2] + z / s];
+            if (kind) code = kind > 0 ? 0 : 1;
+            else if (s == 1) {
+                if (nv >= vcap) goto done;
+                code = 3;
+                values[nv++] = dense[((int64_t)x * ny + y) * nz + z];
+            } else {
+                code = 2;
+                int h = s / 2;
+                for (int c = 0; c < 8; ++c) {
+                    if (tail >= cap) goto done;
+                    qx[tail] = x + ((c >> 2) & 1) * h; qy[tail] = y + ((c >> 1) & 1) * h; qz[tail] = z + (c & 1) * h;
+                    ql[tail] = (int8_t)(l + 1); ++tail;
+                }
+            }
+        }
+        codes[head] = code;
+        hist[l * 4 + code] += 1;
+    }
+    result = tail;
+done:
+    *nvalues = nv;
+    free(qx); free(qy); free(qz); free(ql);
+    for (int l = 0; l < 4; ++l) free(lv[l]);
+    return result;
+}
+
+/* Number of nodes before `node` whose code is `code`, from the checkpoint of
+ * its 64-byte group plus the codes in between: FE/field_adaptive_blocks_v1.py:35-50. */
+static int64_t octree_rank(const uint8_t *packed, const int32_t *prefix, int64_t node, int code)
+{
+    int64_t byte = node / 4, group = byte / 64, rank = prefix[group];
+    for (int64_t b = group * 64; b < byte; ++b)
+        for (int f = 0; f < 4; ++f) rank += ((packed[b] >> (2 * f)) & 3) == code;
+    for (int f = 0; f < (int)(node % 4); ++f) rank += ((packed[byte] >> (2 * f)) & 3) == code;
+    return rank;
+}
+
+/* Tree lookup of FE/field_adaptive_blocks_v1.py:52-82: descend from the root
+ * block; exterior/interior leaves give +/- f32(((1.0 * sqrt(3)) * size) * pitch)
+ * (f64 product), value leaves their stored sample.  Returns 0, or 1 when a
+ * branch would go below side 1. */
+int or_octree_sample(const uint8_t *packed, const int32_t *branch_prefix, const int32_t *value_prefix,
+                     const float *values, const int32_t root_shape[3], int64_t root_count, double pitch,
+                     const int64_t *ijk, int64_t n, float *out)
+{
+    for (int64_t q = 0; q < n; ++q) {
+        int64_t i = ijk[3 * q], j = ijk[3 * q + 1], k = ijk[3 * q + 2];
+        int64_t node = ((i / 8) * root_shape[1] + j / 8) * root_shape[2] + k / 8, size = 8;
+        for (;;) {
+            int code = (packed[node / 4] >> (2 * (node % 4))) & 3;
+            if (code < 2) {
+                out[q] = (float)(((code == 0 ? 1.0 : -1.0) * sqrt(3.0)) * (double)size * pitch);
+                break;
+            }
+            if (code == 3) {
+                out[q] = values[octree_rank(packed, value_prefix, node, 3)];
+                break;
+            }
+            int64_t half = size / 2;
+            if (half < 1) return 1;
+            int64_t child = 4 * ((i / half) % 2) + 2 * ((j / half) % 2) + (k / half) % 2;
+            node = root_count + 8 * octree_rank(packed, branch_prefix, node, 2) + child;
+            size = half;
+        }
+    }
+    return 0;
+}

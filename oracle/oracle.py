@@ -61,6 +61,9 @@ def lib():
             "or_csg_execute": (ct.c_int, [P, P, P, P, ct.c_int32, P, P, ct.c_int32, P]),
             "or_samples_at": (None, [P, P, P, ct.c_int64, P]),
             "or_query": (None, [P, P, P, P, ct.c_double, P, ct.c_int64, P]),
+            "or_octree_build": (ct.c_int64, [P, ct.c_int, ct.c_int, ct.c_int, ct.c_double, P, ct.c_int64, P,
+                                             ct.c_int64, ct.POINTER(ct.c_int64), P]),
+            "or_octree_sample": (ct.c_int, [P, P, P, P, P, ct.c_int64, ct.c_double, P, ct.c_int64, P]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -230,3 +233,52 @@ def ref_lib(name):
     """A reference C++ native compiled by `make -C oracle ref` (None if absent)."""
     path = REF_DIR / f"libref_{name}.so"
     return ct.CDLL(str(path)) if path.exists() else None
+
+
+# ---------------------------------------------------------------- 2-bit adaptive block tree
+OCT_LOOKUP = np.array([[sum(((byte >> (2 * i)) & 3) == code for i in range(4)) for byte in range(256)]
+                       for code in range(4)], dtype=np.int32)
+
+
+def octree_build(dense, pitch):
+    """FE/field_adaptive_blocks_v1.py:97-138: node codes and values from the C
+    restatement, then the packing (four codes per byte, code i at bits 2i) and
+    the rank checkpoints every 64 bytes restated in numpy.  Returns a dict with
+    the reference's AdaptiveBlocks fields and the per-level code counts."""
+    d = np.ascontiguousarray(dense, np.float32)
+    nx, ny, nz = d.shape
+    root_shape = tuple(-(-n // 8) for n in d.shape)
+    cap = 9 * int(np.prod(root_shape)) * 8 + 64
+    while True:
+        codes = np.empty(cap, np.uint8)
+        values = np.empty(max(d.size, 1), np.float32)
+        nv = ct.c_int64(0)
+        hist = np.zeros(16, np.int64)
+        count = lib().or_octree_build(_p(d), nx, ny, nz, float(pitch), _p(codes), cap, _p(values), values.size,
+                                      ct.byref(nv), _p(hist))
+        if count >= 0:
+            break
+        cap *= 4
+    codes = codes[:count]
+    padded = np.pad(codes, (0, (-len(codes)) % 4))
+    packed = np.bitwise_or.reduce(padded.reshape(-1, 4) << np.array([0, 2, 4, 6], np.uint8), axis=1).astype(np.uint8)
+    prefixes = []
+    for code in (2, 3):
+        cumulative = np.concatenate(([0], np.cumsum(OCT_LOOKUP[code, packed], dtype=np.int32)))
+        prefixes.append(cumulative[np.arange(0, len(packed) + 1, 64)].astype(np.int32))
+    return dict(shape=d.shape, pitch=float(pitch), root_shape=root_shape, root_count=int(np.prod(root_shape)),
+                node_count=int(count), codes=codes, packed=packed, branch_prefix=prefixes[0],
+                value_prefix=prefixes[1], values=values[:nv.value].copy(), level_counts=hist.reshape(4, 4))
+
+
+def octree_sample(tree, indices):
+    """FE/field_adaptive_blocks_v1.py:52-82 through the C restatement."""
+    ijk = np.ascontiguousarray(indices, np.int64)
+    out = np.empty(len(ijk), np.float32)
+    rs = np.ascontiguousarray(tree["root_shape"], np.int32)
+    vals = tree["values"] if len(tree["values"]) else np.zeros(1, np.float32)
+    status = lib().or_octree_sample(_p(tree["packed"]), _p(tree["branch_prefix"]), _p(tree["value_prefix"]),
+                                    _p(vals), _p(rs), tree["root_count"], tree["pitch"], _p(ijk), len(ijk), _p(out))
+    if status:
+        raise RuntimeError("Invalid tree depth")
+    return out
