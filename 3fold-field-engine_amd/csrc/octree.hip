@@ -463,7 +463,7 @@ extern "C" void fe_octree_free(void* handle) noexcept { oct_free(static_cast<Oct
 
 // Lookups of FE/field_adaptive_blocks_v1.py:52-82.  packed: device codes,
 // zero-padded to a multiple of 64 bytes; prefixes: device checkpoints
-// (packed bytes / 64 + 1 each); ijk: device int64 [n][3] inside the domain
+// (packed bytes / 64 + 1 each); root_shape: host int32[3]; ijk: device int64 [n][3] inside the domain
 // (checked by the caller).  Returns 1 if the tree is malformed (a branch below
 // side 1), 0 otherwise; `flag` is a device int used for that report.
 extern "C" int fe_octree_sample(const uint8_t* packed, const int32_t* branch_prefix, const int32_t* value_prefix,

@@ -70,8 +70,7 @@ class AdaptiveBlocks:
             padded[:len(self.packed)] = self.packed
             vals = self.values if len(self.values) else np.zeros(1, np.float32)
             self._dev = (torch.from_numpy(padded).to(dev), torch.from_numpy(self.branch_prefix.copy()).to(dev),
-                         torch.from_numpy(self.value_prefix.copy()).to(dev), torch.from_numpy(vals.copy()).to(dev),
-                         torch.tensor(self.root_shape, dtype=torch.int32, device=dev))
+                         torch.from_numpy(self.value_prefix.copy()).to(dev), torch.from_numpy(vals.copy()).to(dev))
         return self._dev
 
     def sample(self, indices):
@@ -80,12 +79,13 @@ class AdaptiveBlocks:
         ijk = np.asarray(indices, dtype=np.int64)
         if ijk.ndim != 2 or ijk.shape[1] != 3 or np.any(ijk < 0) or np.any(ijk >= self.shape):
             raise ValueError("Expected in-domain integer grid coordinates[N,3]")
-        packed, bpre, vpre, vals, rs = self._device()
+        packed, bpre, vpre, vals = self._device()
+        rs = np.ascontiguousarray(self.root_shape, np.int32)
         q = torch.from_numpy(np.ascontiguousarray(ijk)).to(packed.device)
         out = torch.empty(len(ijk), dtype=torch.float32, device=packed.device)
         flag = torch.zeros(1, dtype=torch.int32, device=packed.device)
         status = _lib.lib().fe_octree_sample(_lib.ptr(packed), _lib.ptr(bpre), _lib.ptr(vpre), _lib.ptr(vals),
-                                             _lib.ptr(rs), self.root_count, self.pitch, _lib.ptr(q), len(ijk),
+                                             rs.ctypes.data, self.root_count, self.pitch, _lib.ptr(q), len(ijk),
                                              _lib.ptr(out), _lib.ptr(flag), _lib.stream_handle(None))
         if status:
             raise RuntimeError(f"octree sample status {status}")

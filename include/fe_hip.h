@@ -87,7 +87,7 @@ int fe_edt_signed_device(const int32_t *pair, int64_t n, double pitch, float *ou
  * packed bytes, checkpoints; counts[16] and first[16] per level (side 8,4,2,1)
  * and code.  copy: device destinations (any may be NULL).  sample: packed
  * codes zero-padded to 64-byte groups, device checkpoints, values,
- * root_shape[3] (device int32), in-domain int64 coordinates [n][3]; *flag (a
+ * root_shape[3] (host int32), in-domain int64 coordinates [n][3]; *flag (a
  * device int) is set to 1 if the tree is malformed. */
 void *fe_octree_build(const float *dense, int nx, int ny, int nz, const int32_t *const *kinds, int *status,
                       void *stream);
