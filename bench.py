@@ -204,6 +204,33 @@ def kernel_launches(fe, torch, n, reps=3):
     return {"n": n, "n_active": na, "n_blocks": meta.n_blocks, "copy_bytes_each_way": 1 << 30, "reps": reps}
 
 
+def bench_octree(fe, torch, dense, reps=3, queries=1 << 20):
+    """2-bit adaptive block tree of the 512^3 dense field: device build
+    (classification at sides 8/4/2/1, breadth-first levels, packing,
+    checkpoints) and random in-domain lookups."""
+    tree, hist = fe.octree.build(None, dense, 1.0)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        tree, hist = fe.octree.build(None, dense, 1.0)
+    torch.cuda.synchronize()
+    build_ms = (time.perf_counter() - t0) * 1e3 / reps
+    rng = np.random.default_rng(6)
+    ijk = np.stack([rng.integers(0, s, queries) for s in dense.shape], 1)
+    packed, bpre, vpre, vals = tree._device()
+    rs = np.ascontiguousarray(tree.root_shape, np.int32)
+    q = torch.from_numpy(ijk).to(dense.device)
+    out = torch.empty(queries, dtype=torch.float32, device=dense.device)
+    flag = torch.zeros(1, dtype=torch.int32, device=dense.device)
+    L = fe._lib.lib()
+    P = fe._lib.ptr
+    ms = event_ms(torch, lambda: L.fe_octree_sample(P(packed), P(bpre), P(vpre), P(vals), rs.ctypes.data,
+                                                    tree.root_count, tree.pitch, P(q), queries, P(out), P(flag),
+                                                    fe._lib.stream_handle(None)), reps)
+    return {"build_ms_incl_host_copies": build_ms, "nodes": tree.node_count, "values": int(len(tree.values)),
+            "stored_bytes": tree.stored_bytes(), "dense_bytes": int(dense.numel() * 4),
+            "lookups": queries, "lookup_ms": ms, "lookups_per_s": queries / (ms * 1e-3), "leaf_histogram": hist}
+
+
 def bench_serve(fe, torch, rates, count):
     """Poisson arrivals of 512^3 / 1024-primitive recipes into the field server."""
     rng = np.random.default_rng(5)
@@ -329,6 +356,8 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": profiled_traffic(dom, n),
                      "algorithmic_bytes": dbytes},
     }
+    if not args.quick:
+        out["octree"] = bench_octree(fe, torch, dense)
     occ_host = occ.cpu().numpy()
     out["config"]["occupancy_fraction"] = float(occ_host.mean())
     del plan, dense, pair, work
