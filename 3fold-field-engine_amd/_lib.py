@@ -30,6 +30,7 @@ SIGNATURES = {
     "fe_edt_batch_create": (P, [P, P, P, P, ct.c_int, ct.c_int, I64, I64, P]),
     "fe_edt_batch_run": (ct.c_int, [P, P, P, P, P]),
     "fe_edt_batch_destroy": (None, [P]),
+    "fe_checksum_pair": (ct.c_int, [P, I64, F64, P, P, P]),
     "fe_octree_build": (P, [P, ct.c_int, ct.c_int, ct.c_int, P, P, P]),
     "fe_octree_info": (ct.c_int, [P, P, P, P]),
     "fe_octree_copy": (ct.c_int, [P, P, P, P, P, P]),

@@ -707,6 +707,36 @@ extern "C" int fe_field_checksum(const void* descs, int r, double scale, int64_t
     return FE_OK;
 }
 
+// The two checksums of FE/faltkarna_v1.py:238-255 over a flat array of values
+// below zero: out32 accumulates every value with a float atomic add (order-
+// dependent, the negative control), out64 the int64 of f64(v) * scale
+// (order-independent).  Both must be zeroed by the caller.
+__global__ __launch_bounds__(256) void k_checksum_pair(const float* __restrict__ vals, int64_t n, double scale,
+                                                       float* __restrict__ out32, unsigned long long* __restrict__ out64)
+{
+    long long s = 0;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+        const float v = vals[i];
+        if (v < 0.0f) {
+            atomicAdd(out32, v);
+            s += (long long)((double)v * scale);
+        }
+    }
+    for (int d = 32; d >= 1; d >>= 1) s += __shfl_xor(s, d);
+    if ((threadIdx.x & 63) == 0) atomicAdd(out64, (unsigned long long)s);
+}
+
+extern "C" int fe_checksum_pair(const float* vals, int64_t n, double scale, float* out32, int64_t* out64,
+                                void* stream) noexcept
+{
+    if (!vals || !out32 || !out64 || n < 0) return FE_INVALID;
+    if (n == 0) return FE_OK;
+    hipLaunchKernelGGL(k_checksum_pair, dim3(1024), dim3(256), 0, static_cast<hipStream_t>(stream), vals, n, scale,
+                       out32, reinterpret_cast<unsigned long long*>(out64));
+    FE_TRY(hipGetLastError());
+    return FE_OK;
+}
+
 // Dense-source classify and tile extraction (FE/faltkarna_v1_mesh_to_sdf.py:660-692 on the device).
 extern "C" int fe_dense_classify(const float* dense, int nx, int ny, int nz, int B, float margin, int32_t* kind,
                                  void* stream) noexcept

@@ -372,28 +372,46 @@ def determinism_cert(varden):
     return dict(n_korningar=len(varden), varden=varden, bit_identiska=bool(all(v == varden[0] for v in varden)))
 
 
+def reduktion_checksum_single_process(tiles_npy_path, device="cuda:0"):
+    """FE/faltkarna_v1.py:368-386: one run of both checksums over the values < 0
+    of a saved tile array: (float32 atomic sum, int64 fixed-point sum / scale)."""
+    torch = _torch()
+    vals = np.load(tiles_npy_path, allow_pickle=False).reshape(-1).astype(np.float32)
+    dev = torch.device(device)
+    v = torch.from_numpy(vals).to(dev)
+    out32 = torch.zeros(1, dtype=torch.float32, device=dev)
+    out64 = torch.zeros(1, dtype=torch.int64, device=dev)
+    _check(_lib.lib().fe_checksum_pair(_lib.ptr(v), len(vals), float(FIXPT_SCALE), _lib.ptr(out32), _lib.ptr(out64),
+                                       _lib.stream_handle(None)), "checksum pair")
+    return float(out32.item()), float(out64.item()) / FIXPT_SCALE
+
+
 def volym_via_gpu_reduktion(sf, n_korningar=6, tmp_dir="/tmp"):
-    """FE/faltkarna_v1.py:389-415: the int64 fixed-point checksum of the tiles in
-    n separate processes must be bit-identical."""
+    """FE/faltkarna_v1.py:389-415: both checksums of the tiles in n separate
+    processes; the int64 fixed-point totals must be bit-identical, the float32
+    atomic totals are the order-dependent negative control."""
     path = os.path.join(tmp_dir, f"fe_amd_tiles_{os.getpid()}.npy")
     np.save(path, np.ascontiguousarray(sf.tiles, np.float32))
     pkg = os.path.dirname(os.path.abspath(__file__))
-    code = ("import sys,importlib.util,numpy as np\n"
+    code = ("import sys,importlib.util\n"
             f"spec=importlib.util.spec_from_file_location('fe_amd',{os.path.join(pkg, '__init__.py')!r},"
             f"submodule_search_locations=[{pkg!r}])\n"
             "m=importlib.util.module_from_spec(spec);sys.modules['fe_amd']=m;spec.loader.exec_module(m)\n"
-            f"print(m.field.checksum_tiles(np.load({path!r})))\n")
-    totals = []
+            f"r32,r64=m.field.reduktion_checksum_single_process({path!r})\n"
+            "print(repr(r32),repr(r64))\n")
+    f32_totals, i64_totals = [], []
     try:
         for _ in range(n_korningar):
             r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=300)
             if r.returncode != 0:
                 raise RuntimeError(f"subprocess reduction failed: {r.stderr[-2000:]}")
-            totals.append(int(r.stdout.strip().splitlines()[-1]) / FIXPT_SCALE)
+            parts = r.stdout.strip().splitlines()[-1].split()
+            f32_totals.append(float(parts[0]))
+            i64_totals.append(float(parts[1]))
     finally:
         os.unlink(path)
-    return dict(i64_fixpunkt_safe=determinism_cert(totals), n_active_voxels=int(sf.tiles.size),
-                n_processer=n_korningar)
+    return dict(f32_unsafe=determinism_cert(f32_totals), i64_fixpunkt_safe=determinism_cert(i64_totals),
+                n_active_voxels=int(sf.tiles.size), n_processer=n_korningar)
 
 
 def checksum_tiles(tiles, scale=FIXPT_SCALE):

@@ -78,6 +78,12 @@ void fe_edt_batch_destroy(void *handle);
 /* Signed output of edt_persistent_pair_v1/edt.cu:36-42 from a device pair. */
 int fe_edt_signed_device(const int32_t *pair, int64_t n, double pitch, float *out, void *stream);
 
+/* Both checksums of faltkarna_v1.py:238-255 (reduktion_checksum_single_process
+ * :368-386) over n device float32 values: *out32 (device float, zeroed by the
+ * caller) += every value < 0 by float atomics (order-dependent control),
+ * *out64 (device int64, zeroed) += int64(f64(v) * scale) for every v < 0. */
+int fe_checksum_pair(const float *vals, int64_t n, double scale, float *out32, int64_t *out64, void *stream);
+
 /* ---------------------------------------------------------------- 2-bit adaptive block tree */
 /* Replaces the Python builder and lookups of field_adaptive_blocks_v1.py:20-138
  * (build :97-138, sample :52-82).  dense: device float32 (nx,ny,nz); kinds[4]:
