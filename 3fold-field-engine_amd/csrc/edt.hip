@@ -5,20 +5,23 @@
 // result is min over s of f(s) + |p - s|^2, computed one axis at a time.  The
 // per-axis minimum is exact in integers, so the axis order does not change
 // the result; this implementation runs z first (the contiguous axis, straight
-// from the uint8 mask) and then y and x (strided axes, LDS-staged tiles).
+// from the uint8 mask) and then y and x (strided axes).
 //
 //   z pass  one wave per row: 64-bit ballots of the site flags, nearest site
 //           left/right from clz/ctz, squared 1-D distance.  Both polarities of
 //           a pair come from one read of the mask.
-//   y,x     W adjacent lines (contiguous along z, so every row of the tile is
-//           one coalesced segment) are staged in LDS; T threads share a line,
-//           each owns a chunk of positions, builds the exact lower envelope
-//           of its chunk's parabolas (Felzenszwalb, pop test by int64 cross
-//           multiplication, no division) and then evaluates the envelopes of
-//           its own and neighbouring chunks at its positions, visiting a
-//           neighbour only while its lower bound min f + gap^2 can still win.
+//   y,x     a tile of W lines adjacent along z; T threads per line each load
+//           a chunk of positions into registers and build the exact lower
+//           envelope of the chunk's parabolas as a stack of packed 32-bit
+//           entries in LDS (pop test by an exact product comparison, no
+//           division); the chunk envelopes of a line are merged in log2(T)
+//           rounds; every surviving entry is written at the first position
+//           where it is the minimum, and each position takes the last entry
+//           written at or before it.  Constant lines are returned as they are.
 //   signed  FE/edt_persistent_pair_v1/edt.cu:36-42 semantics with IEEE
 //           correctly rounded f64 sqrt/mul and f32 sub/mul.
+//
+// A launch covers a table of volumes (one, or a batch of uneven volumes).
 #include "fe_common.hpp"
 #include <algorithm>
 #include <climits>
@@ -283,31 +286,33 @@ __global__ __launch_bounds__(W* T) void k_edt_lines(const int32_t* __restrict__ 
     __syncthreads();
     const bool busy = active && !lflat[line];
 
-    // 1. hull of this chunk
+    // 1. hull of this chunk (positions past the chunk end read as the sentinel,
+    // which never enters a hull, so one test covers both)
     int k = -1, s_top = 0, s_sec = 0;
     int32_t F_top = 0, F_sec = 0;
     if (busy) {
+        uint32_t* top = stk + me - NT;  // stack slot k lives at stk[k * NT + me]
 #pragma unroll
         for (int i = 0; i < CMAX; ++i) {
             const int q = p0 + i;
-            if (q < p1) {
-                const int32_t fq = v[i];
-                if (fq < sentinel) {
-                    const int32_t Fq = fq + q * q;
-                    while (k >= 1) {
-                        if (above<COMPACT>(Fq - F_top, s_top - s_sec, F_top - F_sec, q - s_top)) break;
-                        --k;
-                        s_top = s_sec;
-                        F_top = F_sec;
-                        if (k >= 1) entry(k - 1, c, s_sec, F_sec);
-                    }
-                    ++k;
-                    stk[k * NT + me] = pack_entry<COMPACT>(Fq, q, i);
-                    s_sec = s_top;
-                    F_sec = F_top;
-                    s_top = q;
-                    F_top = Fq;
+            const int32_t fq = q < p1 ? v[i] : sentinel;
+            if (fq < sentinel) {
+                const int32_t Fq = fq + (int32_t)__umul24((uint32_t)q, (uint32_t)q);
+                while (k >= 1) {
+                    if (above<COMPACT>(Fq - F_top, s_top - s_sec, F_top - F_sec, q - s_top)) break;
+                    --k;
+                    top -= NT;
+                    s_top = s_sec;
+                    F_top = F_sec;
+                    if (k >= 1) entry(k - 1, c, s_sec, F_sec);
                 }
+                ++k;
+                top += NT;
+                *top = pack_entry<COMPACT>(Fq, q, i);
+                s_sec = s_top;
+                F_sec = F_top;
+                s_top = q;
+                F_top = Fq;
             }
         }
     }
