@@ -10,7 +10,7 @@ from pathlib import Path
 LIB_PATH = Path(__file__).resolve().parent / "libfe_hip.so"
 
 P = ct.c_void_p
-I32, I64, F64 = ct.c_int32, ct.c_int64, ct.c_double
+I32, I64, F64, U32 = ct.c_int32, ct.c_int64, ct.c_double, ct.c_uint32
 
 # name: (restype, argtypes); every symbol is declared in include/fe_hip.h.
 SIGNATURES = {
@@ -62,6 +62,24 @@ SIGNATURES = {
     "csg_gather": (ct.c_int, [P, I64, P, I64, P, P, P]),
     "csg_execute": (ct.c_int, [P, P, P, P, I32, P, P, I32, P]),
     "fe_csg_device": (ct.c_int, [P, P, P, P, I32, P, P, I32, P, P]),
+    # mesh stage (reference-shaped)
+    "columns_create": (ct.c_int, [ct.c_char_p, P, U32, U32, U32, P, ct.POINTER(P)]),
+    "columns_query": (ct.c_int, [P, P, U32, P, P, P]),
+    "columns_query_compact": (ct.c_int, [P, P, U32, P, P, P, ct.POINTER(U32)]),
+    "columns_destroy": (ct.c_int, [ct.POINTER(P)]),
+    "column_mask_surface": (ct.c_int, [P, P, ct.c_int, P, P, ct.c_int, ct.c_int, ct.c_int, P, P]),
+    "fused_create": (ct.c_int, [ct.c_char_p, P, U32, P, P, ct.c_int, ct.c_int, ct.c_int, F64, P, ct.POINTER(P)]),
+    "fused_query": (ct.c_int, [P, P, P, P]),
+    "fused_destroy": (ct.c_int, [ct.POINTER(P)]),
+    # mesh stage (uncapped / device-resident)
+    "fe_mesh_create": (ct.c_int, [P, U32, P, P, ct.c_int, ct.c_int, ct.c_int, F64, P, ct.c_int, ct.POINTER(P)]),
+    "fe_gwn_create": (ct.c_int, [P, U32, P, ct.c_int, ct.c_int, ct.c_int, F64, ct.POINTER(P)]),
+    "fe_mesh_query": (ct.c_int, [P, P, P, P]),
+    "fe_mesh_destroy": (ct.c_int, [ct.POINTER(P)]),
+    "fe_mesh_run": (ct.c_int, [P, P]),
+    "fe_mesh_status": (ct.c_int, [P]),
+    "fe_mesh_buffers": (ct.c_int, [P] + [ct.POINTER(P)] * 7 + [ct.POINTER(ct.c_int)]),
+    "fe_gwn_device": (ct.c_int, [P, I64, P, I64, P, P, P, P]),
 }
 
 _lib = None

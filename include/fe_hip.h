@@ -200,6 +200,59 @@ int fe_csg_device(const void *const *nodes, const int64_t *counts, const void *c
                   const int64_t *lengths, int32_t fields, const int32_t *shape, const void *program,
                   int32_t instructions, float *output, void *stream);
 
+/* ---------------------------------------------------------------- mesh stage */
+/* Replace the CUDA column scan of cuda_columns_v1/columns.cu:50-113 (status 0
+ * ok, 1 invalid, 2 runtime, 3 foreign thread, 4 a query with more than `hits`
+ * hits).  input: triangles x 9 doubles (corners a, b, c); hits per query in
+ * face order, depths/signs [count][hits] (compact: [count][*width], *width =
+ * largest count).  backend "cuda-scan-v1" or "hip-scan-v1". */
+int columns_create(const char *backend, const double *input, uint32_t triangles, uint32_t capacity, uint32_t hits,
+                   const double *anchor, void **handle);
+int columns_query(void *handle, const double *xy, uint32_t count, int32_t *counts, double *depths, int32_t *signs);
+int columns_query_compact(void *handle, const double *xy, uint32_t count, int32_t *counts, double *depths,
+                          int32_t *signs, uint32_t *width);
+int columns_destroy(void **handle);
+
+/* Replaces column_mask_surface, column_mask_native_v1/mask.cu:17-37: mask and
+ * 6-neighbour surface from sorted composite keys (CSR offsets per column). */
+int column_mask_surface(const double *keys, const int32_t *signs, int hits, const int32_t *offsets,
+                        const double *fractions, int nx, int ny, int nz, uint8_t *mask, uint8_t *surface);
+
+/* Replace fused_create / fused_query / fused_destroy of
+ * mesh_fused_native_v1/field.cu:39-94 and mesh_fused_graph_v1/field.cu:18-79:
+ * column hits -> composite-key mask -> surface -> paired EDT -> signed
+ * distance, captured once into a graph and replayed per query.  `path` (the
+ * reference's ray-tracing program) is ignored: hits come from the column scan.
+ * Query status 4 (hit overflow / non-finite depth) and 1 (mask lacks a class)
+ * latch the handle. */
+int fused_create(const char *path, const double *corners, uint32_t triangles, const double *anchor, const double *xy,
+                 int nx, int ny, int nz, double pitch, const double *zs, void **handle);
+int fused_query(void *handle, uint8_t *solid, uint8_t *surface, float *distance);
+int fused_destroy(void **handle);
+
+/* The same stage without the reference's size caps (axes <= 4096, < 2^31
+ * voxels, 1..256 hits per column), and its open-mesh form: mask = |ordered
+ * generalized winding number| > .5 at `points` (nx*ny*nz x 3), as
+ * mesh_field_cuda_columns_v1.py:69-70 with generalized_winding_ordered_v1.py:11-32.
+ * fe_mesh_query also returns 5 when a winding point lies on a mesh corner. */
+int fe_mesh_create(const double *corners, uint32_t triangles, const double *anchor, const double *xy, int nx, int ny,
+                   int nz, double pitch, const double *zs, int hit_capacity, void **handle);
+int fe_gwn_create(const double *corners, uint32_t triangles, const double *points, int nx, int ny, int nz,
+                  double pitch, void **handle);
+int fe_mesh_query(void *handle, uint8_t *solid, uint8_t *surface, float *distance);
+int fe_mesh_destroy(void **handle);
+
+/* Device-resident use of a mesh handle: replay on a stream without copies,
+ * the last run's status once the stream is done, and the device buffers. */
+int fe_mesh_run(void *handle, void *stream);
+int fe_mesh_status(const void *handle);
+int fe_mesh_buffers(const void *handle, void **solid, void **surface, void **distance, void **pair, void **counts,
+                    void **keys, void **signs, int *cap);
+
+/* Winding numbers of device points against device corners (w, mask optional). */
+int fe_gwn_device(const double *corners, int64_t triangles, const double *points, int64_t n, double *w, uint8_t *mask,
+                  int *flags, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -841,3 +841,151 @@ int or_octree_sample(const uint8_t *packed, const int32_t *branch_prefix, const 
     }
     return 0;
 }
+
+/* ----------------------------------------------------------- mesh stage -- */
+
+/* Projected-column scan, FE/cuda_columns_v1/columns.cu:28-48 with the area
+ * floor of :61-67 (1e-12 * scale^2, scale = max(1, |x|, |y| over all
+ * corners)).  corners: ntri x 9 doubles (a, b, c).  For each query (x, y) the
+ * triangles are visited in face order; a triangle is hit when the query lies
+ * in its closed xy bounding box, its projected doubled area d clears the
+ * floor and the three edge functions l0, l1, l2 have the sign of d (zero
+ * allowed).  Depth z = (l0*az + l1*bz + l2*cz)/d, sign = sign(d).  The first
+ * `cap` hits are stored at [q*cap + k]; counts hold the full number, which
+ * may exceed cap (the reference's status-4 condition). */
+void or_columns_scan(const double *corners, int64_t ntri, const double *xy, int64_t nq, int cap,
+                     int32_t *counts, double *depths, int32_t *signs)
+{
+    double scale = 1.0;
+    for (int64_t t = 0; t < ntri; ++t)
+        for (int j = 0; j < 3; ++j) {
+            double ax = fabs(corners[9 * t + 3 * j]), ay = fabs(corners[9 * t + 3 * j + 1]);
+            if (ax > scale) scale = ax;
+            if (ay > scale) scale = ay;
+        }
+    const double floor_area = 1e-12 * scale * scale;
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t q = 0; q < nq; ++q) {
+        const double qx = xy[2 * q], qy = xy[2 * q + 1];
+        int32_t found = 0;
+        for (int64_t t = 0; t < ntri; ++t) {
+            const double *p = corners + 9 * t;
+            double ax = p[0], ay = p[1], az = p[2], bx = p[3], by = p[4], bz = p[5], cx = p[6], cy = p[7], cz = p[8];
+            double xmin = fmin(ax, fmin(bx, cx)), xmax = fmax(ax, fmax(bx, cx));
+            double ymin = fmin(ay, fmin(by, cy)), ymax = fmax(ay, fmax(by, cy));
+            if (qx < xmin || qx > xmax || qy < ymin || qy > ymax) continue;
+            double d = (bx - ax) * (cy - ay) - (by - ay) * (cx - ax);
+            if (!(fabs(d) > floor_area)) continue;
+            double l0 = (bx - qx) * (cy - qy) - (by - qy) * (cx - qx);
+            double l1 = (cx - qx) * (ay - qy) - (cy - qy) * (ax - qx);
+            double l2 = (ax - qx) * (by - qy) - (ay - qy) * (bx - qx);
+            int s = d > 0 ? 1 : -1;
+            if (l0 * s < 0 || l1 * s < 0 || l2 * s < 0) continue;
+            double z = (l0 * az + l1 * bz + l2 * cz) / d;
+            if (found < cap) {
+                depths[q * cap + found] = z;
+                signs[q * cap + found] = s;
+            }
+            ++found;
+        }
+        counts[q] = found;
+    }
+}
+
+/* Column mask and surface from per-column hits, the device chain of
+ * FE/mesh_fused_native_v1/field.cu:5-37 (hit_range, prepare_range,
+ * prepare_keys, classify_hits, fused_surface), which equals the host form of
+ * FE/column_mask_native_v1.py:27-33 with FE/column_mask_native_v1/mask.cu:5-16.
+ *   low/high = min/max stored depth (low 0, span 1 when there is none),
+ *   span = max(high - low, 1e-12),
+ *   key(hit) = (col + .25) + (.5*(z - low))/span,
+ *   fraction(z) = clamp(.25 + (.5*(zs[z] - low))/span, 0, .999),
+ *   voxel (col, z) solid iff the signs of hits with key > col + fraction sum
+ *   to a nonzero value;
+ *   surface = solid voxel on the grid border or with a 6-neighbour outside.
+ * Returns the reference's flag bits: 1 some voxel outside, 2 some voxel
+ * inside, 4 a count outside [0, cap], 8 a non-finite depth. */
+int or_column_mask(const int32_t *counts, const double *depths, const int32_t *signs, int cap, int nx, int ny, int nz,
+                   const double *zs, uint8_t *mask, uint8_t *surface)
+{
+    const int64_t columns = (int64_t)nx * ny, size = columns * nz;
+    int flags = 0;
+    double lo = INFINITY, hi = -INFINITY;
+    for (int64_t c = 0; c < columns; ++c) {
+        int n = counts[c];
+        if (n < 0 || n > cap) flags |= 4;
+        n = n < 0 ? 0 : (n > cap ? cap : n);
+        for (int j = 0; j < n; ++j) {
+            double z = depths[c * cap + j];
+            if (!isfinite(z)) flags |= 8;
+            else {
+                lo = fmin(lo, z);
+                hi = fmax(hi, z);
+            }
+        }
+    }
+    double span = 1.0;
+    if (!isfinite(lo)) lo = 0.0;
+    else span = fmax(hi - lo, 1e-12);
+    double *fr = (double *)malloc(sizeof(double) * (size_t)nz);
+    for (int z = 0; z < nz; ++z) {
+        double f = .25 + (.5 * (zs[z] - lo)) / span;
+        fr[z] = fmax(0.0, fmin(.999, f));
+    }
+    for (int64_t c = 0; c < columns; ++c) {
+        int n = counts[c];
+        n = n < 0 ? 0 : (n > cap ? cap : n);
+        const double base = (double)c + .25;
+        for (int z = 0; z < nz; ++z) {
+            const double q = (double)c + fr[z];
+            int sum = 0;
+            for (int j = 0; j < n; ++j) {
+                double key = base + (.5 * (depths[c * cap + j] - lo)) / span;
+                if (key > q) sum += signs[c * cap + j];
+            }
+            mask[c * nz + z] = sum != 0;
+            flags |= sum != 0 ? 2 : 1;
+        }
+    }
+    free(fr);
+    for (int64_t i = 0; i < size; ++i) {
+        int64_t z = i % nz, y = (i / nz) % ny, x = i / ((int64_t)ny * nz);
+        surface[i] = mask[i] && (x == 0 || x == nx - 1 || y == 0 || y == ny - 1 || z == 0 || z == nz - 1 ||
+                                 !mask[i - (int64_t)ny * nz] || !mask[i + (int64_t)ny * nz] || !mask[i - nz] ||
+                                 !mask[i + nz] || !mask[i - 1] || !mask[i + 1]);
+    }
+    return flags;
+}
+
+/* Ordered generalized winding number, FE/generalized_winding_ordered_v1.py:11-32:
+ * for each point, the sum over triangles in face order of
+ * atan2(a.(b x c), |a||b||c| + (a.b)|c| + (b.c)|a| + (c.a)|b|) with a, b, c
+ * the corners minus the point, divided by 2*pi.  Dot products add their
+ * three terms left to right.  Returns 1 when a point coincides with a corner
+ * (the reference's ValueError), else 0. */
+int or_gwn(const double *corners, int64_t ntri, const double *points, int64_t n, double *w)
+{
+    int bad = 0;
+#pragma omp parallel for schedule(static) reduction(| : bad)
+    for (int64_t i = 0; i < n; ++i) {
+        const double px = points[3 * i], py = points[3 * i + 1], pz = points[3 * i + 2];
+        double total = 0.0;
+        for (int64_t t = 0; t < ntri; ++t) {
+            const double *p = corners + 9 * t;
+            double a0 = p[0] - px, a1 = p[1] - py, a2 = p[2] - pz;
+            double b0 = p[3] - px, b1 = p[4] - py, b2 = p[5] - pz;
+            double c0 = p[6] - px, c1 = p[7] - py, c2 = p[8] - pz;
+            double la = sqrt(a0 * a0 + a1 * a1 + a2 * a2);
+            double lb = sqrt(b0 * b0 + b1 * b1 + b2 * b2);
+            double lc = sqrt(c0 * c0 + c1 * c1 + c2 * c2);
+            if (la == 0 || lb == 0 || lc == 0) bad = 1;
+            double x0 = b1 * c2 - b2 * c1, x1 = b2 * c0 - b0 * c2, x2 = b0 * c1 - b1 * c0;
+            double num = a0 * x0 + a1 * x1 + a2 * x2;
+            double den = la * lb * lc + (a0 * b0 + a1 * b1 + a2 * b2) * lc + (b0 * c0 + b1 * c1 + b2 * c2) * la +
+                         (c0 * a0 + c1 * a1 + c2 * a2) * lb;
+            total += atan2(num, den);
+        }
+        w[i] = total / (2.0 * 3.141592653589793);
+    }
+    return bad;
+}

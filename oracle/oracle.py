@@ -64,6 +64,9 @@ def lib():
             "or_octree_build": (ct.c_int64, [P, ct.c_int, ct.c_int, ct.c_int, ct.c_double, P, ct.c_int64, P,
                                              ct.c_int64, ct.POINTER(ct.c_int64), P]),
             "or_octree_sample": (ct.c_int, [P, P, P, P, P, ct.c_int64, ct.c_double, P, ct.c_int64, P]),
+            "or_columns_scan": (None, [P, ct.c_int64, P, ct.c_int64, ct.c_int, P, P, P]),
+            "or_column_mask": (ct.c_int, [P, P, P, ct.c_int, ct.c_int, ct.c_int, ct.c_int, P, P, P]),
+            "or_gwn": (ct.c_int, [P, ct.c_int64, P, ct.c_int64, P]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -282,3 +285,76 @@ def octree_sample(tree, indices):
     if status:
         raise RuntimeError("Invalid tree depth")
     return out
+
+
+# ---------------------------------------------------------------- mesh stage
+def columns_scan(corners, xy, cap=64):
+    """Per-column hits of FE/cuda_columns_v1/columns.cu:28-48 (face order):
+    (counts int32[q], depths f64[q, cap], signs int32[q, cap])."""
+    c = np.ascontiguousarray(corners, dtype=np.float64).reshape(-1, 9)
+    q = np.ascontiguousarray(xy, dtype=np.float64).reshape(-1, 2)
+    counts = np.empty(len(q), np.int32)
+    depths = np.zeros((len(q), cap), np.float64)
+    signs = np.zeros((len(q), cap), np.int32)
+    lib().or_columns_scan(_p(c), len(c), _p(q), len(q), cap, _p(counts), _p(depths), _p(signs))
+    return counts, depths, signs
+
+
+def sorted_hits(counts, depths, signs):
+    """Flatten per-column hits and order them as the reference's handles do
+    (FE/rt_columns_handle_v1.py:47-49): lexsort by column, depth, sign."""
+    cap = depths.shape[1]
+    valid = np.arange(cap)[None, :] < np.minimum(counts, cap)[:, None]
+    c = np.broadcast_to(np.arange(len(counts))[:, None], valid.shape)[valid]
+    z = depths[valid]
+    s = signs[valid]
+    order = np.lexsort((s, z, c))
+    return c[order], z[order], s[order]
+
+
+def column_mask(counts, depths, signs, shape, zs):
+    """(solid bool, surface bool, flags) of the fused mask chain."""
+    nx, ny, nz = shape
+    cap = depths.shape[1]
+    cn = np.ascontiguousarray(counts, dtype=np.int32)
+    d = np.ascontiguousarray(depths, dtype=np.float64)
+    s = np.ascontiguousarray(signs, dtype=np.int32)
+    z = np.ascontiguousarray(zs, dtype=np.float64)
+    mask = np.empty(shape, np.uint8)
+    surface = np.empty(shape, np.uint8)
+    flags = lib().or_column_mask(_p(cn), _p(d), _p(s), cap, nx, ny, nz, _p(z), _p(mask), _p(surface))
+    return mask.view(np.bool_), surface.view(np.bool_), flags
+
+
+def hits_to_columns(c, z, s, n_columns, cap=64):
+    """Inverse of sorted_hits: per-column arrays from flat (column, depth, sign)."""
+    counts = np.bincount(np.asarray(c, np.int64), minlength=n_columns).astype(np.int32)
+    depths = np.zeros((n_columns, cap), np.float64)
+    signs = np.zeros((n_columns, cap), np.int32)
+    order = np.argsort(c, kind="stable")
+    c, z, s = np.asarray(c)[order], np.asarray(z)[order], np.asarray(s)[order]
+    start = np.r_[0, np.cumsum(counts)]
+    k = np.arange(len(c)) - start[c]
+    depths[c, k] = z
+    signs[c, k] = s
+    return counts, depths, signs
+
+
+def gwn(corners, points):
+    """Ordered generalized winding numbers (FE/generalized_winding_ordered_v1.py:11-32)."""
+    c = np.ascontiguousarray(corners, dtype=np.float64).reshape(-1, 9)
+    q = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+    w = np.empty(len(q), np.float64)
+    if lib().or_gwn(_p(c), len(c), _p(q), len(q), _p(w)):
+        raise ValueError("query coincides with a vertex")
+    return w
+
+
+def surface_of(solid):
+    """Solid voxels on the grid border or with a 6-neighbour outside
+    (FE/mesh_fused_native_v1/field.cu:31-36)."""
+    m = np.asarray(solid, bool)
+    inner = np.zeros_like(m)
+    inner[1:-1, 1:-1, 1:-1] = (m[1:-1, 1:-1, 1:-1] & m[:-2, 1:-1, 1:-1] & m[2:, 1:-1, 1:-1] & m[1:-1, :-2, 1:-1] &
+                               m[1:-1, 2:, 1:-1] & m[1:-1, 1:-1, :-2] & m[1:-1, 1:-1, 2:])
+    return m & ~inner
