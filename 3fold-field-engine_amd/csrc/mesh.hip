@@ -313,27 +313,44 @@ __global__ void k_keys(const int32_t* __restrict__ counts, double* __restrict__ 
     }
 }
 
-__device__ __forceinline__ void note_classes(bool valid, bool inside, int* flags)
+// ORs each thread's flag bits into *flags with at most one global atomic per
+// block, and none once the bits are already there: one atomic per wave on a
+// single word serialises at L2 (millions of waves on a large grid).  Every
+// thread of the block must call it.
+__device__ __forceinline__ void block_flags(int local, int* flags)
 {
-    const uint64_t yes = __ballot(valid && inside), no = __ballot(valid && !inside);
-    if ((threadIdx.x & (FE_WAVE - 1)) == 0) {
-        const int f = (yes ? MF_IN : 0) | (no ? MF_OUT : 0);
-        if (f) atomicOr(flags, f);
+    __shared__ int acc;
+    if (threadIdx.x == 0) acc = 0;
+    __syncthreads();
+    int w = 0;
+    for (int bit = 1; bit <= MF_VERTEX; bit <<= 1)
+        if (__ballot(local & bit)) w |= bit;
+    if ((threadIdx.x & (FE_WAVE - 1)) == 0 && w) atomicOr(&acc, w);
+    __syncthreads();
+    if (threadIdx.x == 0 && acc) {
+        const int seen = __hip_atomic_load(flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if ((seen & acc) != acc) atomicOr(flags, acc);
     }
 }
+
+constexpr int VPT = 8;  // voxels per thread of the classification kernels
 
 // One thread per voxel (column-major, z fastest): solid iff the signs of the
 // column's hits whose key exceeds column + fraction(z) sum to nonzero
 // (FE/mesh_fused_native_v1/field.cu:24-30).
+// VPT voxels per thread, 256 apart, so each store instruction stays coalesced.
 __global__ __launch_bounds__(MT) void k_classify(const int32_t* __restrict__ counts, const int32_t* __restrict__ signs,
                                                   const double* __restrict__ keys,
                                                   const double* __restrict__ fractions, int cap, int nz,
                                                   uint32_t size, uint8_t* __restrict__ mask, int* flags)
 {
-    const uint32_t i = blockIdx.x * MT + threadIdx.x;
-    const bool valid = i < size;
-    bool inside = false;
-    if (valid) {
+    int local = 0;
+    const uint64_t base = (uint64_t)blockIdx.x * MT * VPT + threadIdx.x;
+#pragma unroll 1
+    for (int v = 0; v < VPT; ++v) {
+        const uint64_t iv = base + (uint64_t)v * MT;
+        if (iv >= size) break;
+        const uint32_t i = (uint32_t)iv;
         const uint32_t col = i / (uint32_t)nz, z = i - col * (uint32_t)nz;
         const int n = min(max(counts[col], 0), cap);
         const double q = (double)col + fractions[z];
@@ -342,10 +359,10 @@ __global__ __launch_bounds__(MT) void k_classify(const int32_t* __restrict__ cou
         int sum = 0;
         for (int j = 0; j < n; ++j)
             if (k[j] > q) sum += s[j];
-        inside = sum != 0;
-        mask[i] = inside;
+        mask[i] = sum != 0;
+        local |= sum != 0 ? MF_IN : MF_OUT;
     }
-    note_classes(valid, inside, flags);
+    block_flags(local, flags);
 }
 
 // The same classification from sorted keys in CSR form
@@ -425,15 +442,15 @@ __global__ __launch_bounds__(MT) void k_gwn(const double* __restrict__ tris, int
             total += atan2(num, den);
         }
     }
-    bool inside = false;
+    int local = bad ? MF_VERTEX : 0;
     if (valid) {
         const double w = total / (2.0 * 3.141592653589793);
         if (w_out) w_out[i] = w;
-        inside = fabs(w) > .5;
+        const bool inside = fabs(w) > .5;
         if (mask) mask[i] = inside;
+        local |= inside ? MF_IN : MF_OUT;
     }
-    if (bad) atomicOr(flags, MF_VERTEX);
-    note_classes(valid, inside, flags);
+    block_flags(local, flags);
 }
 
 // ------------------------------------------------------------------ host side
@@ -684,8 +701,8 @@ int mesh_enqueue(MeshField& f, hipStream_t st)
         hipLaunchKernelGGL(k_keys, dim3((unsigned)((n + MT - 1) / MT)), dim3(MT), 0, st, f.counts, f.depths, f.columns,
                            f.cap, f.cells, f.zs, f.nz, f.fractions);
         FE_TRY(hipGetLastError());
-        hipLaunchKernelGGL(k_classify, dim3(vox_blocks), dim3(MT), 0, st, f.counts, f.signs, f.depths, f.fractions,
-                           f.cap, f.nz, (uint32_t)f.size, f.mask, flags);
+        hipLaunchKernelGGL(k_classify, dim3((unsigned)((f.size + MT * VPT - 1) / (MT * VPT))), dim3(MT), 0, st,
+                           f.counts, f.signs, f.depths, f.fractions, f.cap, f.nz, (uint32_t)f.size, f.mask, flags);
         FE_TRY(hipGetLastError());
     } else {
         hipLaunchKernelGGL(k_gwn, dim3(vox_blocks), dim3(MT), 0, st, f.geo.tris, f.geo.ntri, f.points,
@@ -1068,7 +1085,7 @@ extern "C" int fe_mesh_query(void* handle, uint8_t* solid, uint8_t* surface, flo
 
 extern "C" int fe_mesh_destroy(void** handle) noexcept { return fused_destroy(handle); }
 
-// Replays the captured stage on `stream` (the handle's own when null) without
+// Replays the captured stage on `stream` (null: the default stream) without
 // copying anything out; read the results through fe_mesh_buffers and the
 // status through fe_mesh_status after the stream has finished.
 extern "C" int fe_mesh_run(void* handle, void* stream) noexcept
@@ -1077,7 +1094,7 @@ extern "C" int fe_mesh_run(void* handle, void* stream) noexcept
     MeshField& f = *static_cast<MeshField*>(handle);
     if (f.owner != std::this_thread::get_id()) return 3;
     if (f.failed) return FE_RUNTIME;
-    FE_TRY(hipGraphLaunch(f.exec, stream ? static_cast<hipStream_t>(stream) : f.stream));
+    FE_TRY(hipGraphLaunch(f.exec, static_cast<hipStream_t>(stream)));
     return FE_OK;
 }
 

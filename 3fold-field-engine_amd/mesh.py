@@ -184,11 +184,11 @@ class PreparedMeshField:
         return self._gmin.copy(), self.shape, surface.view(np.bool_), solid.view(np.bool_), distance
 
     def run_device(self, stream=None):
-        """Replay the stage on a torch stream without copying anything out
-        (results stay in the handle's device buffers, `device_buffers()`)."""
+        """Replay the stage on a torch stream (default: the current one)
+        without copying anything out (results stay in the handle's device
+        buffers, `device_buffers()`)."""
         self._check()
-        st = _lib.stream_handle(stream) if stream is not None else None
-        status = self._lib.fe_mesh_run(self._handle, st)
+        status = self._lib.fe_mesh_run(self._handle, _lib.stream_handle(stream))
         if status:
             self._failed = True
             raise RuntimeError(f'mesh field status {status}')

@@ -200,3 +200,41 @@ def thin_feature_batch(seed=4, count=64, lo=96, hi=256):
     for i, shape in enumerate(thin_feature_shapes(seed, count, lo, hi)):
         out.append(obstacle_volume(shape, np.random.default_rng([seed, 2, i]), obstacle_count(shape)))
     return out
+
+
+# ------------------------------------------------------------------ meshes
+def torus_mesh(R, r, nu, nv, center, axis_angles=(0.0, 0.0)):
+    """Closed, outward-wound torus of nu x nv quads (two triangles each),
+    tilted by axis_angles (about x, then z) and moved to `center`."""
+    u = 2 * np.pi * np.arange(nu) / nu
+    w = 2 * np.pi * np.arange(nv) / nv
+    U, W = np.meshgrid(u, w, indexing="ij")
+    v = np.stack([(R + r * np.cos(W)) * np.cos(U), (R + r * np.cos(W)) * np.sin(U), r * np.sin(W)], -1).reshape(-1, 3)
+    a, b = axis_angles
+    rx = np.array([[1, 0, 0], [0, math.cos(a), -math.sin(a)], [0, math.sin(a), math.cos(a)]])
+    rz = np.array([[math.cos(b), -math.sin(b), 0], [math.sin(b), math.cos(b), 0], [0, 0, 1]])
+    v = v @ (rz @ rx).T + np.asarray(center, np.float64)
+    i, j = np.meshgrid(np.arange(nu), np.arange(nv), indexing="ij")
+    p00 = (i * nv + j).ravel()
+    p10 = (((i + 1) % nu) * nv + j).ravel()
+    p01 = (i * nv + (j + 1) % nv).ravel()
+    p11 = (((i + 1) % nu) * nv + (j + 1) % nv).ravel()
+    f = np.concatenate([np.stack([p00, p10, p11], 1), np.stack([p00, p11, p01], 1)])
+    return v, f.astype(np.int64)
+
+
+def mesh_scene(seed=7, count=64, extent=500.0, nu=128, nv=32):
+    """Closed triangle soup of `count` seeded tori (R ~ U[15, 45] mm, r/R ~
+    U[0.15, 0.35], random tilt) inside [0, extent)^3: count*nu*nv*2
+    triangles (524,288 by default)."""
+    rng = np.random.default_rng(seed)
+    vs, fs, off = [], [], 0
+    for _ in range(count):
+        R = rng.uniform(15, 45)
+        r = R * rng.uniform(0.15, 0.35)
+        c = rng.uniform(R + r, extent - R - r, 3)
+        v, f = torus_mesh(R, r, nu, nv, c, tuple(rng.uniform(0, math.pi, 2)))
+        vs.append(v)
+        fs.append(f + off)
+        off += len(v)
+    return np.concatenate(vs), np.concatenate(fs)

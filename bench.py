@@ -231,6 +231,46 @@ def bench_octree(fe, torch, dense, reps=3, queries=1 << 20):
             "lookups": queries, "lookup_ms": ms, "lookups_per_s": queries / (ms * 1e-3), "leaf_histogram": hist}
 
 
+def bench_mesh(fe, torch, reps=5):
+    """Mesh stage (column hits -> mask -> surface -> paired EDT -> signed
+    distance, or winding numbers for open meshes), one captured graph per
+    prepared mesh.  query_ms: one evaluate() including the copies of solid,
+    surface and distance to the host (what the reference's fused query times,
+    reports/field_mesh_fused.json: 0.253 ms rotated box, 9.390 ms open box on
+    an RTX 5070, leg 0); replay_ms: the device work alone (HIP events)."""
+    out = {}
+    with np.load(ROOT / "tests" / "golden" / "mesh_stage.npz", allow_pickle=False) as z:
+        rv = z["rotated_0.0_vertices"]
+        cases = {"rotated_0.0": (rv, z["rotated_0.0_faces"], 0.25, rv.min(0) - 1, 0.253),
+                 "open": (z["open_vertices"], z["open_faces"], 0.25, np.array([-4.031, -4.043, -4.057]), 9.390)}
+    for name, (v, f, pitch, lo, ref) in cases.items():
+        field = fe.PreparedMeshField(v, f, pitch, lo)
+        field.evaluate()
+        t0 = time.perf_counter()
+        for _ in range(20):
+            field.evaluate()
+        q_ms = (time.perf_counter() - t0) * 1e3 / 20
+        r_ms = event_ms(torch, lambda: field.run_device(torch.cuda.current_stream()), 20)
+        out[name] = {"method": field.method, "shape": list(field.shape), "query_ms": q_ms, "replay_ms": r_ms,
+                     "reference_query_ms": ref}
+        field.close()
+    v, f = fe.recipes.mesh_scene()
+    t0 = time.perf_counter()
+    field = fe.PreparedMeshField(v, f, 1.0, np.zeros(3))
+    prep_ms = (time.perf_counter() - t0) * 1e3
+    field.run_device(torch.cuda.current_stream())
+    torch.cuda.synchronize()
+    status = field.device_status()
+    r_ms = event_ms(torch, lambda: field.run_device(torch.cuda.current_stream()), reps)
+    nvox = int(np.prod(field.shape))
+    out["scene"] = {"workload": "64 seeded tori, 524,288 triangles, 1 mm grid", "shape": list(field.shape),
+                    "voxels": nvox, "triangles": int(len(f)), "status": status, "prepare_ms": prep_ms,
+                    "replay_ms": r_ms, "mvox_s": nvox / (r_ms * 1e-3) / 1e6}
+    field.close()
+    torch.cuda.empty_cache()
+    return out
+
+
 def bench_serve(fe, torch, rates, count):
     """Poisson arrivals of 512^3 / 1024-primitive recipes into the field server."""
     rng = np.random.default_rng(5)
@@ -265,9 +305,10 @@ def main():
     ap.add_argument("--quick", action="store_true", help="single-field step only")
     ap.add_argument("--batch-count", type=int, default=256)
     ap.add_argument("--serve-count", type=int, default=48)
-    ap.add_argument("--only", choices=["edt-heavy", "kernels"], default=None,
+    ap.add_argument("--only", choices=["edt-heavy", "kernels", "mesh"], default=None,
                     help="edt-heavy: the thin-feature EDT section only; kernels: each hot kernel of the "
-                         "512^3 step three times plus a 1 GiB copy (for rocprofv3 counter runs)")
+                         "512^3 step three times plus a 1 GiB copy (for rocprofv3 counter runs); "
+                         "mesh: the mesh-stage section only")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU: the block-sparse build and the EDT run on one "
@@ -281,6 +322,9 @@ def main():
         return 0
     if args.only == "kernels":
         print(json.dumps(kernel_launches(fe, torch, args.n)))
+        return 0
+    if args.only == "mesh":
+        print(json.dumps({"mesh": bench_mesh(fe, torch)}))
         return 0
     n = args.n
     rec = fe.recipes.large_field_512(seed=2) if n == 512 else fe.recipes.mixed_recipe(np.random.default_rng(2), 1024, n)
@@ -366,6 +410,7 @@ def main():
         out["batched"] = bench_batch(fe, torch, args.batch_count, plain_sample=32)
         out["served"] = bench_serve(fe, torch, (200.0, 2000.0), args.serve_count)
         out["edt_heavy"] = bench_edt_heavy(fe, torch)
+        out["mesh"] = bench_mesh(fe, torch)
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", "16"))
         out["cpu_baseline"] = cpu_baseline(fe, rec, occ_host, threads, n)
