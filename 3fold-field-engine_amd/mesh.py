@@ -183,6 +183,44 @@ class PreparedMeshField:
             raise RuntimeError(_STATUS_TEXT.get(status, f'mesh field status {status}'))
         return self._gmin.copy(), self.shape, surface.view(np.bool_), solid.view(np.bool_), distance
 
+    def evaluate_into(self, solid, surface, distance):
+        """evaluate() into caller arrays (bool, bool, float32 of `shape`, C-contiguous,
+        writable, not overlapping), as FE/mesh_fused_into_v1.py:43-62; the
+        device copies land in them directly."""
+        self._check()
+        arrays = (solid, surface, distance)
+        for array, dtype in zip(arrays, (np.bool_, np.bool_, np.float32)):
+            if not isinstance(array, np.ndarray) or array.dtype != dtype or array.shape != tuple(self.shape):
+                raise ValueError('Matching output shape and dtype required')
+            if not array.flags.c_contiguous or not array.flags.writeable:
+                raise ValueError('Contiguous writable outputs required')
+        spans = sorted((a.ctypes.data, a.ctypes.data + a.nbytes) for a in arrays)
+        if any(left[1] > right[0] for left, right in zip(spans, spans[1:])):
+            raise ValueError('Output buffers must not overlap')
+        status = self._lib.fe_mesh_query(self._handle, *(_arr(a) for a in arrays))
+        if status:
+            self._failed = True
+            if status == 1:
+                raise ValueError('Both classes must be present')
+            if status == 5:
+                raise ValueError('query coincides with a vertex')
+            raise RuntimeError(_STATUS_TEXT.get(status, f'mesh field status {status}'))
+
+    def pin_outputs(self, region):
+        """Checks an output region of the service layout (FE/mesh_fused_into_v1.py:8-30)
+        and returns False: memory that already exists is not page-locked here,
+        so device copies into it are staged by the runtime."""
+        self._check()
+        size = int(np.prod(self.shape))
+        expected = (2 * size + 63) // 64 * 64 + 4 * size
+        if len(region) != expected or region.readonly:
+            raise ValueError('Writable exact-size region required')
+        return False
+
+    def unpin_outputs(self):
+        if threading.get_ident() != self._owner:
+            raise RuntimeError('Field belongs to another thread')
+
     def run_device(self, stream=None):
         """Replay the stage on a torch stream (default: the current one)
         without copying anything out (results stay in the handle's device
