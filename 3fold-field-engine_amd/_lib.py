@@ -80,6 +80,10 @@ SIGNATURES = {
     "fe_mesh_status": (ct.c_int, [P]),
     "fe_mesh_buffers": (ct.c_int, [P] + [ct.POINTER(P)] * 7 + [ct.POINTER(ct.c_int)]),
     "fe_gwn_device": (ct.c_int, [P, I64, P, I64, P, P, P, P]),
+    # IKARUS trees
+    "fe_ikarus_eval": (ct.c_int, [P, ct.c_int, P, P, P, P, I64, P, P]),
+    "fe_ikarus_classify": (ct.c_int, [P, ct.c_int, P, P, P, P, P, ct.c_int, P, P, P]),
+    "fe_ikarus_tiles": (ct.c_int, [P, ct.c_int, P, P, P, P, P, ct.c_int, P, I64, P, P]),
 }
 
 _lib = None

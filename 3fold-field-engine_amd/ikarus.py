@@ -1,0 +1,451 @@
+"""IKARUS expression trees on the device.
+
+A part is an expression tree of plain dict nodes (the format of
+FE/ikarus_v1/expr.py: JSON round-trips it, `save`/`load` wrap the versioned
+envelope).  Leaves: halfspace, sphere, box, cylinder, cone, rect_frustum,
+gridfield (trilinear samples); interior nodes: transform (affine 4x4),
+axis_project, offset, scale, and union / intersect / subtract with a smooth
+blend radius k (k = 0: hard).  Sign convention: < 0 inside.
+
+`compile_tree` turns a tree into a program for the device interpreter in
+csrc/ikarus.hip: 16-byte instructions (opcode, argument, parameter offset) in
+post order and an fp32 parameter vector.  Parameters are rounded to fp32
+from the double values the reference bakes into its generated kernels
+(FE/ikarus_v1/eval_warp.py:160-320; rect_frustum as
+FE/ikarus_v1/eval_warp_uniform_v2.py:180-214).  Each leaf is preceded by its
+ancestors' point maps, applied from the world point outward-in; the two
+operands of a boolean are evaluated larger-stack-need first (the result does
+not depend on the order), so the LDS value stack stays shallow.
+
+Entry points: `eval_batch(node, points)` and `gradient_batch(node, points, h)`
+(FE/ikarus_v1/eval_warp.py:471-485, 718-735), and
+`klassificera_och_evaluera_ikarus(meta, node)`: the block-sparse build of a
+field from a tree (eight-corner classification, host compaction, tiles), in
+the structure of the plain per-call build.
+"""
+import json
+import math
+
+import numpy as np
+
+from . import _lib
+
+IKARUS_VERSION = "1.0"
+
+# opcodes of csrc/ikarus.hip
+WORLD, XFORM, PROJ, SCALE_IN = 0, 1, 2, 3
+HALFSPACE, SPHERE, BOX, CYL, CONE, FRUSTUM, GRID = 10, 11, 12, 13, 14, 15, 16
+OFFSET, SCALE_OUT, UNION, INTERSECT, SUBTRACT = 20, 21, 22, 23, 24
+STACK_SLOTS = 16
+
+GRID_REC = np.dtype([("off", "<i8"), ("nx", "<i4"), ("ny", "<i4"), ("nz", "<i4"), ("is_sdf", "<i4"),
+                     ("ox", "<f4"), ("oy", "<f4"), ("oz", "<f4"), ("pitch", "<f4"), ("iso", "<f4"), ("pad", "<f4")])
+assert GRID_REC.itemsize == 48
+AXES = {"x": 0, "y": 1, "z": 2}
+
+
+# ------------------------------------------------------------------ 4x4 affine helpers (FE/ikarus_v1/expr.py:24-83)
+def mat4_identity():
+    return [[1.0, 0.0, 0.0, 0.0], [0.0, 1.0, 0.0, 0.0], [0.0, 0.0, 1.0, 0.0], [0.0, 0.0, 0.0, 1.0]]
+
+
+def mat4_translate(tx, ty, tz):
+    m = mat4_identity()
+    m[0][3], m[1][3], m[2][3] = tx, ty, tz
+    return m
+
+
+def mat4_rot_axis(axis, deg):
+    a = math.radians(deg)
+    c, s = math.cos(a), math.sin(a)
+    m = mat4_identity()
+    if axis == "x":
+        m[1][1], m[1][2], m[2][1], m[2][2] = c, -s, s, c
+    elif axis == "y":
+        m[0][0], m[0][2], m[2][0], m[2][2] = c, s, -s, c
+    elif axis == "z":
+        m[0][0], m[0][1], m[1][0], m[1][1] = c, -s, s, c
+    else:
+        raise ValueError(f"axis must be x/y/z, got {axis!r}")
+    return m
+
+
+def mat4_mul(a, b):
+    return [[sum(a[i][k] * b[k][j] for k in range(4)) for j in range(4)] for i in range(4)]
+
+
+def mat4_inverse_affine(m):
+    """Inverse of an affine 4x4 by cofactors of its 3x3 block (the world ->
+    local map a transform node applies; the same double operations as
+    FE/ikarus_v1/expr.py:57-83, so the fp32 parameters round alike)."""
+    R = [[m[i][j] for j in range(3)] for i in range(3)]
+    t = [m[i][3] for i in range(3)]
+    det = (R[0][0] * (R[1][1] * R[2][2] - R[1][2] * R[2][1])
+           - R[0][1] * (R[1][0] * R[2][2] - R[1][2] * R[2][0])
+           + R[0][2] * (R[1][0] * R[2][1] - R[1][1] * R[2][0]))
+    if abs(det) < 1e-15:
+        raise ValueError("singular transform, cannot invert")
+
+    def cof(r, c):
+        return (R[(r + 1) % 3][(c + 1) % 3] * R[(r + 2) % 3][(c + 2) % 3]
+                - R[(r + 1) % 3][(c + 2) % 3] * R[(r + 2) % 3][(c + 1) % 3])
+    inv = [[0.0] * 3 for _ in range(3)]
+    for i in range(3):
+        for j in range(3):
+            inv[j][i] = cof(i, j) / det
+    out = mat4_identity()
+    for i in range(3):
+        for j in range(3):
+            out[i][j] = inv[i][j]
+    for i in range(3):
+        out[i][3] = -(inv[i][0] * t[0] + inv[i][1] * t[1] + inv[i][2] * t[2])
+    return out
+
+
+# ------------------------------------------------------------------ nodes (FE/ikarus_v1/expr.py:90-211)
+def halfspace(normal, offset):
+    nx, ny, nz = normal
+    nrm = math.sqrt(nx * nx + ny * ny + nz * nz)
+    if not nrm > 1e-12:
+        raise ValueError("halfspace normal must be nonzero")
+    return {"op": "halfspace", "normal": [nx / nrm, ny / nrm, nz / nrm], "offset": float(offset)}
+
+
+def sphere(radius, center=(0.0, 0.0, 0.0)):
+    return {"op": "sphere", "radius": float(radius), "center": list(map(float, center))}
+
+
+def box(half_extents, center=(0.0, 0.0, 0.0)):
+    return {"op": "box", "half_extents": list(map(float, half_extents)), "center": list(map(float, center))}
+
+
+def cylinder(radius, height, axis="z", center=(0.0, 0.0, 0.0)):
+    return {"op": "cylinder", "radius": float(radius), "height": float(height), "axis": axis,
+            "center": list(map(float, center))}
+
+
+def cone(radius1, radius2, height, axis="z", center=(0.0, 0.0, 0.0)):
+    return {"op": "cone", "radius1": float(radius1), "radius2": float(radius2), "height": float(height),
+            "axis": axis, "center": list(map(float, center))}
+
+
+def rect_frustum(z0, z1, hx0, hy0, hx1, hy1, cx0=0.0, cy0=0.0, cx1=0.0, cy1=0.0):
+    if not z1 > z0:
+        raise ValueError("rect_frustum requires z1 > z0")
+    return {"op": "rect_frustum", "z0": float(z0), "z1": float(z1), "hx0": float(hx0), "hy0": float(hy0),
+            "hx1": float(hx1), "hy1": float(hy1), "cx0": float(cx0), "cy0": float(cy0), "cx1": float(cx1),
+            "cy1": float(cy1)}
+
+
+def gridfield(npz_path, key="occupancy", mode="occupancy", iso=0.5, pitch_override=None):
+    """A trilinear grid leaf read from an .npz (keys `key`, optional
+    `global_origin_mm` (3,) and `pitch_mm`); mode "sdf" uses the values as
+    distances, "occupancy" reads (iso - value) * pitch."""
+    return {"op": "gridfield", "npz_path": npz_path, "key": key, "mode": mode, "iso": float(iso),
+            "pitch_override": pitch_override}
+
+
+def grid_leaf(values, origin=(0.0, 0.0, 0.0), pitch=1.0, mode="sdf", iso=0.5):
+    """A grid leaf holding its samples in memory (not JSON-serialisable)."""
+    return {"op": "gridfield", "values": np.asarray(values, np.float32), "origin": list(map(float, origin)),
+            "pitch": float(pitch), "mode": mode, "iso": float(iso)}
+
+
+def transform(child, mat4):
+    return {"op": "transform", "child": child, "mat4": mat4}
+
+
+def translate(child, tx, ty, tz):
+    return transform(child, mat4_translate(tx, ty, tz))
+
+
+def rotate(child, axis, deg, about=(0.0, 0.0, 0.0)):
+    ax, ay, az = about
+    m = mat4_mul(mat4_translate(ax, ay, az), mat4_mul(mat4_rot_axis(axis, deg), mat4_translate(-ax, -ay, -az)))
+    return transform(child, m)
+
+
+def axis_project(child, axis, ref):
+    if axis not in (0, 1, 2):
+        raise ValueError(f"axis_project: axis must be 0/1/2 (x/y/z), got {axis!r}")
+    return {"op": "axis_project", "child": child, "axis": int(axis), "ref": float(ref)}
+
+
+def offset(child, delta):
+    return {"op": "offset", "child": child, "delta": float(delta)}
+
+
+def scale(child, s):
+    return {"op": "scale", "child": child, "s": float(s)}
+
+
+def union(a, b, k=0.0):
+    return {"op": "union", "a": a, "b": b, "k": float(k)}
+
+
+def intersect(a, b, k=0.0):
+    return {"op": "intersect", "a": a, "b": b, "k": float(k)}
+
+
+def subtract(a, b, k=0.0):
+    return {"op": "subtract", "a": a, "b": b, "k": float(k)}
+
+
+def save(node, path):
+    with open(path, "w") as f:
+        json.dump({"ikarus_version": IKARUS_VERSION, "tree": node}, f, indent=1)
+    return path
+
+
+def load(path):
+    with open(path) as f:
+        data = json.load(f)
+    if isinstance(data, dict) and "tree" in data and "ikarus_version" in data:
+        return data["tree"]
+    return data
+
+
+def node_count(node):
+    if not isinstance(node, dict):
+        return 0
+    return 1 + sum(node_count(node[k]) for k in ("child", "a", "b") if k in node)
+
+
+def leaf_ops(node):
+    out = set()
+
+    def walk(n):
+        if n["op"] in ("halfspace", "sphere", "box", "cylinder", "cone", "rect_frustum", "gridfield"):
+            out.add(n["op"])
+        for k in ("child", "a", "b"):
+            if k in n:
+                walk(n[k])
+    walk(node)
+    return out
+
+
+# ------------------------------------------------------------------ compiler
+def _grid_samples(node):
+    """(fp32 samples, origin, pitch, iso, is_sdf) of a grid leaf
+    (FE/ikarus_v1/eval_warp.py:433-450 for the .npz form)."""
+    if "values" in node:
+        arr = np.ascontiguousarray(node["values"], np.float32)
+        return arr, node["origin"], float(node["pitch"]), float(node["iso"]), node["mode"] == "sdf"
+    with np.load(node["npz_path"], allow_pickle=False) as d:
+        arr = np.ascontiguousarray(d[node["key"]], np.float32)
+        origin = d["global_origin_mm"] if "global_origin_mm" in d else np.zeros(3)
+        pitch = node["pitch_override"] or float(d["pitch_mm"]) if "pitch_mm" in d else 1.0
+    return arr, [float(x) for x in origin], float(pitch), float(node["iso"]), node["mode"] == "sdf"
+
+
+def _need(node, memo):
+    """Value-stack slots a subtree needs (Sethi-Ullman, the register top counted)."""
+    key = id(node)
+    if key in memo:
+        return memo[key]
+    op = node["op"]
+    if op in ("union", "intersect", "subtract"):
+        na, nb = _need(node["a"], memo), _need(node["b"], memo)
+        r = max(na, nb) if na != nb else na + 1
+    elif "child" in node:
+        r = _need(node["child"], memo)
+    else:
+        r = 1
+    memo[key] = r
+    return r
+
+
+class Program:
+    """A compiled tree: instructions (int32 [n, 4]), fp32 parameters, grid
+    records and samples, and the value-stack depth it needs."""
+
+    def __init__(self, ins, params, grids, gdata, depth):
+        self.ins = ins
+        self.params = params
+        self.grids = grids
+        self.gdata = gdata
+        self.depth = depth
+        self._dev = {}
+
+    def device(self, device):
+        """Device copies (torch tensors), made once per device."""
+        import torch
+        dev = torch.device(device)
+        if dev not in self._dev:
+            def up(a, dtype):
+                return torch.from_numpy(np.ascontiguousarray(a)).to(dev) if len(a) else torch.zeros(1, dtype=dtype,
+                                                                                                  device=dev)
+            self._dev[dev] = (up(self.ins.reshape(-1), torch.int32), up(self.params, torch.float32),
+                              up(self.grids.view(np.uint8), torch.uint8), up(self.gdata, torch.float32))
+        return self._dev[dev]
+
+
+def compile_tree(node):
+    ins, params, grids, chunks = [], [], [], []
+    goff = [0]
+    state = {"chain": None}
+
+    def slot(values):
+        off = len(params)
+        params.extend(np.float32(v) for v in values)
+        return off
+
+    def emit(op, a=0, values=()):
+        ins.append((op, a, slot(values), 0))
+
+    def leaf(chain, op, a, values):
+        if state["chain"] != [id(c[0]) for c in chain]:
+            emit(WORLD)
+            for c in chain:
+                emit(*c[1])
+            state["chain"] = [id(c[0]) for c in chain]
+        emit(op, a, values)
+
+    memo = {}
+
+    def walk(n, chain):
+        op = n["op"]
+        if op == "halfspace":
+            leaf(chain, HALFSPACE, 0, list(n["normal"]) + [n["offset"]])
+        elif op == "sphere":
+            leaf(chain, SPHERE, 0, list(n["center"]) + [n["radius"]])
+        elif op == "box":
+            leaf(chain, BOX, 0, list(n["center"]) + list(n["half_extents"]))
+        elif op == "cylinder":
+            leaf(chain, CYL, AXES[n["axis"]], list(n["center"]) + [n["radius"], n["height"] / 2.0])
+        elif op == "cone":
+            leaf(chain, CONE, AXES[n["axis"]], list(n["center"]) + [n["height"] / 2.0, n["height"], n["radius1"],
+                                                                     n["radius2"]])
+        elif op == "rect_frustum":
+            z0, z1 = n["z0"], n["z1"]
+            kx = (n["hx1"] - n["hx0"]) / (z1 - z0)
+            ky = (n["hy1"] - n["hy0"]) / (z1 - z0)
+            kcx = (n["cx1"] - n["cx0"]) / (z1 - z0)
+            kcy = (n["cy1"] - n["cy0"]) / (z1 - z0)
+            leaf(chain, FRUSTUM, 0, [z0, z1,
+                                     -(kx + kcx), n["hx0"] + n["cx0"] - (kx + kcx) * z0,
+                                     -(kx - kcx), n["hx0"] - n["cx0"] - (kx - kcx) * z0,
+                                     -(ky + kcy), n["hy0"] + n["cy0"] - (ky + kcy) * z0,
+                                     -(ky - kcy), n["hy0"] - n["cy0"] - (ky - kcy) * z0])
+        elif op == "gridfield":
+            arr, origin, pitch, iso, is_sdf = _grid_samples(n)
+            if arr.ndim != 3 or min(arr.shape) < 1:
+                raise ValueError("grid leaf needs a 3-D array")
+            grids.append((goff[0], *arr.shape, 1 if is_sdf else 0, *(np.float32(x) for x in origin),
+                          np.float32(pitch), np.float32(iso), 0.0))
+            chunks.append(arr.reshape(-1))
+            goff[0] += arr.size
+            leaf(chain, GRID, len(grids) - 1, ())
+        elif op == "transform":
+            inv = mat4_inverse_affine(n["mat4"])
+            walk(n["child"], chain + [(n, (XFORM, 0, [inv[r][c] for r in range(3) for c in range(4)]))])
+        elif op == "axis_project":
+            walk(n["child"], chain + [(n, (PROJ, int(n["axis"]), [n["ref"]]))])
+        elif op == "scale":
+            walk(n["child"], chain + [(n, (SCALE_IN, 0, [n["s"]]))])
+            emit(SCALE_OUT, 0, [n["s"]])
+        elif op == "offset":
+            walk(n["child"], chain)
+            emit(OFFSET, 0, [n["delta"]])
+        elif op in ("union", "intersect", "subtract"):
+            na, nb = _need(n["a"], memo), _need(n["b"], memo)
+            if nb > na:  # evaluate b first: the top then holds a
+                walk(n["b"], chain)
+                walk(n["a"], chain)
+                swapped = 1
+            else:
+                walk(n["a"], chain)
+                walk(n["b"], chain)
+                swapped = 0
+            emit({"union": UNION, "intersect": INTERSECT, "subtract": SUBTRACT}[op], swapped, [n["k"]])
+        else:
+            raise ValueError(f"unknown op {op!r}")
+
+    depth = _need(node, memo)
+    if depth > STACK_SLOTS + 1:
+        raise ValueError(f"tree needs {depth} value slots; the device stack holds {STACK_SLOTS + 1}")
+    walk(node, [])
+    ins_a = np.array(ins, np.int32).reshape(-1, 4)
+    par = np.array(params, np.float32) if params else np.zeros(1, np.float32)
+    grid_a = np.array(grids, GRID_REC) if grids else np.zeros(0, GRID_REC)
+    gdata = np.concatenate(chunks).astype(np.float32) if chunks else np.zeros(0, np.float32)
+    return Program(ins_a, par, grid_a, gdata, depth)
+
+
+# ------------------------------------------------------------------ device entry points
+def _program(node):
+    return node if isinstance(node, Program) else compile_tree(node)
+
+
+def _check(status, what):
+    if status:
+        raise RuntimeError(f"{what} failed with status {status}")
+
+
+def eval_device(prog, points, out, stream=None):
+    """Values of a compiled program at device points (n, 3) fp32 into out (n,) fp32."""
+    ins, par, grids, gdata = prog.device(points.device)
+    L = _lib.lib()
+    P = _lib.ptr
+    _check(L.fe_ikarus_eval(P(ins), len(prog.ins), P(par), P(grids), P(gdata), P(points), points.shape[0], P(out),
+                            _lib.stream_handle(stream)), "ikarus eval")
+
+
+def eval_batch(node, points_mm, device="cuda:0"):
+    """(N, 3) points (mm) -> (N,) fp32 values (FE/ikarus_v1/eval_warp.py:471-485)."""
+    import torch
+    prog = _program(node)
+    pts = torch.from_numpy(np.ascontiguousarray(np.asarray(points_mm).astype(np.float32)).reshape(-1, 3)).to(device)
+    out = torch.empty(pts.shape[0], dtype=torch.float32, device=pts.device)
+    eval_device(prog, pts, out)
+    return out.cpu().numpy()
+
+
+def gradient_batch(node, points_mm, h=0.05, device="cuda:0"):
+    """Central differences at +-h per axis in one evaluation of 6N points
+    (FE/ikarus_v1/eval_warp.py:718-735); (N, 3)."""
+    points_mm = np.asarray(points_mm)
+    n = points_mm.shape[0]
+    offsets = np.array([[h, 0, 0], [-h, 0, 0], [0, h, 0], [0, -h, 0], [0, 0, h], [0, 0, -h]])
+    stacked = (points_mm[None, :, :] + offsets[:, None, :]).reshape(-1, 3)
+    vals = eval_batch(node, stacked, device=device).reshape(6, n)
+    gx = (vals[0] - vals[1]) / (2 * h)
+    gy = (vals[2] - vals[3]) / (2 * h)
+    gz = (vals[4] - vals[5]) / (2 * h)
+    return np.stack([gx, gy, gz], axis=1)
+
+
+def klassificera_och_evaluera_ikarus(meta, node, device="cuda:0"):
+    """Block-sparse field of a tree on the grid `meta` (a field.GridMeta):
+    classification at the eight block corners (FE/faltkarna_v1.py:128-164)
+    on the device, ascending compaction of the surface blocks on the host,
+    tiles of the active blocks on the device (FE/faltkarna_v1.py:166-190).
+    Returns a field.SparseField."""
+    import torch
+    from .field import SparseField, _stats
+    prog = _program(node)
+    dev = torch.device(device)
+    ins, par, grids, gdata = prog.device(dev)
+    L = _lib.lib()
+    P = _lib.ptr
+    st = _lib.stream_handle(None)
+    opm = np.array([meta.x0, meta.y0, meta.z0, meta.pitch, meta.margin_mm], np.float32)
+    blocks = np.array([meta.n_bx, meta.n_by, meta.n_bz], np.int32)
+    kind_d = torch.empty(meta.n_blocks, dtype=torch.int32, device=dev)
+    cma_d = torch.empty(meta.n_blocks, dtype=torch.float32, device=dev)
+    _check(L.fe_ikarus_classify(P(ins), len(prog.ins), P(par), P(grids), P(gdata), opm.ctypes.data, blocks.ctypes.data,
+                                meta.block, P(kind_d), P(cma_d), st), "ikarus classify")
+    kind = kind_d.cpu().numpy()
+    cma = cma_d.cpu().numpy()
+    active_ids = np.where(kind == 0)[0].astype(np.int32)
+    B = meta.block
+    if len(active_ids):
+        ids_d = torch.from_numpy(active_ids).to(dev)
+        tiles_d = torch.empty(len(active_ids) * B ** 3, dtype=torch.float32, device=dev)
+        _check(L.fe_ikarus_tiles(P(ins), len(prog.ins), P(par), P(grids), P(gdata), opm.ctypes.data,
+                                 blocks.ctypes.data, B, P(ids_d), len(active_ids), P(tiles_d), st), "ikarus tiles")
+        tiles = tiles_d.cpu().numpy().reshape(-1, B, B, B)
+    else:
+        tiles = np.zeros((0, B, B, B), np.float32)
+    return SparseField(meta=meta, kind=kind, active_ids=active_ids, tiles=tiles, corner_min_abs=cma,
+                       stats=_stats(meta, kind, len(active_ids)))

@@ -253,6 +253,26 @@ int fe_mesh_buffers(const void *handle, void **solid, void **surface, void **dis
 int fe_gwn_device(const double *corners, int64_t triangles, const double *points, int64_t n, double *w, uint8_t *mask,
                   int *flags, void *stream);
 
+/* ---------------------------------------------------------------- IKARUS expression trees */
+/* Replaces the per-tree generated Warp kernels of ikarus_v1/eval_warp.py:160-485
+ * (and eval_warp_uniform_v2.py:100-270 for rect_frustum) with one interpreter:
+ * ins = n_ins 16-byte instructions {int32 op, arg, param offset, 0} compiled by
+ * the package's ikarus.compile_tree; params fp32; grids = 48-byte grid-leaf
+ * records {int64 offset; int32 nx, ny, nz, is_sdf; float ox, oy, oz, pitch,
+ * iso, pad} over grid_data (null without grid leaves).  All pointers are
+ * device pointers except origin_pitch_margin (5 floats) and blocks (3 int32),
+ * which are host arrays.  eval: values at points (n x 3 fp32).  classify:
+ * faltkarna_v1.py:128-164 rule per block (kind int32, corner_min_abs fp32);
+ * tiles: faltkarna_v1.py:166-190 for the ascending active ids. */
+int fe_ikarus_eval(const void *ins, int n_ins, const float *params, const void *grids, const float *grid_data,
+                   const float *points, int64_t n, float *out, void *stream);
+int fe_ikarus_classify(const void *ins, int n_ins, const float *params, const void *grids, const float *grid_data,
+                       const float *origin_pitch_margin, const int32_t *blocks, int B, int32_t *kind, float *cma,
+                       void *stream);
+int fe_ikarus_tiles(const void *ins, int n_ins, const float *params, const void *grids, const float *grid_data,
+                    const float *origin_pitch_margin, const int32_t *blocks, int B, const int32_t *ids,
+                    int64_t n_active, float *tiles, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

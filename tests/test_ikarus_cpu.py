@@ -1,0 +1,162 @@
+"""IKARUS trees on the CPU: the fp32 oracle and the float64 ground truth
+against closed forms and the reference's own gates, the compiler's program
+shape and stack bound, the JSON format.  No GPU.
+
+What pins the oracle (oracle/ikarus_ref.py):
+* eval_f64 (FE/ikarus_v1/expr.py:263-385): closed forms -- exact distances of a
+  sphere, an axis-aligned box (brute-force nearest point), a halfspace and a
+  capped cylinder on its axis; transforms and rotations equal the moved
+  primitive; scale(sphere r, s) = sphere(r s); offset; hard booleans = min/max;
+  a smooth union lies between min - k/4 and min and equals min where the
+  operands differ by >= k; rect_frustum with equal profiles is the box's
+  inside distance; a grid leaf of an affine field reproduces the field;
+* eval_f32 (the generated fp32 arithmetic): within fp32 rounding of eval_f64
+  on random trees of every node type, and inside the reference's own
+  acceptance gates on its self-test trees (max error < 1e-3 for
+  FE/ikarus_v1/eval_warp.py:737-770, < 1e-2 for eval_warp_uniform_v2.py:57-109).
+"""
+import json
+import math
+
+import numpy as np
+import pytest
+
+import ikarus_ref as IR
+from ikarus_trees import balanced, chain, linear_grid, random_tree, selftest_v1, selftest_v2
+
+
+@pytest.fixture(scope="module")
+def E(fe):
+    return fe.ikarus
+
+
+def test_f64_closed_forms(E):
+    rng = np.random.default_rng(0)
+    p = rng.uniform(-10, 10, (2000, 3))
+    c = np.array([1.0, -2.0, 0.5])
+    d = np.linalg.norm(p - c, axis=1)
+    assert np.allclose(IR.eval_f64(E.sphere(3.0, c), p), d - 3.0, atol=1e-12)
+    # box: distance to the nearest point of the box (outside), -distance to the faces (inside)
+    h = np.array([2.0, 3.0, 1.5])
+    q = np.abs(p - c) - h
+    nearest = np.clip(p, c - h, c + h)
+    outside = np.linalg.norm(p - nearest, axis=1)
+    inside = np.max(q, axis=1)
+    want = np.where(np.all(q <= 0, axis=1), inside, outside)
+    assert np.allclose(IR.eval_f64(E.box(h, c), p), want, atol=1e-12)
+    n = np.array([1.0, 2.0, -2.0]) / 3.0
+    assert np.allclose(IR.eval_f64(E.halfspace([1.0, 2.0, -2.0], 0.7), p), p @ n - 0.7, atol=1e-12)
+    # cylinder on its axis: |z| - h/2 beyond the caps, max(-r, |z| - h/2) inside
+    z = np.linspace(-12, 12, 101)
+    on_axis = np.stack([np.full_like(z, c[0]), np.full_like(z, c[1]), c[2] + z], 1)
+    v = IR.eval_f64(E.cylinder(2.0, 8.0, "z", c), on_axis)
+    assert np.allclose(v, np.maximum(-2.0, np.abs(z) - 4.0), atol=1e-12)
+    # moved / rotated / scaled primitives
+    s = E.sphere(2.5, (0.0, 0.0, 0.0))
+    assert np.allclose(IR.eval_f64(E.translate(s, *c), p), d - 2.5, atol=1e-12)
+    m = E.mat4_mul(E.mat4_translate(*c), E.mat4_rot_axis("y", 33.0))
+    assert np.allclose(IR.eval_f64(E.transform(s, m), p), d - 2.5, atol=1e-12)
+    rb = E.rotate(E.box(h, (0.0, 0.0, 0.0)), "z", 90.0)
+    assert np.allclose(IR.eval_f64(rb, p), IR.eval_f64(E.box([h[1], h[0], h[2]], (0.0, 0.0, 0.0)), p), atol=1e-12)
+    assert np.allclose(IR.eval_f64(E.scale(s, 2.0), p), np.linalg.norm(p, axis=1) - 5.0, atol=1e-12)
+    assert np.allclose(IR.eval_f64(E.offset(s, 0.5), p), np.linalg.norm(p, axis=1) - 3.0, atol=1e-12)
+
+
+def test_f64_booleans_and_frustum(E):
+    rng = np.random.default_rng(1)
+    p = rng.uniform(-8, 8, (3000, 3))
+    a, b = E.sphere(4.0, (1.0, 0.0, 0.0)), E.box((2.0, 3.0, 2.5), (-1.0, 0.5, 0.0))
+    da, db = IR.eval_f64(a, p), IR.eval_f64(b, p)
+    assert np.array_equal(IR.eval_f64(E.union(a, b), p), np.minimum(da, db))
+    assert np.array_equal(IR.eval_f64(E.intersect(a, b), p), np.maximum(da, db))
+    assert np.array_equal(IR.eval_f64(E.subtract(a, b), p), np.maximum(da, -db))
+    k = 1.5
+    su = IR.eval_f64(E.union(a, b, k=k), p)
+    assert np.all(su <= np.minimum(da, db) + 1e-12) and np.all(su >= np.minimum(da, db) - k / 4 - 1e-12)
+    far = np.abs(da - db) >= k
+    assert np.array_equal(su[far], np.minimum(da, db)[far])
+    # equal profiles: the frustum is a box of half height (z1 - z0)/2; inside, its planes are the box's faces
+    rf = E.rect_frustum(-2.0, 4.0, 3.0, 2.0, 3.0, 2.0)
+    bx = E.box((3.0, 2.0, 3.0), (0.0, 0.0, 1.0))
+    inner = rng.uniform([-3, -2, -2], [3, 2, 4], (500, 3))
+    assert np.allclose(IR.eval_f64(rf, inner), IR.eval_f64(bx, inner), atol=1e-12)
+    # axis_project freezes one coordinate before the child
+    ap = E.axis_project(a, 2, 0.75)
+    frozen = p.copy()
+    frozen[:, 2] = 0.75
+    assert np.array_equal(IR.eval_f64(ap, p), IR.eval_f64(a, frozen))
+
+
+def test_f32_grid_leaf_reproduces_an_affine_field(E, tmp_path):
+    g = linear_grid()
+    origin, pitch = (-6.0, -5.0, -4.5), 1.5
+    rng = np.random.default_rng(2)
+    p = rng.uniform([-6, -5, -4.5], [6, 4, 6], (4000, 3)).astype(np.float32).astype(np.float64)
+    want = -1.0 + 0.5 * p[:, 0] - 0.25 * p[:, 1] + 0.75 * p[:, 2]
+    leaf = E.grid_leaf(g, origin, pitch, mode="sdf")
+    got = IR.eval_f32(leaf, p)
+    assert np.max(np.abs(got - want)) < 2e-5
+    far = np.array([[100.0, 0.0, 0.0], [-7.0, 0.0, 0.0]])
+    assert np.all(IR.eval_f32(leaf, far) == np.float32(1.0e6))
+    # the .npz form (FE/ikarus_v1/eval_warp.py:433-450): occupancy mode reads (iso - v) * pitch
+    path = tmp_path / "occ.npz"
+    np.savez(path, occupancy=g, global_origin_mm=np.array(origin), pitch_mm=np.float64(pitch))
+    occ = E.gridfield(str(path), key="occupancy", mode="occupancy", iso=0.5)
+    got = IR.eval_f32(occ, p)
+    assert np.max(np.abs(got - (0.5 - want) * pitch)) < 5e-5
+
+
+def test_f32_within_rounding_of_f64(E):
+    rng = np.random.default_rng(5)
+    for trial in range(40):
+        tree = random_tree(E, rng, 4)
+        p = rng.uniform(-10, 10, (1500, 3)).astype(np.float32).astype(np.float64)
+        v32 = IR.eval_f32(tree, p).astype(np.float64)
+        v64 = IR.eval_f64(tree, p)
+        scale = 1.0 + np.abs(v64) + np.abs(p).max()
+        assert np.all(np.abs(v32 - v64) <= 2e-5 * scale), trial
+
+
+def test_reference_selftest_gates(E):
+    """The reference's acceptance gates on its own trees and points."""
+    tree, pts = selftest_v1(E)
+    err = np.abs(IR.eval_f32(tree, pts).astype(np.float64) - IR.eval_f64(tree, pts))
+    assert err.max() < 1e-3
+    tree, pts = selftest_v2(E)
+    err = np.abs(IR.eval_f32(tree, pts).astype(np.float64) - IR.eval_f64(tree, pts))
+    assert err.max() < 1e-2
+
+
+def test_compiler_program_shape(E):
+    tree, _ = selftest_v1(E)
+    prog = E.compile_tree(tree)
+    ops = prog.ins[:, 0].tolist()
+    assert ops.count(E.SPHERE) == 1 and ops.count(E.BOX) == 1 and ops.count(E.CYL) == 1
+    assert ops.count(E.UNION) == 1 and ops.count(E.SUBTRACT) == 1 and ops.count(E.XFORM) == 1
+    assert prog.params.dtype == np.float32 and prog.depth == 2
+    # a smooth k of 2.0 and the cylinder's half height 20.0, as fp32
+    assert np.float32(2.0) in prog.params and np.float32(20.0) in prog.params
+    for right in (True, False):
+        p = E.compile_tree(chain(E, 200, right))
+        assert p.depth == 2
+    assert E.compile_tree(balanced(E, 8)).depth == 9
+    assert E.compile_tree(balanced(E, 16)).depth == 17
+    with pytest.raises(ValueError):
+        E.compile_tree(balanced(E, 17))
+    with pytest.raises(ValueError):
+        E.compile_tree({"op": "torus"})
+
+
+def test_json_round_trip(E, tmp_path):
+    rng = np.random.default_rng(9)
+    tree = random_tree(E, rng, 5)
+    path = E.save(tree, tmp_path / "t.json")
+    assert E.load(path) == json.loads(json.dumps(tree))
+    assert E.node_count(tree) >= 1
+    assert E.leaf_ops(E.union(E.sphere(1), E.cone(1, 2, 3))) == {"sphere", "cone"}
+    (tmp_path / "bare.json").write_text(json.dumps(tree))
+    assert E.load(tmp_path / "bare.json") == json.loads(json.dumps(tree))
+    m = E.mat4_mul(E.mat4_translate(1, 2, 3), E.mat4_rot_axis("x", 30))
+    inv = E.mat4_inverse_affine(m)
+    assert np.allclose(np.array(E.mat4_mul(m, inv)), np.eye(4), atol=1e-14)
+    assert inv == IR.inverse_affine(m)

@@ -1,0 +1,100 @@
+"""IKARUS trees on the GPU: the device interpreter bit-exact against the fp32
+oracle (oracle/ikarus_ref.py, which walks the tree itself, not the compiled
+program) on the reference's self-test trees, seeded random trees over every
+node type (grid leaves included), deep chains and balanced trees at the stack
+bound; the reference's acceptance gates against the float64 ground truth;
+gradients; and block-sparse fields built from trees, bit-exact (kinds, corner
+minima, active ids, tiles) against the oracle's classification and tiles."""
+import numpy as np
+import pytest
+
+import ikarus_ref as IR
+from ikarus_trees import balanced, chain, linear_grid, random_tree, selftest_v1, selftest_v2
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def E(fe):
+    return fe.ikarus
+
+
+def same_bits(a, b):
+    return np.asarray(a, np.float32).tobytes() == np.asarray(b, np.float32).tobytes()
+
+
+def test_reference_trees(E, gpu):
+    for make, gate in ((selftest_v1, 1e-3), (selftest_v2, 1e-2)):
+        tree, pts = make(E)
+        got = E.eval_batch(tree, pts)
+        assert same_bits(got, IR.eval_f32(tree, pts)), make.__name__
+        assert np.max(np.abs(got.astype(np.float64) - IR.eval_f64(tree, pts))) < gate
+
+
+def test_random_trees_match_oracle(E, gpu):
+    rng = np.random.default_rng(11)
+    leaf = E.grid_leaf(linear_grid(), (-6.0, -5.0, -4.5), 1.5, mode="sdf")
+    occ = E.grid_leaf(linear_grid(coef=(0.1, 0.2, -0.1), c0=0.4), (-4.0, -3.0, -2.0), 0.75, mode="occupancy",
+                      iso=0.5)
+    for trial in range(60):
+        tree = random_tree(E, rng, int(rng.integers(1, 6)), grid=leaf if trial % 2 else occ)
+        pts = rng.uniform(-12, 12, (5000, 3))
+        got = E.eval_batch(tree, pts)
+        want = IR.eval_f32(tree, pts)
+        if not same_bits(got, want):
+            bad = np.flatnonzero(got.view(np.uint32) != want.view(np.uint32))
+            raise AssertionError((trial, bad[:5], got[bad[:5]], want[bad[:5]]))
+
+
+def test_stack_bound_and_chains(E, gpu):
+    rng = np.random.default_rng(12)
+    pts = rng.uniform(-20, 20, (20000, 3))
+    for tree in (chain(E, 300, True), chain(E, 300, False), balanced(E, 6), balanced(E, 16)):
+        prog = E.compile_tree(tree)
+        print("depth", prog.depth, "instructions", len(prog.ins), flush=True)
+        sample = pts if len(prog.ins) < 10000 else pts[:256]
+        assert same_bits(E.eval_batch(prog, sample), IR.eval_f32(tree, sample))
+
+
+def test_gradient(E, gpu):
+    tree, pts = selftest_v1(E)
+    g = E.gradient_batch(tree, pts[:500], h=0.05)
+    offsets = np.array([[0.05, 0, 0], [-0.05, 0, 0], [0, 0.05, 0], [0, -0.05, 0], [0, 0, 0.05], [0, 0, -0.05]])
+    vals = IR.eval_f32(tree, (pts[:500][None] + offsets[:, None]).reshape(-1, 3)).reshape(6, 500)
+    want = np.stack([(vals[0] - vals[1]) / (2 * 0.05), (vals[2] - vals[3]) / (2 * 0.05),
+                     (vals[4] - vals[5]) / (2 * 0.05)], 1)
+    assert g.tobytes() == want.tobytes()
+    # unit length away from the blend, creases and the rotated sphere's neighbourhood
+    s = E.sphere(5.0, (1.0, 2.0, 3.0))
+    p = np.random.default_rng(3).normal(size=(300, 3))
+    p = p / np.linalg.norm(p, axis=1)[:, None] * 8.0 + np.array([1.0, 2.0, 3.0])
+    n = np.linalg.norm(E.gradient_batch(s, p), axis=1)
+    assert np.allclose(n, 1.0, atol=1e-3)
+
+
+def field_meta(fe, x0, y0, z0, pitch, n, B):
+    return fe.GridMeta(x0, y0, z0, pitch, n[0], n[1], n[2], block=B)
+
+
+def test_field_build_matches_oracle(fe, E, gpu):
+    rng = np.random.default_rng(13)
+    leaf = E.grid_leaf(linear_grid(), (-6.0, -5.0, -4.5), 1.5, mode="sdf")
+    trees = [selftest_v1(E)[0], selftest_v2(E)[0], random_tree(E, rng, 4, grid=leaf), random_tree(E, rng, 5)]
+    grids = [(-30.0, -30.0, -30.0, 0.5, (120, 120, 120), 8), (-30.0, -30.0, -30.0, 0.75, (130, 81, 81), 4),
+             (-12.0, -12.0, -12.0, 0.25, (96, 100, 90), 16), (-12.5, -11.0, -10.0, 0.3, (80, 70, 75), 8)]
+    for i, (tree, (x0, y0, z0, h, n, B)) in enumerate(zip(trees, grids)):
+        meta = field_meta(fe, x0, y0, z0, h, n, B)
+        sf = E.klassificera_och_evaluera_ikarus(meta, tree)
+        om = dict(x0=meta.x0, y0=meta.y0, z0=meta.z0, pitch=meta.pitch, margin=meta.margin_mm, B=B,
+                  nbx=meta.n_bx, nby=meta.n_by, nbz=meta.n_bz)
+        kind, cma = IR.classify_f32(tree, om)
+        print("field", i, meta.n_blocks, int((kind == 0).sum()), flush=True)
+        assert np.array_equal(sf.kind, kind), i
+        assert same_bits(sf.corner_min_abs, cma), i
+        ids = np.flatnonzero(kind == 0)
+        assert np.array_equal(sf.active_ids, ids), i
+        assert same_bits(sf.tiles, IR.tiles_f32(tree, om, ids)), i
+        assert 0 < len(ids) < meta.n_blocks
+        # the existing reconstruction serves tree-built fields
+        dense = fe.rekonstruera_tat_gpu(sf)
+        assert dense.shape == tuple(n)
