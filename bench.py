@@ -271,6 +271,76 @@ def bench_mesh(fe, torch, reps=5):
     return out
 
 
+def bench_ikarus(fe, torch, reps=5, n_points=1 << 24, cpu_points=1 << 18, cpu=True):
+    """IKARUS trees through the device interpreter: the reference's self-test
+    tree (smooth union of a cylinder minus a box and a rotated sphere,
+    FE/ikarus_v1/eval_warp.py:745-749) and a seeded 63-leaf random tree at 2^24
+    device-resident points, and the block-sparse 512^3 field of the self-test
+    tree (B = 8 over [-32, 32)^3); the CPU oracle (numpy fp32 tree walk, one
+    core) timed on a bounded sample beside it unless cpu is False."""
+    import random
+    E = fe.ikarus
+    dev = torch.device("cuda:0")
+    c = E.cylinder(radius=12.5, height=40.0)
+    b = E.box(half_extents=(8.0, 8.0, 8.0), center=(3.0, 0.0, 5.0))
+    s = E.sphere(radius=6.0, center=(-4.0, 2.0, -3.0))
+    selftest = E.union(E.subtract(c, b, k=0.0), E.rotate(s, "y", 37.0), k=2.0)
+    rng = np.random.default_rng(21)
+    leaves = []
+    for i in range(64):
+        cc = rng.uniform(-25, 25, 3)
+        leaf = (E.sphere(rng.uniform(2, 6), cc) if i % 3 == 0 else
+                E.box(rng.uniform(1, 5, 3), cc) if i % 3 == 1 else
+                E.rotate(E.cylinder(rng.uniform(1, 4), rng.uniform(4, 12), "z", cc), "x", rng.uniform(0, 90)))
+        leaves.append(leaf)
+    big = leaves[0]
+    for i, leaf in enumerate(leaves[1:]):
+        big = (E.union(big, leaf, k=1.0) if i % 4 else E.subtract(big, leaf))
+    pts = (torch.rand((n_points, 3), device=dev) * 64.0 - 32.0).contiguous()
+    out = torch.empty(n_points, dtype=torch.float32, device=dev)
+    res = {}
+    for name, tree in (("selftest_tree", selftest), ("random_63_ops", big)):
+        prog = E.compile_tree(tree)
+        ms = event_ms(torch, lambda: E.eval_device(prog, pts, out), reps)
+        res[name] = {"instructions": int(len(prog.ins)), "points": n_points, "ms": ms,
+                     "gpoints_s": n_points / (ms * 1e-3) / 1e9}
+        if cpu:
+            sys.path.insert(0, str(ROOT / "oracle"))
+            import ikarus_ref as IR
+            sample = pts[:cpu_points].cpu().numpy().astype(np.float64)
+            t0 = time.perf_counter()
+            IR.eval_f32(tree, sample)
+            cpu_s = time.perf_counter() - t0
+            res[name].update(cpu_oracle_mpoints_s=cpu_points / cpu_s / 1e6, cpu_sample_points=cpu_points,
+                             cpu_cores=1)
+    meta = fe.GridMeta(-32.0, -32.0, -32.0, 0.125, 512, 512, 512, block=8)
+    prog = E.compile_tree(selftest)
+    ins, par, grids, gdata = prog.device(dev)
+    L = fe._lib.lib()
+    P = fe._lib.ptr
+    st = fe._lib.stream_handle(None)
+    opm = np.array([meta.x0, meta.y0, meta.z0, meta.pitch, meta.margin_mm], np.float32)
+    blocks = np.array([meta.n_bx, meta.n_by, meta.n_bz], np.int32)
+    kind = torch.empty(meta.n_blocks, dtype=torch.int32, device=dev)
+    cma = torch.empty(meta.n_blocks, dtype=torch.float32, device=dev)
+    cls_ms = event_ms(torch, lambda: L.fe_ikarus_classify(P(ins), len(prog.ins), P(par), P(grids), P(gdata),
+                                                          opm.ctypes.data, blocks.ctypes.data, 8, P(kind), P(cma),
+                                                          st), reps)
+    ids = torch.nonzero(kind == 0).reshape(-1).to(torch.int32)
+    na = int(ids.numel())
+    tiles = torch.empty(na * 512, dtype=torch.float32, device=dev)
+    tile_ms = event_ms(torch, lambda: L.fe_ikarus_tiles(P(ins), len(prog.ins), P(par), P(grids), P(gdata),
+                                                        opm.ctypes.data, blocks.ctypes.data, 8, P(ids), na, P(tiles),
+                                                        st), reps)
+    t0 = time.perf_counter()
+    sf = E.klassificera_och_evaluera_ikarus(meta, selftest)
+    plain_ms = (time.perf_counter() - t0) * 1e3
+    res["field_512"] = {"n_blocks": meta.n_blocks, "n_active": na, "classify_ms": cls_ms, "tiles_ms": tile_ms,
+                        "device_mvox_s": 512 ** 3 / ((cls_ms + tile_ms) * 1e-3) / 1e6,
+                        "plain_call_ms_incl_host_copies": plain_ms, "plain_n_active": int(len(sf.active_ids))}
+    return res
+
+
 def bench_serve(fe, torch, rates, count):
     """Poisson arrivals of 512^3 / 1024-primitive recipes into the field server."""
     rng = np.random.default_rng(5)
@@ -305,10 +375,10 @@ def main():
     ap.add_argument("--quick", action="store_true", help="single-field step only")
     ap.add_argument("--batch-count", type=int, default=256)
     ap.add_argument("--serve-count", type=int, default=48)
-    ap.add_argument("--only", choices=["edt-heavy", "kernels", "mesh"], default=None,
+    ap.add_argument("--only", choices=["edt-heavy", "kernels", "mesh", "ikarus"], default=None,
                     help="edt-heavy: the thin-feature EDT section only; kernels: each hot kernel of the "
                          "512^3 step three times plus a 1 GiB copy (for rocprofv3 counter runs); "
-                         "mesh: the mesh-stage section only")
+                         "mesh: the mesh-stage section only; ikarus: the IKARUS-tree section only")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU: the block-sparse build and the EDT run on one "
@@ -325,6 +395,9 @@ def main():
         return 0
     if args.only == "mesh":
         print(json.dumps({"mesh": bench_mesh(fe, torch)}))
+        return 0
+    if args.only == "ikarus":
+        print(json.dumps({"ikarus": bench_ikarus(fe, torch, cpu=not args.no_cpu)}))
         return 0
     n = args.n
     rec = fe.recipes.large_field_512(seed=2) if n == 512 else fe.recipes.mixed_recipe(np.random.default_rng(2), 1024, n)
@@ -411,6 +484,7 @@ def main():
         out["served"] = bench_serve(fe, torch, (200.0, 2000.0), args.serve_count)
         out["edt_heavy"] = bench_edt_heavy(fe, torch)
         out["mesh"] = bench_mesh(fe, torch)
+        out["ikarus"] = bench_ikarus(fe, torch, cpu=not args.no_cpu)
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", "16"))
         out["cpu_baseline"] = cpu_baseline(fe, rec, occ_host, threads, n)
