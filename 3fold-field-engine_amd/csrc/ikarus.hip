@@ -306,6 +306,92 @@ __global__ __launch_bounds__(IK_T) void k_ik_tiles(IkProgram P, IkGridMeta G, co
     tiles[t] = ik_eval(P.ins, P.n, P.par, P.grids, P.gdata, x, y, z, stk + threadIdx.x);
 }
 
+// Block-level sums of two counts into 64-bit global counters (one atomic
+// each per block).
+__device__ __forceinline__ void block_count(int a, int b, unsigned long long* out)
+{
+    __shared__ int sa[IK_T / FE_WAVE], sb[IK_T / FE_WAVE];
+    const int lane = threadIdx.x & (FE_WAVE - 1), wave = threadIdx.x / FE_WAVE;
+    const int wa = __popcll(__ballot(a != 0)), wb = __popcll(__ballot(b != 0));
+    if (lane == 0) {
+        sa[wave] = wa;
+        sb[wave] = wb;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int ta = 0, tb = 0;
+        for (int w = 0; w < IK_T / FE_WAVE; ++w) {
+            ta += sa[w];
+            tb += sb[w];
+        }
+        if (ta) atomicAdd(out, (unsigned long long)ta);
+        if (tb) atomicAdd(out + 1, (unsigned long long)tb);
+    }
+}
+
+// Regular-grid pass of a hierarchical query (FE/ikarus_v1/queries.py:120-131,
+// 211-223): sample i = (lo + pitch*i) per axis in f64, rounded to fp32 (what
+// NumPy's grid and eval_batch's fp32 cast produce), C order; counts[0] +=
+// samples with v < below, counts[1] += samples with |v| <= band; the indices
+// of the band samples are appended to `cells` (order unspecified) through
+// cursor (cells may be null to count only).
+__global__ __launch_bounds__(IK_T) void k_ik_grid(IkProgram P, double lx, double ly, double lz, double pitch, int nx,
+                                                   int ny, int nz, float below, float band,
+                                                   unsigned long long* counts, int32_t* cells,
+                                                   unsigned int* cursor)
+{
+    __shared__ float stk[IK_STACK * IK_T];
+    const int64_t n = (int64_t)nx * ny * nz;
+    const int64_t t = (int64_t)blockIdx.x * IK_T + threadIdx.x;
+    bool in = false, bd = false;
+    if (t < n) {
+        const int i = (int)(t / ((int64_t)ny * nz)), j = (int)((t / nz) % ny), k = (int)(t % nz);
+        const float x = (float)(lx + pitch * (double)i), y = (float)(ly + pitch * (double)j),
+                    z = (float)(lz + pitch * (double)k);
+        const float v = ik_eval(P.ins, P.n, P.par, P.grids, P.gdata, x, y, z, stk + threadIdx.x);
+        in = v < below;
+        bd = fabsf(v) <= band;
+    }
+    if (cells) {
+        const uint64_t m = __ballot(bd);
+        const int lane = threadIdx.x & (FE_WAVE - 1);
+        unsigned int base = 0;
+        if (lane == 0 && m) base = atomicAdd(cursor, (unsigned int)__popcll(m));
+        base = __shfl(base, 0);
+        if (bd) cells[base + __popcll(m & ((1ull << lane) - 1))] = (int32_t)t;
+    }
+    block_count(in, bd, counts);
+}
+
+// Refinement of band cells (FE/ikarus_v1/queries.py:234-249): cell c (grid
+// index) gets r^3 samples at centre(c) + ((a + 0.5)/r * coarse - half) per axis
+// (f64, a-major order), rounded to fp32; counts[0] += samples with v < 0,
+// counts[1] += samples with |v| <= band.
+__global__ __launch_bounds__(IK_T) void k_ik_refine(IkProgram P, double lx, double ly, double lz, double pitch, int ny,
+                                                     int nz, const int32_t* __restrict__ cells, int64_t ncells, int r,
+                                                     double half, float band, unsigned long long* counts)
+{
+    __shared__ float stk[IK_STACK * IK_T];
+    const int64_t r3 = (int64_t)r * r * r;
+    const int64_t t = (int64_t)blockIdx.x * IK_T + threadIdx.x;
+    bool in = false, bd = false;
+    if (t < ncells * r3) {
+        const int64_t c = cells[t / r3];
+        const int sub = (int)(t % r3);
+        const int a = sub / (r * r), b = (sub / r) % r, e = sub % r;
+        const int i = (int)(c / ((int64_t)ny * nz)), j = (int)((c / nz) % ny), k = (int)(c % nz);
+        const double ox = ((double)a + 0.5) / (double)r * pitch - half;
+        const double oy = ((double)b + 0.5) / (double)r * pitch - half;
+        const double oz = ((double)e + 0.5) / (double)r * pitch - half;
+        const float x = (float)((lx + pitch * (double)i) + ox), y = (float)((ly + pitch * (double)j) + oy),
+                    z = (float)((lz + pitch * (double)k) + oz);
+        const float v = ik_eval(P.ins, P.n, P.par, P.grids, P.gdata, x, y, z, stk + threadIdx.x);
+        in = v < 0.0f;
+        bd = fabsf(v) <= band;
+    }
+    block_count(in, bd, counts);
+}
+
 bool valid_program(const void* ins, int n_ins, const void* par)
 {
     return ins && par && n_ins > 0 && n_ins <= (1 << 20);
@@ -366,6 +452,46 @@ extern "C" int fe_ikarus_tiles(const void* ins, int n_ins, const float* params, 
     const int64_t total = n_active * B * B * B;
     hipLaunchKernelGGL(k_ik_tiles, dim3((unsigned)((total + IK_T - 1) / IK_T)), dim3(IK_T), 0,
                        static_cast<hipStream_t>(stream), P, G, ids, n_active, tiles);
+    FE_TRY(hipGetLastError());
+    return FE_OK;
+}
+
+// One regular-grid pass of a hierarchical query (see k_ik_grid): lo and pitch
+// in f64 (host values), nx*ny*nz < 2^31 samples; counts: device uint64[2]
+// (added to); cells/cursor: device int32[nx*ny*nz] and uint32 (zeroed by the
+// caller), or both null.
+extern "C" int fe_ikarus_grid(const void* ins, int n_ins, const float* params, const void* grids, const float* grid_data,
+                              const double* lo, double pitch, const int32_t* shape, float below, float band,
+                              unsigned long long* counts, int32_t* cells, unsigned int* cursor, void* stream) noexcept
+{
+    if (!valid_program(ins, n_ins, params) || !lo || !shape || !counts || !(pitch > 0) || (!cells) != (!cursor))
+        return FE_INVALID;
+    const int64_t n = (int64_t)shape[0] * shape[1] * shape[2];
+    if (shape[0] < 1 || shape[1] < 1 || shape[2] < 1 || n >= INT32_MAX) return FE_INVALID;
+    const IkProgram P{static_cast<const IkIns*>(ins), params, static_cast<const IkGrid*>(grids), grid_data, n_ins};
+    hipLaunchKernelGGL(k_ik_grid, dim3((unsigned)((n + IK_T - 1) / IK_T)), dim3(IK_T), 0,
+                       static_cast<hipStream_t>(stream), P, lo[0], lo[1], lo[2], pitch, shape[0], shape[1], shape[2],
+                       below, band, counts, cells, cursor);
+    FE_TRY(hipGetLastError());
+    return FE_OK;
+}
+
+// Refinement pass (see k_ik_refine) over ncells band cells of the grid (lo,
+// pitch, shape) with r^3 samples each.
+extern "C" int fe_ikarus_refine(const void* ins, int n_ins, const float* params, const void* grids,
+                                const float* grid_data, const double* lo, double pitch, const int32_t* shape,
+                                const int32_t* cells, int64_t ncells, int r, float band, unsigned long long* counts,
+                                void* stream) noexcept
+{
+    if (!valid_program(ins, n_ins, params) || !lo || !shape || !counts || !(pitch > 0) || r < 1 || r > 64 ||
+        ncells < 0 || (ncells && !cells))
+        return FE_INVALID;
+    const int64_t total = ncells * r * r * r;
+    if (total == 0) return FE_OK;
+    const IkProgram P{static_cast<const IkIns*>(ins), params, static_cast<const IkGrid*>(grids), grid_data, n_ins};
+    hipLaunchKernelGGL(k_ik_refine, dim3((unsigned)((total + IK_T - 1) / IK_T)), dim3(IK_T), 0,
+                       static_cast<hipStream_t>(stream), P, lo[0], lo[1], lo[2], pitch, shape[1], shape[2], cells,
+                       ncells, r, pitch / 2.0, band, counts);
     FE_TRY(hipGetLastError());
     return FE_OK;
 }

@@ -273,6 +273,21 @@ int fe_ikarus_tiles(const void *ins, int n_ins, const float *params, const void 
                     const float *origin_pitch_margin, const int32_t *blocks, int B, const int32_t *ids,
                     int64_t n_active, float *tiles, void *stream);
 
+/* The passes of the hierarchical volume queries of ikarus_v1/queries.py:138-277
+ * on an implicit grid (lo: 3 host doubles, shape: 3 host int32, samples
+ * lo + pitch*i per axis in f64, rounded to fp32).  grid: counts[0] += samples
+ * with v < below, counts[1] += samples with |v| <= band (device uint64[2]);
+ * the band samples' grid indices are appended to cells through *cursor
+ * (device int32[n] and uint32, zeroed by the caller; both may be NULL).
+ * refine: r^3 samples per listed cell at centre + ((a+0.5)/r*pitch - pitch/2);
+ * counts[0] += v < 0, counts[1] += |v| <= band. */
+int fe_ikarus_grid(const void *ins, int n_ins, const float *params, const void *grids, const float *grid_data,
+                   const double *lo, double pitch, const int32_t *shape, float below, float band,
+                   unsigned long long *counts, int32_t *cells, unsigned int *cursor, void *stream);
+int fe_ikarus_refine(const void *ins, int n_ins, const float *params, const void *grids, const float *grid_data,
+                     const double *lo, double pitch, const int32_t *shape, const int32_t *cells, int64_t ncells, int r,
+                     float band, unsigned long long *counts, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -327,3 +327,37 @@ def tiles_f32(node, meta, ids):
     y = F(meta["y0"]) + (bj[:, None, None, None] * B + lj[None]).astype(F) * h
     z = F(meta["z0"]) + (bk[:, None, None, None] * B + lk[None]).astype(F) * h
     return eval_f32(node, np.stack([x.ravel(), y.ravel(), z.ravel()], 1)).reshape(len(ids), B, B, B)
+
+
+# ------------------------------------------------------------------ hierarchical volume (FE/ikarus_v1/queries.py)
+def grid_points(lo, hi, pitch):
+    """FE/ikarus_v1/queries.py:120-131: samples lo + pitch*i, C order (float64)."""
+    n = [max(int(math.ceil((hi[a] - lo[a]) / pitch)) + 1, 1) for a in range(3)]
+    axes = [lo[a] + pitch * np.arange(n[a]) for a in range(3)]
+    X, Y, Z = np.meshgrid(*axes, indexing="ij")
+    return np.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=1), tuple(n)
+
+
+def grid_volume(node, lo, hi, coarse_pitch, refine_factor):
+    """The coarse and refine stages of FE/ikarus_v1/queries.py:211-264 at given
+    pitches, on eval_f32 (points cast to fp32 as eval_batch does)."""
+    pts_c, shape = grid_points(lo, hi, coarse_pitch)
+    vals_c = eval_f32(node, pts_c)
+    diag_half = 0.5 * coarse_pitch * math.sqrt(3.0)
+    inside = vals_c < F(-diag_half)
+    boundary = np.abs(vals_c) <= F(diag_half)
+    out = dict(n_fully_inside=int(inside.sum()), n_boundary=int(boundary.sum()), grid_shape=shape,
+               volume_mm3=float(inside.sum()) * coarse_pitch ** 3)
+    fine_in = fine_bd = 0
+    if out["n_boundary"] and refine_factor > 1:
+        half = coarse_pitch / 2.0
+        offs = (np.arange(refine_factor) + 0.5) / refine_factor * coarse_pitch - half
+        OX, OY, OZ = np.meshgrid(offs, offs, offs, indexing="ij")
+        local = np.stack([OX.ravel(), OY.ravel(), OZ.ravel()], axis=1)
+        fine = (pts_c[boundary][:, None, :] + local[None, :, :]).reshape(-1, 3)
+        fv = eval_f32(node, fine)
+        fine_in = int((fv < F(0)).sum())
+        fine_bd = int((np.abs(fv) <= F(0.5 * (coarse_pitch / refine_factor) * math.sqrt(3.0))).sum())
+        out["volume_mm3"] += float(fine_in) * (coarse_pitch / refine_factor) ** 3
+    out.update(n_fine_inside=fine_in, n_fine_boundary=fine_bd)
+    return out

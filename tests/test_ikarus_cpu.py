@@ -160,3 +160,25 @@ def test_json_round_trip(E, tmp_path):
     inv = E.mat4_inverse_affine(m)
     assert np.allclose(np.array(E.mat4_mul(m, inv)), np.eye(4), atol=1e-14)
     assert inv == IR.inverse_affine(m)
+
+
+def test_query_helpers(fe, E):
+    Q = fe.ikarus_queries
+    lo, hi = Q.bbox_of(E.union(E.sphere(2.0, (1, 2, 3)), E.translate(E.box((1, 2, 3)), 10, 0, 0)), pad=0.0)
+    assert np.allclose(lo, [-1, -2, -3]) and np.allclose(hi, [11, 4, 5])
+    lo, hi = Q.bbox_of(E.rotate(E.box((2, 1, 1)), "z", 90.0), pad=0.5)
+    assert np.allclose(lo, [-1.5, -2.5, -1.5]) and np.allclose(hi, [1.5, 2.5, 1.5])
+    assert Q.grid_shape([0, 0, 0], [1, 2, 3], 0.5) == (3, 5, 7)
+    assert Q.affordable_points(100.0, 0.01, 1e6) == 90000
+    assert Q.affordable_points(5.0, 0.01, 1e6, floor_points=1000) == 1000
+
+
+def test_oracle_grid_volume_of_box_overlap(E):
+    """Two axis-aligned boxes: the counting restatement against the exact overlap."""
+    a = E.box((3.0, 2.0, 2.5), (0.0, 0.0, 0.0))
+    b = E.box((2.0, 3.0, 1.5), (1.5, 0.5, 1.0))
+    exact = (3.0 - (-0.5)) * (2.0 - (-2.0)) * (2.5 - (-0.5))
+    lo, hi = np.array([-0.5, -2.0, -0.5]) - 1, np.array([3.0, 2.0, 2.5]) + 1
+    r = IR.grid_volume(E.intersect(a, b), lo, hi, 0.4, 4)
+    assert abs(r["volume_mm3"] - exact) < 0.05 * exact
+    assert r["n_boundary"] > 0 and r["n_fine_inside"] > 0

@@ -5,6 +5,8 @@ node type (grid leaves included), deep chains and balanced trees at the stack
 bound; the reference's acceptance gates against the float64 ground truth;
 gradients; and block-sparse fields built from trees, bit-exact (kinds, corner
 minima, active ids, tiles) against the oracle's classification and tiles."""
+import math
+
 import numpy as np
 import pytest
 
@@ -98,3 +100,36 @@ def test_field_build_matches_oracle(fe, E, gpu):
         # the existing reconstruction serves tree-built fields
         dense = fe.rekonstruera_tat_gpu(sf)
         assert dense.shape == tuple(n)
+
+
+def test_grid_volume_counts_match_oracle(fe, E, gpu):
+    Q = fe.ikarus_queries
+    rng = np.random.default_rng(14)
+    trees = [selftest_v1(E)[0], E.intersect(E.sphere(6.0, (0.5, 0.2, 0.1)), E.box((4, 5, 3), (1, 0, 0))),
+             random_tree(E, rng, 4)]
+    for tree in trees:
+        lo, hi = np.array([-14.0, -13.0, -12.5]), np.array([13.0, 12.0, 14.5])
+        for pitch, r in ((1.0, 4), (0.7, 3), (1.3, 1)):
+            got = Q.grid_volume(tree, lo, hi, pitch, r)
+            want = IR.grid_volume(tree, lo, hi, pitch, r)
+            for k in ("n_fully_inside", "n_boundary", "n_fine_inside", "n_fine_boundary"):
+                assert got[k] == want[k], (k, pitch, r)
+            assert got["volume_mm3"] == want["volume_mm3"]
+
+
+def test_intrusion_volume_closed_forms(fe, E, gpu):
+    Q = fe.ikarus_queries
+    r1, r2, d = 6.0, 4.0, 7.0
+    lens = math.pi * (r1 + r2 - d) ** 2 * (d * d + 2 * d * r2 - 3 * r2 * r2 + 2 * d * r1 + 6 * r1 * r2
+                                           - 3 * r1 * r1) / (12 * d)
+    res = Q.intrusion_volume(E.sphere(r1, (0, 0, 0)), E.sphere(r2, (d, 0, 0)), budget_ms=200.0)
+    assert res["method"] == "hierarchical_prune_adaptive_refine"
+    assert abs(res["volume_mm3"] - lens) <= res["uncertainty_mm3"] + 0.02 * lens, (res["volume_mm3"], lens)
+    assert res["dispatch_log"]["refine_factor_chosen"] >= 1 and res["dispatch_log"]["n_coarse_points"] > 0
+    a = E.box((3.0, 2.0, 2.5), (0.0, 0.0, 0.0))
+    b = E.box((2.0, 3.0, 1.5), (1.5, 0.5, 1.0))
+    exact = 3.5 * 4.0 * 3.0
+    res = Q.intrusion_volume(a, b, budget_ms=100.0)
+    assert abs(res["volume_mm3"] - exact) <= res["uncertainty_mm3"] + 0.02 * exact
+    far = Q.intrusion_volume(E.sphere(1.0, (0, 0, 0)), E.sphere(1.0, (50, 0, 0)))
+    assert far["volume_mm3"] == 0.0 and far["uncertainty_mm3"] == 0.0
