@@ -231,7 +231,37 @@ def bench_octree(fe, torch, dense, reps=3, queries=1 << 20):
             "lookups": queries, "lookup_ms": ms, "lookups_per_s": queries / (ms * 1e-3), "leaf_histogram": hist}
 
 
-def bench_mesh(fe, torch, reps=5):
+def mesh_cpu_oracle_ms(v, f, pitch, lo, closed, reps=5):
+    """The oracle's whole mesh stage (C column scan + column mask + paired EDT,
+    or the C winding numbers) on the host cores, median of reps."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle as O
+    g = 0.3819660112501051, 0.6180339887498949
+    gmin = np.floor((v.min(0) - lo) / pitch).astype(np.int64) - 2
+    gmax = np.ceil((v.max(0) - lo) / pitch).astype(np.int64) + 3
+    shape = tuple(int(x) for x in gmax - gmin)
+    anchor = lo + gmin * pitch
+    times = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        if closed:
+            ij = np.indices(shape[:2]).reshape(2, -1).T
+            xy = anchor[:2] + ij * pitch
+            xy[:, 0] += pitch * 1e-4 * g[0]
+            xy[:, 1] += pitch * 1e-4 * g[1]
+            counts, depths, signs = O.columns_scan(v[f], xy)
+            solid = O.column_mask(counts, depths, signs, shape, anchor[2] + np.arange(shape[2]) * pitch)[0]
+        else:
+            pts = anchor + np.indices(shape).reshape(3, -1).T * pitch
+            pts[:, 0] += pitch * 1e-4 * g[0]
+            pts[:, 1] += pitch * 1e-4 * g[1]
+            solid = (np.abs(O.gwn((v - anchor)[f], pts - anchor)) > .5).reshape(shape)
+        O.edt_pair(solid, pitch)
+        times.append((time.perf_counter() - t0) * 1e3)
+    return float(np.median(times))
+
+
+def bench_mesh(fe, torch, reps=5, cpu=True):
     """Mesh stage (column hits -> mask -> surface -> paired EDT -> signed
     distance, or winding numbers for open meshes), one captured graph per
     prepared mesh.  query_ms: one evaluate() including the copies of solid,
@@ -254,6 +284,8 @@ def bench_mesh(fe, torch, reps=5):
         out[name] = {"method": field.method, "shape": list(field.shape), "query_ms": q_ms, "replay_ms": r_ms,
                      "reference_query_ms": ref}
         field.close()
+        if cpu:
+            out[name]["cpu_oracle_ms"] = mesh_cpu_oracle_ms(v, f, pitch, lo, field.method == "cuda_columns")
     v, f = fe.recipes.mesh_scene()
     t0 = time.perf_counter()
     field = fe.PreparedMeshField(v, f, 1.0, np.zeros(3))
@@ -394,7 +426,7 @@ def main():
         print(json.dumps(kernel_launches(fe, torch, args.n)))
         return 0
     if args.only == "mesh":
-        print(json.dumps({"mesh": bench_mesh(fe, torch)}))
+        print(json.dumps({"mesh": bench_mesh(fe, torch, cpu=not args.no_cpu)}))
         return 0
     if args.only == "ikarus":
         print(json.dumps({"ikarus": bench_ikarus(fe, torch, cpu=not args.no_cpu)}))
@@ -483,7 +515,7 @@ def main():
         out["batched"] = bench_batch(fe, torch, args.batch_count, plain_sample=32)
         out["served"] = bench_serve(fe, torch, (200.0, 2000.0), args.serve_count)
         out["edt_heavy"] = bench_edt_heavy(fe, torch)
-        out["mesh"] = bench_mesh(fe, torch)
+        out["mesh"] = bench_mesh(fe, torch, cpu=not args.no_cpu)
         out["ikarus"] = bench_ikarus(fe, torch, cpu=not args.no_cpu)
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", "16"))
