@@ -14,11 +14,15 @@
 // contraction (the library is built with -ffp-contract=off), so hits are
 // bit-identical, in the same per-column order.
 //
+// Each wave further keeps, of every 256 staged triangles, only those meeting
+// its own 16 x 4 columns' box (ballot compaction, order kept).
+//
 // Mask.  The depth range is reduced in the scan kernel through order-preserving
 // int64 atomics (min/max are exact, so the order of arrival does not matter),
-// keys and per-z fractions follow FE/mesh_fused_native_v1/field.cu:14-23, one
-// thread per voxel sums the signs of the hits above it, and the surface is the
-// 6-neighbour boundary of the mask.  The paired EDT and the signed output are
+// keys and per-z fractions follow FE/mesh_fused_native_v1/field.cu:14-23; one
+// wave per column sums the signs of the hits above each voxel (the column's
+// hits are wave-uniform, so they are scalar loads) and derives the
+// 6-neighbour surface in the same pass.  The paired EDT and the signed output are
 // edt.hip's device passes.  The whole chain is captured once into a HIP graph
 // (no memset nodes: a one-thread kernel clears the range cells and flags).
 #include "fe_common.hpp"
@@ -213,12 +217,17 @@ __global__ __launch_bounds__(MT) void k_scan(const double* __restrict__ tris, co
     if (tile >= ntiles) return;
     __shared__ double tri[9][MT];
     __shared__ double red[2][MT / FE_WAVE];
+    __shared__ uint8_t wlist[MT / FE_WAVE][MT];
     const int64_t q = tile_query(m, tile, threadIdx.x);
     double qx = 0.0, qy = 0.0;
     if (q >= 0) {
         qx = xy[2 * q];
         qy = xy[2 * q + 1];
     }
+    // this wave's query box: each wave keeps only the staged triangles that meet it
+    const int wave = threadIdx.x / FE_WAVE, lane = threadIdx.x & (FE_WAVE - 1);
+    const double wx0 = wave_min(q >= 0 ? qx : INFINITY), wx1 = wave_max(q >= 0 ? qx : -INFINITY);
+    const double wy0 = wave_min(q >= 0 ? qy : INFINITY), wy1 = wave_max(q >= 0 ? qy : -INFINITY);
     const int64_t beg = offset[tile], end = offset[tile + 1];
     int found = 0;
     bool bad = false;
@@ -232,8 +241,24 @@ __global__ __launch_bounds__(MT) void k_scan(const double* __restrict__ tris, co
             for (int c = 0; c < 9; ++c) tri[c][threadIdx.x] = p[c];
         }
         __syncthreads();
+        int cnt = 0;  // the wave's candidates among the staged triangles, in list (= face) order
+        for (int r = 0; r < MT / FE_WAVE; ++r) {
+            const int k = r * FE_WAVE + lane;
+            bool hit = false;
+            if (k < n) {
+                const double ax = tri[0][k], ay = tri[1][k], bx = tri[3][k], by = tri[4][k], cx = tri[6][k],
+                             cy = tri[7][k];
+                hit = !(fmax(ax, fmax(bx, cx)) < wx0 || fmin(ax, fmin(bx, cx)) > wx1 ||
+                        fmax(ay, fmax(by, cy)) < wy0 || fmin(ay, fmin(by, cy)) > wy1);
+            }
+            const uint64_t m = __ballot(hit);
+            if (hit) wlist[wave][cnt + __popcll(m & ((1ull << lane) - 1))] = (uint8_t)k;
+            cnt += __popcll(m);
+        }
+        __syncthreads();
         if (q < 0) continue;
-        for (int k = 0; k < n; ++k) {
+        for (int u = 0; u < cnt; ++u) {
+            const int k = wlist[wave][u];
             const double ax = tri[0][k], ay = tri[1][k], bx = tri[3][k], by = tri[4][k], cx = tri[6][k], cy = tri[7][k];
             if (qx < fmin(ax, fmin(bx, cx)) || qx > fmax(ax, fmax(bx, cx)) || qy < fmin(ay, fmin(by, cy)) ||
                 qy > fmax(ay, fmax(by, cy)))
@@ -264,8 +289,7 @@ __global__ __launch_bounds__(MT) void k_scan(const double* __restrict__ tris, co
     if (f) atomicOr(reinterpret_cast<int*>(cells + 2), f);
     lo = wave_min(lo);
     hi = wave_max(hi);
-    const int wave = threadIdx.x / FE_WAVE;
-    if ((threadIdx.x & (FE_WAVE - 1)) == 0) {
+    if (lane == 0) {
         red[0][wave] = lo;
         red[1][wave] = hi;
     }
@@ -333,34 +357,120 @@ __device__ __forceinline__ void block_flags(int local, int* flags)
     }
 }
 
-constexpr int VPT = 8;  // voxels per thread of the classification kernels
 
-// One thread per voxel (column-major, z fastest): solid iff the signs of the
-// column's hits whose key exceeds column + fraction(z) sum to nonzero
-// (FE/mesh_fused_native_v1/field.cu:24-30).
-// VPT voxels per thread, 256 apart, so each store instruction stays coalesced.
-__global__ __launch_bounds__(MT) void k_classify(const int32_t* __restrict__ counts, const int32_t* __restrict__ signs,
-                                                  const double* __restrict__ keys,
-                                                  const double* __restrict__ fractions, int cap, int nz,
-                                                  uint32_t size, uint8_t* __restrict__ mask, int* flags)
+// Sum of the signs of a column's stored hits whose key exceeds q
+// (FE/mesh_fused_native_v1/field.cu:24-30).  col must be wave-uniform: the
+// hit loads are then scalar loads, shared by the wave.
+__device__ __forceinline__ int column_sum(const int32_t* __restrict__ counts, const int32_t* __restrict__ signs,
+                                          const double* __restrict__ keys, int cap, int64_t col, double q)
 {
+    const int n = min(max(counts[col], 0), cap);
+    const double* k = keys + col * cap;
+    const int32_t* s = signs + col * cap;
+    int sum = 0;
+    for (int j = 0; j < n; ++j)
+        if (k[j] > q) sum += s[j];
+    return sum;
+}
+
+// A column's hits held one per lane (columns of at most 64 stored hits): the
+// sum over them reads each key and sign with readlane, so the loop has no
+// memory latency.
+struct LaneHits {
+    double key;
+    int sign, n;
+};
+
+__device__ __forceinline__ LaneHits lane_hits(const int32_t* __restrict__ counts, const int32_t* __restrict__ signs,
+                                              const double* __restrict__ keys, int cap, int64_t col, int lane)
+{
+    LaneHits h;
+    h.n = min(max(counts[col], 0), cap);
+    h.key = 0.0;
+    h.sign = 0;
+    if (lane < h.n && lane < FE_WAVE) {
+        h.key = keys[col * cap + lane];
+        h.sign = signs[col * cap + lane];
+    }
+    return h;
+}
+
+__device__ __forceinline__ int lane_sum(const LaneHits& h, double q)
+{
+    int sum = 0;
+    for (int j = 0; j < h.n; ++j) {
+        const double k = __longlong_as_double((long long)readlane64((uint64_t)__double_as_longlong(h.key), j));
+        const int sg = __builtin_amdgcn_readlane(h.sign, j);
+        sum += k > q ? sg : 0;
+    }
+    return sum;
+}
+
+// Mask and surface of the closed-mesh stage in one pass: one wave per column
+// (x, y).  The column's mask (one voxel per lane, 64 z at a time) is written
+// and kept in LDS; a solid voxel off every border is surface unless its z
+// neighbours (LDS) and its four xy-neighbour columns (evaluated from their
+// hits at this z) are all solid -- the 6-neighbour rule of
+// FE/mesh_fused_native_v1/field.cu:31-36.  Dynamic LDS: 4 * nz bytes.
+__global__ __launch_bounds__(MT) void k_mask_surface(const int32_t* __restrict__ counts,
+                                                      const int32_t* __restrict__ signs,
+                                                      const double* __restrict__ keys,
+                                                      const double* __restrict__ fractions, int cap, int nx, int ny,
+                                                      int nz, uint8_t* __restrict__ mask,
+                                                      uint8_t* __restrict__ surface, int* flags)
+{
+    extern __shared__ uint8_t colmask[];
+    const int wave = threadIdx.x / FE_WAVE, lane = threadIdx.x & (FE_WAVE - 1);
+    const int64_t ncol = (int64_t)nx * ny;
+    const int64_t col = (int64_t)__builtin_amdgcn_readfirstlane((int)(blockIdx.x * (MT / FE_WAVE) + wave));
+    const bool valid = col < ncol;
+    uint8_t* own = colmask + wave * nz;
     int local = 0;
-    const uint64_t base = (uint64_t)blockIdx.x * MT * VPT + threadIdx.x;
-#pragma unroll 1
-    for (int v = 0; v < VPT; ++v) {
-        const uint64_t iv = base + (uint64_t)v * MT;
-        if (iv >= size) break;
-        const uint32_t i = (uint32_t)iv;
-        const uint32_t col = i / (uint32_t)nz, z = i - col * (uint32_t)nz;
-        const int n = min(max(counts[col], 0), cap);
-        const double q = (double)col + fractions[z];
-        const double* k = keys + (int64_t)col * cap;
-        const int32_t* s = signs + (int64_t)col * cap;
-        int sum = 0;
-        for (int j = 0; j < n; ++j)
-            if (k[j] > q) sum += s[j];
-        mask[i] = sum != 0;
-        local |= sum != 0 ? MF_IN : MF_OUT;
+    if (valid) {
+        const LaneHits h = lane_hits(counts, signs, keys, cap, col, lane);
+        for (int z0 = 0; z0 < nz; z0 += FE_WAVE) {
+            const int z = z0 + lane;
+            const double q = (double)col + (z < nz ? fractions[z] : 0.0);
+            const int sum = h.n <= FE_WAVE ? lane_sum(h, q) : column_sum(counts, signs, keys, cap, col, q);
+            if (z < nz) {
+                const bool b = sum != 0;
+                own[z] = b;
+                mask[col * nz + z] = b;
+                local |= b ? MF_IN : MF_OUT;
+            }
+        }
+    }
+    __syncthreads();
+    if (valid) {
+        const int x = (int)(col / ny), y = (int)(col % ny);
+        const bool inner = x > 0 && x + 1 < nx && y > 0 && y + 1 < ny;
+        LaneHits nb[4];
+        if (inner)
+            for (int d = 0; d < 4; ++d)
+                nb[d] = lane_hits(counts, signs, keys, cap, d == 0 ? col - ny : d == 1 ? col + ny : d == 2 ? col - 1
+                                                                                                           : col + 1,
+                                  lane);
+        for (int z0 = 0; z0 < nz; z0 += FE_WAVE) {
+            const int z = z0 + lane;
+            bool sfc = false;
+            bool need = false;
+            double f = 0.0;
+            if (z < nz) {
+                sfc = own[z];
+                need = sfc && inner && z > 0 && z + 1 < nz && own[z - 1] && own[z + 1];
+                f = fractions[z];
+            }
+            bool all = need;
+            if (__ballot(need))
+                for (int d = 0; d < 4; ++d) {  // uniform loop: the neighbour column stays wave-uniform
+                    const int64_t c = d == 0 ? col - ny : d == 1 ? col + ny : d == 2 ? col - 1 : col + 1;
+                    const int sum = nb[d].n <= FE_WAVE ? lane_sum(nb[d], (double)c + f)
+                                                       : column_sum(counts, signs, keys, cap, c, (double)c + f);
+                    all = all && sum != 0;
+                }
+            if (need) sfc = !all;
+            if (z < nz) surface[col * nz + z] = sfc;
+        }
     }
     block_flags(local, flags);
 }
@@ -701,16 +811,17 @@ int mesh_enqueue(MeshField& f, hipStream_t st)
         hipLaunchKernelGGL(k_keys, dim3((unsigned)((n + MT - 1) / MT)), dim3(MT), 0, st, f.counts, f.depths, f.columns,
                            f.cap, f.cells, f.zs, f.nz, f.fractions);
         FE_TRY(hipGetLastError());
-        hipLaunchKernelGGL(k_classify, dim3((unsigned)((f.size + MT * VPT - 1) / (MT * VPT))), dim3(MT), 0, st,
-                           f.counts, f.signs, f.depths, f.fractions, f.cap, f.nz, (uint32_t)f.size, f.mask, flags);
+        const unsigned col_blocks = (unsigned)((f.columns + MT / FE_WAVE - 1) / (MT / FE_WAVE));
+        hipLaunchKernelGGL(k_mask_surface, dim3(col_blocks), dim3(MT), (size_t)(MT / FE_WAVE) * f.nz, st, f.counts,
+                           f.signs, f.depths, f.fractions, f.cap, f.nx, f.ny, f.nz, f.mask, f.surface, flags);
         FE_TRY(hipGetLastError());
     } else {
         hipLaunchKernelGGL(k_gwn, dim3(vox_blocks), dim3(MT), 0, st, f.geo.tris, f.geo.ntri, f.points,
                            (uint32_t)f.size, nullptr, f.mask, flags);
         FE_TRY(hipGetLastError());
+        hipLaunchKernelGGL(k_surface, dim3(vox_blocks), dim3(MT), 0, st, f.mask, f.surface, f.nx, f.ny, f.nz);
+        FE_TRY(hipGetLastError());
     }
-    hipLaunchKernelGGL(k_surface, dim3(vox_blocks), dim3(MT), 0, st, f.mask, f.surface, f.nx, f.ny, f.nz);
-    FE_TRY(hipGetLastError());
     if (fe_edt_device(f.mask, f.nx, f.ny, f.nz, 2, f.work, f.pair, st) != FE_OK) return FE_RUNTIME;
     if (fe_edt_signed_device(f.pair, f.size, f.pitch, f.distance, st) != FE_OK) return FE_RUNTIME;
     FE_TRY(hipMemcpyAsync(f.flags_host, flags, sizeof(int), hipMemcpyDeviceToHost, st));
