@@ -8,8 +8,11 @@
 // from the uint8 mask) and then y and x (strided axes).
 //
 //   z pass  one wave per row: 64-bit ballots of the site flags, nearest site
-//           left/right from clz/ctz, squared 1-D distance.  Both polarities of
-//           a pair come from one read of the mask.
+//           left/right from clz/ctz.  Both polarities of a pair come from one
+//           read of the mask.  Rows of >= 16 voxels write z summaries (per
+//           16-voxel tile: the site bits and the distances to the nearest
+//           sites outside the tile, 8 bytes per 16 voxels and plane) that the
+//           y pass expands in registers, instead of int32 planes.
 //   y,x     a tile of W lines adjacent along z; T threads per line each load
 //           a chunk of positions into registers and build the exact lower
 //           envelope of the chunk's parabolas as a stack of packed 32-bit
@@ -77,25 +80,15 @@ __device__ __forceinline__ EdtVol load_vol(const EdtVol* __restrict__ table, int
 }
 
 // ---------------------------------------------------------------- z pass
-// Four rows (one per wave) per block; sites of plane 0 are voxels whose mask
-// equals site_one (1) or is zero (0); plane 1 (PAIR) takes the complement.
-template <bool PAIR>
-__global__ __launch_bounds__(256) void k_edt_zpass(const uint8_t* __restrict__ masks, int32_t* __restrict__ outs,
-                                                   const EdtVol* __restrict__ table, int nvol, EdtVol one,
-                                                   int site_one)
+// Site flags of one row (nz <= 4096) as 64-bit words, word s held by lane s:
+// plane-0 sites are voxels whose mask equals site_one (1) or is zero (0);
+// `valid` marks the positions < nz.
+__device__ __forceinline__ void row_sites(const uint8_t* __restrict__ m, int nz, int lane, int site_one, uint64_t& w0,
+                                          uint64_t& valid)
 {
-    const int lane = threadIdx.x & 63;
-    const EdtVol V = load_vol(table, nvol, one, 0, blockIdx.x);
-    const int64_t rows = (int64_t)V.nx * V.ny;
-    const int64_t row = (int64_t)(blockIdx.x - V.first[0]) * 4 + (threadIdx.x >> 6);
-    if (row >= rows) return;  // uniform per wave
-    const int nz = V.nz;
-    const int64_t plane = V.plane;
-    const int32_t sentinel = V.sentinel;
-    int32_t* out = outs + V.data;
-    const uint8_t* m = masks + V.mask + row * nz;
     const int nseg = (nz + 63) >> 6;
-    uint64_t w0 = 0, valid = 0;
+    w0 = 0;
+    valid = 0;
     for (int s0 = 0; s0 < nseg; s0 += 8) {  // eight segments' loads in flight before their ballots
         uint8_t by[8];
 #pragma unroll
@@ -114,6 +107,102 @@ __global__ __launch_bounds__(256) void k_edt_zpass(const uint8_t* __restrict__ m
             if (lane == s) { w0 = b; valid = v; }
         }
     }
+}
+
+// 64-bit value of lane `src` (per-lane index; every lane of the wave active).
+__device__ __forceinline__ uint64_t lane_fetch64(uint64_t v, int src)
+{
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_ds_bpermute(src << 2, (int)(uint32_t)v);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_ds_bpermute(src << 2, (int)(uint32_t)(v >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// Z summaries: the z pass in compressed form, read by the y pass in place of
+// full int32 planes.  For every row (x, y) and 16-voxel z tile t of it, each
+// plane gets two words, A = the tile's 16 site bits | dl << 16 and B = dr,
+// where dl = 16 t - (last site before the tile) and dr = (first site after the
+// tile) - (16 t + 15), both ZSUM_NONE where there is no such site.  Plane pl's
+// words sit at element offset data + pl * plane of the int32 arena, ordered
+// (x, t, y) so that a y-pass tile reads them contiguously: A of (x, y, t) at
+// (x * tz + t) * ny + y, B at nx * ny * tz further (tz = ceil(nz / 16);
+// 2 * nx * ny * tz <= nx * ny * nz for nz >= 2).  The squared z distance at
+// z = 16 t + j follows from the in-tile bits, or else from dl / dr; with no
+// site in the row both distances exceed any axis, so the value clamps to the
+// sentinel.  Four rows (one per wave) per block; consecutive blocks stay on
+// one XCD, so the (x, t, y) words of neighbouring rows meet in one L2.
+#define ZSUM_NONE 0x7fff
+template <bool PAIR>
+__global__ __launch_bounds__(256) void k_edt_zsum(const uint8_t* __restrict__ masks, int32_t* __restrict__ outs,
+                                                  const EdtVol* __restrict__ table, int nvol, EdtVol one, int site_one,
+                                                  uint32_t tiles, uint32_t grid)
+{
+    const int lane = threadIdx.x & 63;
+    const uint32_t tile = xcd_tile(blockIdx.x, grid);
+    if (tile >= tiles) return;  // uniform per block
+    const EdtVol V = load_vol(table, nvol, one, 0, tile);
+    const int64_t rows = (int64_t)V.nx * V.ny;
+    const int64_t row = (int64_t)(tile - V.first[0]) * 4 + (threadIdx.x >> 6);
+    if (row >= rows) return;  // uniform per wave
+    const int nz = V.nz;
+    uint64_t w0, valid;
+    row_sites(masks + V.mask + row * nz, nz, lane, site_one, w0, valid);
+    const int tz = (nz + 15) >> 4;
+#pragma unroll
+    for (int pl = 0; pl < (PAIR ? 2 : 1); ++pl) {
+        const uint64_t mw = pl ? ~w0 & valid : w0;
+        const uint64_t segs = __ballot(mw != 0);
+        const int64_t x = row / V.ny, y = row - x * V.ny;
+        uint32_t* A = reinterpret_cast<uint32_t*>(outs + V.data + pl * V.plane) + x * tz * V.ny + y;
+        uint32_t* B = A + rows * tz;
+        for (int t0 = 0; t0 < tz; t0 += 64) {  // uniform loop: every lane takes part in the fetches
+            const int t = t0 + lane;              // < 256, so s < 64
+            const int s = t >> 2, q = t & 3;
+            const uint64_t ws = lane_fetch64(mw, s);
+            const uint32_t bits = (uint32_t)(ws >> (16 * q)) & 0xffffu;
+            // last site before the tile: in this word below the tile, else in an earlier word
+            const uint64_t below = q ? ws & ((1ull << (16 * q)) - 1) : 0ull;
+            const uint64_t prev = segs & ((1ull << s) - 1);
+            const int jp = prev ? 63 - __builtin_clzll(prev) : 0;
+            const uint64_t wp = lane_fetch64(mw, jp);
+            int dl = ZSUM_NONE;
+            if (below) dl = 16 * t - ((s << 6) + 63 - __builtin_clzll(below));
+            else if (prev) dl = 16 * t - ((jp << 6) + 63 - __builtin_clzll(wp));
+            // first site after the tile: in this word above the tile, else in a later word
+            const uint64_t above = q < 3 ? ws >> (16 * (q + 1)) : 0ull;
+            const uint64_t next = s == 63 ? 0ull : segs >> (s + 1);
+            const int jn = next ? s + 1 + __builtin_ctzll(next) : 0;
+            const uint64_t wn = lane_fetch64(mw, jn);
+            int dr = ZSUM_NONE;
+            if (above) dr = (s << 6) + 16 * (q + 1) + __builtin_ctzll(above) - (16 * t + 15);
+            else if (next) dr = (jn << 6) + __builtin_ctzll(wn) - (16 * t + 15);
+            if (t < tz) {
+                A[(int64_t)t * V.ny] = bits | ((uint32_t)dl << 16);
+                B[(int64_t)t * V.ny] = (uint32_t)dr;
+            }
+        }
+    }
+}
+
+// The z pass with full int32 planes (rows shorter than 16 voxels, and the
+// single-axis profiling entry); four rows (one per wave) per block; plane 1
+// (PAIR) takes the complement of plane 0's sites.
+template <bool PAIR>
+__global__ __launch_bounds__(256) void k_edt_zpass(const uint8_t* __restrict__ masks, int32_t* __restrict__ outs,
+                                                   const EdtVol* __restrict__ table, int nvol, EdtVol one,
+                                                   int site_one)
+{
+    const int lane = threadIdx.x & 63;
+    const EdtVol V = load_vol(table, nvol, one, 0, blockIdx.x);
+    const int64_t rows = (int64_t)V.nx * V.ny;
+    const int64_t row = (int64_t)(blockIdx.x - V.first[0]) * 4 + (threadIdx.x >> 6);
+    if (row >= rows) return;  // uniform per wave
+    const int nz = V.nz;
+    const int64_t plane = V.plane;
+    const int32_t sentinel = V.sentinel;
+    int32_t* out = outs + V.data;
+    const int nseg = (nz + 63) >> 6;
+    uint64_t w0, valid;
+    row_sites(masks + V.mask + row * nz, nz, lane, site_one, w0, valid);
     const uint64_t w1 = ~w0 & valid;
     const uint64_t seg0 = __ballot(w0 != 0), seg1 = __ballot(w1 != 0);
     int32_t* o0 = out + row * nz;
@@ -200,8 +289,9 @@ __device__ __forceinline__ bool keep_mid(int sa, int32_t Fa, int sb, int32_t Fb,
 // contributes sentinel + (p-s)^2 >= sentinel, with equality only at its own
 // position, so the result is min(envelope of the other sites, sentinel).
 // LDS: kmax*W*T packed entries (kmax = the largest chunk length of the launch)
-// plus four bytes per thread and twelve bytes per line.
-template <int W, int T, int CMAX, bool COMPACT>
+// plus four bytes per thread and twelve bytes per line.  ZSUM: the y pass
+// reading z summaries (k_edt_zsum) instead of int32 planes.
+template <int W, int T, int CMAX, bool COMPACT, bool ZSUM>
 __global__ __launch_bounds__(W* T) void k_edt_lines(const int32_t* __restrict__ src, int32_t* __restrict__ dst,
                                                     const EdtVol* __restrict__ table, int nvol, EdtVol one, int axis,
                                                     uint32_t tiles, uint32_t grid, int region)
@@ -257,7 +347,46 @@ __global__ __launch_bounds__(W* T) void k_edt_lines(const int32_t* __restrict__ 
     const int64_t lbase = base + line;
 
     int32_t v[CMAX];
-    if (active) {
+    if constexpr (ZSUM) {
+        // y pass over z summaries: rebuild the squared z distance of (x = outer,
+        // y = p, z) from the two words of its 16-voxel z tile
+        if (active) {
+            const int z = zt * W + line, j = z & 15;
+            const int tz = (nz + 15) >> 4;
+            const uint32_t* pa = reinterpret_cast<const uint32_t*>(src) + V.data + (int64_t)plane * V.plane +
+                                 ((int64_t)outer * tz + (z >> 4)) * L + p0;
+            const int64_t bofs = (int64_t)V.nx * V.ny * tz;
+            const uint32_t lowm = (2u << j) - 1u;
+            auto expand = [&](uint32_t a, uint32_t b) {
+                const uint32_t bits = a & 0xffffu;
+                const uint32_t lo = bits & lowm, hi = bits >> j;
+                const int left = lo ? j - 31 + __builtin_clz(lo) : j + (int)(a >> 16);
+                const int right = hi ? __builtin_ctz(hi) : 15 - j + (int)b;
+                const int d = min(left, right);
+                return min((int32_t)__umul24((uint32_t)d, (uint32_t)d), sentinel);
+            };
+            // 16-byte loads when both word runs start 16-byte aligned and the
+            // chunk length is a multiple of 4 (CMAX is one in every tile shape)
+            static_assert(CMAX % 4 == 0, "tile shapes hold whole 4-word groups");
+            if (((reinterpret_cast<uintptr_t>(pa) & 15) | ((bofs | (p1 - p0)) & 3)) == 0) {
+#pragma unroll
+                for (int i = 0; i < CMAX; i += 4) {
+                    if (p0 + i < p1) {
+                        const uint4 a = *reinterpret_cast<const uint4*>(pa + i);
+                        const uint4 b = *reinterpret_cast<const uint4*>(pa + bofs + i);
+                        v[i] = expand(a.x, b.x);
+                        v[i + 1] = expand(a.y, b.y);
+                        v[i + 2] = expand(a.z, b.z);
+                        v[i + 3] = expand(a.w, b.w);
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int i = 0; i < CMAX; ++i)
+                    if (p0 + i < p1) v[i] = expand(pa[i], pa[bofs + i]);
+            }
+        }
+    } else if (active) {
         const int32_t* ps = src + lbase + (int64_t)p0 * axis_stride;
 #pragma unroll
         for (int i = 0; i < CMAX; ++i) {
@@ -594,7 +723,7 @@ LineCfg pick_cfg(int L)
     return {8, 64, 64};
 }
 
-template <int W, int T, int CMAX, bool COMPACT>
+template <int W, int T, int CMAX, bool COMPACT, bool ZSUM>
 hipError_t launch_lines_t(const int32_t* src, int32_t* dst, const EdtVol* table, int nvol, const EdtVol& one,
                           int axis, int Lmax, uint64_t tiles, hipStream_t st)
 {
@@ -603,37 +732,41 @@ hipError_t launch_lines_t(const int32_t* src, int32_t* dst, const EdtVol* table,
     // hull stacks; with COMPACT the same words later hold Lmax rows of W start entries
     const int region = kmax * W * T;
     const size_t lds = (size_t)region * 4 + (size_t)W * T * 4 + (size_t)W * 12;
-    const void* fn = reinterpret_cast<const void*>(&k_edt_lines<W, T, CMAX, COMPACT>);
+    const void* fn = reinterpret_cast<const void*>(&k_edt_lines<W, T, CMAX, COMPACT, ZSUM>);
     if (lds > 65536) {
         hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL((k_edt_lines<W, T, CMAX, COMPACT>), dim3(grid), dim3(W * T), lds, st, src, dst, table, nvol,
-                       one, axis, (uint32_t)tiles, grid, region);
+    hipLaunchKernelGGL((k_edt_lines<W, T, CMAX, COMPACT, ZSUM>), dim3(grid), dim3(W * T), lds, st, src, dst, table,
+                       nvol, one, axis, (uint32_t)tiles, grid, region);
     return hipGetLastError();
 }
 
 // One line pass over every volume: Lmax (the longest line of the pass over
 // all volumes) picks the tile shape, `tiles` counts the tiles of that shape.
-// compact: every axis of every volume is <= 1024 (packed 32-bit envelope).
+// compact: every axis of every volume is <= 1024 (packed 32-bit envelope);
+// zsum (y pass only): src holds z summaries.
 hipError_t launch_lines(const int32_t* src, int32_t* dst, const EdtVol* table, int nvol, const EdtVol& one, int axis,
-                        int Lmax, uint64_t tiles, bool compact, hipStream_t st)
+                        int Lmax, uint64_t tiles, bool compact, bool zsum, hipStream_t st)
 {
     const LineCfg c = pick_cfg(Lmax);
+#define FE_LINES_Z(w, t, cm, cp)                                                                             \
+    return zsum ? launch_lines_t<w, t, cm, cp, true>(src, dst, table, nvol, one, axis, Lmax, tiles, st)     \
+                : launch_lines_t<w, t, cm, cp, false>(src, dst, table, nvol, one, axis, Lmax, tiles, st);
 #define FE_LINES(w, t, cm)                                                                                   \
-    if (c.W == w && c.T == t && c.CMAX == cm)                                                                \
-        return compact ? launch_lines_t<w, t, cm, true>(src, dst, table, nvol, one, axis, Lmax, tiles, st)   \
-                       : launch_lines_t<w, t, cm, false>(src, dst, table, nvol, one, axis, Lmax, tiles, st);
+    if (c.W == w && c.T == t && c.CMAX == cm) {                                                              \
+        if (compact) { FE_LINES_Z(w, t, cm, true) }                                                          \
+        FE_LINES_Z(w, t, cm, false)                                                                          \
+    }
     FE_LINES(32, 8, 16)
     FE_LINES(32, 16, 16)
     FE_LINES(16, 32, 16)
     FE_LINES(16, 32, 24)
     FE_LINES(16, 32, 32)
 #undef FE_LINES
-    if (c.W == 8 && c.T == 64 && c.CMAX == 32)
-        return launch_lines_t<8, 64, 32, false>(src, dst, table, nvol, one, axis, Lmax, tiles, st);
-    if (c.W == 8 && c.T == 64 && c.CMAX == 64)
-        return launch_lines_t<8, 64, 64, false>(src, dst, table, nvol, one, axis, Lmax, tiles, st);
+    if (c.W == 8 && c.T == 64 && c.CMAX == 32) { FE_LINES_Z(8, 64, 32, false) }
+    if (c.W == 8 && c.T == 64 && c.CMAX == 64) { FE_LINES_Z(8, 64, 64, false) }
+#undef FE_LINES_Z
     return hipErrorInvalidValue;
 }
 
@@ -650,24 +783,45 @@ void vol_tiles(int nx, int ny, int nz, int planes, int Wy, int Wx, uint64_t t[3]
     t[2] = (uint64_t)((nz + Wx - 1) / Wx) * ny * planes;
 }
 
-// The three passes over a table (nvol > 0, device table) or one volume (nvol == 0).
-// z pass into `out`, y pass into `work`, x pass back into `out`.  planes == 1
-// sites: mask == 0; planes == 2, plane 0 sites: mask != 0.
-hipError_t run_passes(const uint8_t* masks, int32_t* work, int32_t* out, const EdtVol* table, int nvol,
-                      const EdtVol& one, int planes, int Lmax_y, int Lmax_x, const uint64_t tiles[3], bool compact,
-                      hipStream_t st)
+// Rows of at least ZSUM_MIN_NZ voxels take the z-summary path (the summaries
+// of a plane then fit in its nx*ny*nz words).
+#define ZSUM_MIN_NZ 16
+
+// z pass (full planes, or summaries when zsum) into `out`, for a table
+// (nvol > 0, device table) or one volume (nvol == 0).
+void launch_zpass(const uint8_t* masks, int32_t* out, const EdtVol* table, int nvol, const EdtVol& one, int planes,
+                  uint64_t tiles, bool zsum, hipStream_t st)
 {
     const int site_one = planes == 2 ? 1 : 0;
-    if (planes == 2)
-        hipLaunchKernelGGL(k_edt_zpass<true>, dim3((uint32_t)tiles[0]), dim3(256), 0, st, masks, out, table, nvol, one,
-                           site_one);
-    else
-        hipLaunchKernelGGL(k_edt_zpass<false>, dim3((uint32_t)tiles[0]), dim3(256), 0, st, masks, out, table, nvol,
-                           one, site_one);
+    const dim3 grid((uint32_t)tiles), block(256);
+    if (zsum) {
+        const uint32_t g8 = round8(tiles);
+        if (planes == 2)
+            hipLaunchKernelGGL(k_edt_zsum<true>, dim3(g8), block, 0, st, masks, out, table, nvol, one, site_one,
+                               (uint32_t)tiles, g8);
+        else
+            hipLaunchKernelGGL(k_edt_zsum<false>, dim3(g8), block, 0, st, masks, out, table, nvol, one, site_one,
+                               (uint32_t)tiles, g8);
+    } else {
+        if (planes == 2) hipLaunchKernelGGL(k_edt_zpass<true>, grid, block, 0, st, masks, out, table, nvol, one, site_one);
+        else hipLaunchKernelGGL(k_edt_zpass<false>, grid, block, 0, st, masks, out, table, nvol, one, site_one);
+    }
+}
+
+// The three passes over a table (nvol > 0, device table) or one volume (nvol == 0).
+// z pass into `out`, y pass into `work`, x pass back into `out`.  planes == 1
+// sites: mask == 0; planes == 2, plane 0 sites: mask != 0.  zsum: every
+// volume has nz >= ZSUM_MIN_NZ.
+hipError_t run_passes(const uint8_t* masks, int32_t* work, int32_t* out, const EdtVol* table, int nvol,
+                      const EdtVol& one, int planes, int Lmax_y, int Lmax_x, const uint64_t tiles[3], bool compact,
+                      bool zsum, hipStream_t st)
+{
+    launch_zpass(masks, out, table, nvol, one, planes, tiles[0], zsum, st);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    if ((e = launch_lines(out, work, table, nvol, one, 1, Lmax_y, tiles[1], compact, st)) != hipSuccess) return e;
-    return launch_lines(work, out, table, nvol, one, 0, Lmax_x, tiles[2], compact, st);
+    if ((e = launch_lines(out, work, table, nvol, one, 1, Lmax_y, tiles[1], compact, zsum, st)) != hipSuccess)
+        return e;
+    return launch_lines(work, out, table, nvol, one, 0, Lmax_x, tiles[2], compact, false, st);
 }
 
 EdtVol single_vol(int nx, int ny, int nz, int64_t plane)
@@ -705,13 +859,16 @@ extern "C" int fe_edt_device(const uint8_t* mask, int nx, int ny, int nz, int pl
     uint64_t tiles[3];
     vol_tiles(nx, ny, nz, planes, pick_cfg(ny).W, pick_cfg(nx).W, tiles);
     const bool compact = nx <= 1024 && ny <= 1024 && nz <= 1024;
-    FE_TRY(run_passes(mask, work, out, nullptr, 0, one, planes, ny, nx, tiles, compact,
+    FE_TRY(run_passes(mask, work, out, nullptr, 0, one, planes, ny, nx, tiles, compact, nz >= ZSUM_MIN_NZ,
                       static_cast<hipStream_t>(stream)));
     return FE_OK;
 }
 
-// One axis pass of fe_edt_device, for profiling and per-kernel timing:
-// axis 2 reads the uint8 mask `src` (z pass), axes 1 and 0 read int32 planes.
+// One axis pass of fe_edt_device, for profiling and per-kernel timing, with
+// the intermediate layouts fe_edt_device uses: axis 2 reads the uint8 mask
+// `src` and writes the z pass (z summaries when nz >= 16, else int32 planes),
+// axis 1 reads that and writes int32 planes, axis 0 reads and writes int32
+// planes.
 extern "C" int fe_edt_axis_pass(const void* src, int32_t* dst, int nx, int ny, int nz, int planes, int axis,
                                 void* stream) noexcept
 {
@@ -722,19 +879,16 @@ extern "C" int fe_edt_axis_pass(const void* src, int32_t* dst, int nx, int ny, i
     const EdtVol one = single_vol(nx, ny, nz, (int64_t)nx * ny * nz);
     uint64_t tiles[3];
     vol_tiles(nx, ny, nz, planes, pick_cfg(ny).W, pick_cfg(nx).W, tiles);
+    const bool zsum = nz >= ZSUM_MIN_NZ;
     if (axis == 2) {
-        const uint8_t* m = static_cast<const uint8_t*>(src);
-        if (planes == 2)
-            hipLaunchKernelGGL(k_edt_zpass<true>, dim3((uint32_t)tiles[0]), dim3(256), 0, st, m, dst, nullptr, 0, one, 1);
-        else
-            hipLaunchKernelGGL(k_edt_zpass<false>, dim3((uint32_t)tiles[0]), dim3(256), 0, st, m, dst, nullptr, 0, one, 0);
+        launch_zpass(static_cast<const uint8_t*>(src), dst, nullptr, 0, one, planes, tiles[0], zsum, st);
         FE_TRY(hipGetLastError());
         return FE_OK;
     }
     const int32_t* a = static_cast<const int32_t*>(src);
     const bool compact = nx <= 1024 && ny <= 1024 && nz <= 1024;
-    if (axis == 1) FE_TRY(launch_lines(a, dst, nullptr, 0, one, 1, ny, tiles[1], compact, st));
-    else FE_TRY(launch_lines(a, dst, nullptr, 0, one, 0, nx, tiles[2], compact, st));
+    if (axis == 1) FE_TRY(launch_lines(a, dst, nullptr, 0, one, 1, ny, tiles[1], compact, zsum, st));
+    else FE_TRY(launch_lines(a, dst, nullptr, 0, one, 0, nx, tiles[2], compact, false, st));
     return FE_OK;
 }
 
@@ -744,7 +898,7 @@ struct EdtBatch {
     EdtVol* table = nullptr;  // device
     int nvol = 0, planes = 0, Lmax_y = 0, Lmax_x = 0;
     uint64_t tiles[3] = {0, 0, 0};
-    bool compact = true;
+    bool compact = true, zsum = true;
 };
 }  // namespace
 
@@ -769,7 +923,7 @@ extern "C" void* fe_edt_batch_create(const int32_t* dims, const int64_t* mask_of
     try {
         std::vector<EdtVol> host((size_t)nvol);
         std::vector<std::pair<int64_t, int64_t>> ranges((size_t)nvol);
-        int Ly = 1, Lx = 1, Lz = 1;
+        int Ly = 1, Lx = 1, Lz = 1, zmin = FE_EDT_MAX_AXIS;
         for (int v = 0; v < nvol; ++v) {
             const int nx = dims[3 * v], ny = dims[3 * v + 1], nz = dims[3 * v + 2];
             if (nx < 1 || ny < 1 || nz < 1 || nx > FE_EDT_MAX_AXIS || ny > FE_EDT_MAX_AXIS || nz > FE_EDT_MAX_AXIS)
@@ -785,6 +939,7 @@ extern "C" void* fe_edt_batch_create(const int32_t* dims, const int64_t* mask_of
             Ly = std::max(Ly, ny);
             Lx = std::max(Lx, nx);
             Lz = std::max(Lz, nz);
+            zmin = std::min(zmin, nz);
         }
         std::sort(ranges.begin(), ranges.end());
         for (size_t i = 1; i < ranges.size(); ++i)
@@ -808,6 +963,7 @@ extern "C" void* fe_edt_batch_create(const int32_t* dims, const int64_t* mask_of
         b->Lmax_y = Ly;
         b->Lmax_x = Lx;
         b->compact = Lx <= 1024 && Ly <= 1024 && Lz <= 1024;
+        b->zsum = zmin >= ZSUM_MIN_NZ;
         for (int p = 0; p < 3; ++p) b->tiles[p] = tot[p];
         hipError_t e = hipMalloc(&b->table, sizeof(EdtVol) * (size_t)nvol);
         if (e == hipSuccess) e = hipMemcpy(b->table, host.data(), sizeof(EdtVol) * (size_t)nvol, hipMemcpyHostToDevice);
@@ -838,7 +994,7 @@ extern "C" int fe_edt_batch_run(const void* handle, const uint8_t* masks, int32_
     if (!b || !masks || !work || !out) return FE_INVALID;
     const EdtVol none{};
     FE_TRY(run_passes(masks, work, out, b->table, b->nvol, none, b->planes, b->Lmax_y, b->Lmax_x, b->tiles,
-                      b->compact, static_cast<hipStream_t>(stream)));
+                      b->compact, b->zsum, static_cast<hipStream_t>(stream)));
     return FE_OK;
 }
 

@@ -23,13 +23,23 @@ def volumes_golden():
     return json.loads((ROOT / "tests/golden/edt_volumes.json").read_text())
 
 
-def ragged_masks():
+RAGGED = {
+    # rows shorter than 16 voxels in the batch: int32 z planes
+    "short_rows": [(1, 5, 9), (33, 1, 17), (40, 33, 27), (64, 64, 64), (17, 130, 5), (7, 9, 1), (3, 600, 8),
+                   (520, 3, 12), (24, 24, 24), (24, 24, 24)],
+    # every row >= 16 voxels: the y pass reads z summaries (rows of 16, 17, 31
+    # and 1100 voxels, tiles of 16 cut short at the row end)
+    "zsum_rows": [(1, 5, 16), (33, 1, 17), (40, 33, 27), (64, 64, 64), (17, 130, 31), (3, 600, 20), (520, 3, 48),
+                  (5, 4, 1100), (24, 24, 24), (24, 24, 24)],
+}
+
+
+def ragged_masks(kind="short_rows"):
     """Small volumes: 1-voxel axes, nz not a multiple of 4, an empty and a full
     volume, lines longer than 128 and 512 (other tile shapes in one batch)."""
     rng = np.random.default_rng(41)
     out = []
-    for shape in [(1, 5, 9), (33, 1, 17), (40, 33, 27), (64, 64, 64), (17, 130, 5), (7, 9, 1), (3, 600, 8),
-                  (520, 3, 12), (24, 24, 24), (24, 24, 24)]:
+    for shape in RAGGED[kind]:
         if len(out) == 8:
             m = np.zeros(shape, np.uint8)
         elif len(out) == 9:
@@ -43,10 +53,11 @@ def ragged_masks():
     return out
 
 
+@pytest.mark.parametrize("kind", sorted(RAGGED))
 @pytest.mark.parametrize("planes", [2, 1])
-def test_batch_ragged_vs_oracle(fe, gpu, planes):
-    masks = ragged_masks()
-    print("ragged batch", planes, [m.shape for m in masks], flush=True)
+def test_batch_ragged_vs_oracle(fe, gpu, planes, kind):
+    masks = ragged_masks(kind)
+    print("ragged batch", planes, kind, [m.shape for m in masks], flush=True)
     b = fe.EDTBatch([m.shape for m in masks], planes=planes)
     b.set_masks(masks)
     b.run()

@@ -52,7 +52,7 @@ def test_pair_cases_persistent(fe, golden, gpu):
 
 def random_masks(rng):
     for shape in ((1, 1, 2), (3, 70, 5), (130, 9, 11), (17, 300, 3), (64, 64, 64), (5, 6, 1100), (1100, 3, 4),
-                  (2, 1030, 7)):
+                  (2, 1030, 7), (4, 5, 16), (20, 30, 17), (9, 8, 31), (3, 2, 4096)):
         yield shape, rng.random(shape) < rng.uniform(0.01, 0.9)
     m = np.zeros((90, 80, 70), bool)
     m[45, 40, 35] = True
