@@ -128,57 +128,102 @@ __device__ __forceinline__ uint64_t lane_fetch64(uint64_t v, int src)
 // 2 * nx * ny * tz <= nx * ny * nz for nz >= 2).  The squared z distance at
 // z = 16 t + j follows from the in-tile bits, or else from dl / dr; with no
 // site in the row both distances exceed any axis, so the value clamps to the
-// sentinel.  Four rows (one per wave) per block; consecutive blocks stay on
-// one XCD, so the (x, t, y) words of neighbouring rows meet in one L2.
+// sentinel.
+//
+// One lane per row, 64 consecutive rows per one-wave block, so that every
+// word store covers 64 consecutive y.  The rows are staged through LDS 64
+// bytes at a time with coalesced loads (4-byte words when the rows start
+// 4-byte aligned, bytes otherwise); A is written in ascending tile order with
+// the last site carried forward, then B in descending order from the site
+// bits read back from the lane's own A words.
 #define ZSUM_NONE 0x7fff
-template <bool PAIR>
-__global__ __launch_bounds__(256) void k_edt_zsum(const uint8_t* __restrict__ masks, int32_t* __restrict__ outs,
-                                                  const EdtVol* __restrict__ table, int nvol, EdtVol one, int site_one,
-                                                  uint32_t tiles, uint32_t grid)
+#define ZSUM_ROWS 64
+// Rows of at least ZSUM_MIN_NZ voxels take the z-summary path (the summaries
+// of a plane then fit in its nx*ny*nz words).
+#define ZSUM_MIN_NZ 16
+#define ZSUM_STRIDE 80  // LDS bytes per staged row: 16-byte aligned, conflict-free 16-byte reads
+
+// Bit k (k < 4) set iff byte k of w is nonzero.
+__device__ __forceinline__ uint32_t nonzero_nibble(uint32_t w)
 {
-    const int lane = threadIdx.x & 63;
+    const uint32_t h = (((w & 0x7f7f7f7fu) + 0x7f7f7f7fu) | w) & 0x80808080u;
+    return (h * 0x00204081u) >> 28;  // bits 7, 15, 23, 31 land on 28..31 without carries
+}
+
+template <bool PAIR>
+__global__ __launch_bounds__(ZSUM_ROWS) void k_edt_zsum(const uint8_t* __restrict__ masks, int32_t* __restrict__ outs,
+                                                        const EdtVol* __restrict__ table, int nvol, EdtVol one,
+                                                        int site_one, uint32_t tiles, uint32_t grid)
+{
+    __shared__ __align__(16) uint8_t stage[ZSUM_ROWS * ZSUM_STRIDE];
+    const int lane = threadIdx.x;
     const uint32_t tile = xcd_tile(blockIdx.x, grid);
     if (tile >= tiles) return;  // uniform per block
     const EdtVol V = load_vol(table, nvol, one, 0, tile);
     const int64_t rows = (int64_t)V.nx * V.ny;
-    const int64_t row = (int64_t)(tile - V.first[0]) * 4 + (threadIdx.x >> 6);
-    if (row >= rows) return;  // uniform per wave
-    const int nz = V.nz;
-    uint64_t w0, valid;
-    row_sites(masks + V.mask + row * nz, nz, lane, site_one, w0, valid);
-    const int tz = (nz + 15) >> 4;
-#pragma unroll
-    for (int pl = 0; pl < (PAIR ? 2 : 1); ++pl) {
-        const uint64_t mw = pl ? ~w0 & valid : w0;
-        const uint64_t segs = __ballot(mw != 0);
-        const int64_t x = row / V.ny, y = row - x * V.ny;
-        uint32_t* A = reinterpret_cast<uint32_t*>(outs + V.data + pl * V.plane) + x * tz * V.ny + y;
-        uint32_t* B = A + rows * tz;
-        for (int t0 = 0; t0 < tz; t0 += 64) {  // uniform loop: every lane takes part in the fetches
-            const int t = t0 + lane;              // < 256, so s < 64
-            const int s = t >> 2, q = t & 3;
-            const uint64_t ws = lane_fetch64(mw, s);
-            const uint32_t bits = (uint32_t)(ws >> (16 * q)) & 0xffffu;
-            // last site before the tile: in this word below the tile, else in an earlier word
-            const uint64_t below = q ? ws & ((1ull << (16 * q)) - 1) : 0ull;
-            const uint64_t prev = segs & ((1ull << s) - 1);
-            const int jp = prev ? 63 - __builtin_clzll(prev) : 0;
-            const uint64_t wp = lane_fetch64(mw, jp);
-            int dl = ZSUM_NONE;
-            if (below) dl = 16 * t - ((s << 6) + 63 - __builtin_clzll(below));
-            else if (prev) dl = 16 * t - ((jp << 6) + 63 - __builtin_clzll(wp));
-            // first site after the tile: in this word above the tile, else in a later word
-            const uint64_t above = q < 3 ? ws >> (16 * (q + 1)) : 0ull;
-            const uint64_t next = s == 63 ? 0ull : segs >> (s + 1);
-            const int jn = next ? s + 1 + __builtin_ctzll(next) : 0;
-            const uint64_t wn = lane_fetch64(mw, jn);
-            int dr = ZSUM_NONE;
-            if (above) dr = (s << 6) + 16 * (q + 1) + __builtin_ctzll(above) - (16 * t + 15);
-            else if (next) dr = (jn << 6) + __builtin_ctzll(wn) - (16 * t + 15);
-            if (t < tz) {
-                A[(int64_t)t * V.ny] = bits | ((uint32_t)dl << 16);
-                B[(int64_t)t * V.ny] = (uint32_t)dr;
+    const int64_t row0 = (int64_t)(tile - V.first[0]) * ZSUM_ROWS;
+    const int nrows = (int)min((int64_t)ZSUM_ROWS, rows - row0);
+    const int nz = V.nz, tz = (nz + 15) >> 4;
+    const uint8_t* m0 = masks + V.mask + row0 * nz;
+    const bool row_ok = lane < nrows;
+    const int64_t row = row0 + lane;
+    const int64_t x = row / V.ny, y = row - x * V.ny;
+    const int64_t ys = V.ny;  // words from one tile of a row to the next
+    uint32_t* A0 = reinterpret_cast<uint32_t*>(outs + V.data) + x * tz * ys + y;
+    uint32_t* A1 = reinterpret_cast<uint32_t*>(outs + V.data + V.plane) + x * tz * ys + y;
+    const int64_t bofs = rows * tz;
+    const bool words = ((reinterpret_cast<uintptr_t>(masks + V.mask) | (uintptr_t)nz) & 3) == 0;
+    int last0 = -1, last1 = -1;  // last site so far
+    for (int c0 = 0; c0 < nz; c0 += 64) {
+        if (words) {
+            const int d = lane & 15;
+            for (int r = lane >> 4; r < nrows; r += 4) {
+                uint32_t w = 0;
+                if (c0 + 4 * d < nz) w = *reinterpret_cast<const uint32_t*>(m0 + (int64_t)r * nz + c0 + 4 * d);
+                *reinterpret_cast<uint32_t*>(stage + r * ZSUM_STRIDE + 4 * d) = w;
             }
+        } else {
+            for (int r = 0; r < nrows; ++r)
+                stage[r * ZSUM_STRIDE + lane] = c0 + lane < nz ? m0[(int64_t)r * nz + c0 + lane] : (uint8_t)0;
+        }
+        __syncthreads();
+        if (row_ok) {
+            uint4 q[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) q[k] = *reinterpret_cast<const uint4*>(stage + lane * ZSUM_STRIDE + 16 * k);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int t = (c0 >> 4) + k;
+                if (t < tz) {
+                    const uint32_t nzb = nonzero_nibble(q[k].x) | nonzero_nibble(q[k].y) << 4 |
+                                         nonzero_nibble(q[k].z) << 8 | nonzero_nibble(q[k].w) << 12;
+                    const int rem = nz - 16 * t;
+                    const uint32_t valid = rem >= 16 ? 0xffffu : (1u << rem) - 1u;
+                    const uint32_t b0 = (site_one ? nzb : ~nzb) & valid;
+                    A0[t * ys] = b0 | (uint32_t)(last0 < 0 ? ZSUM_NONE : 16 * t - last0) << 16;
+                    if (b0) last0 = 16 * t + 31 - __builtin_clz(b0);
+                    if constexpr (PAIR) {
+                        const uint32_t b1 = ~b0 & valid;
+                        A1[t * ys] = b1 | (uint32_t)(last1 < 0 ? ZSUM_NONE : 16 * t - last1) << 16;
+                        if (b1) last1 = 16 * t + 31 - __builtin_clz(b1);
+                    }
+                }
+            }
+        }
+        __syncthreads();  // the stage is reused
+    }
+    if (!row_ok) return;  // no barrier below
+    int next0 = -1, next1 = -1;  // first site after the tile
+    for (int t = tz - 1; t >= 0; --t) {
+        const int rem = nz - 16 * t;
+        const uint32_t valid = rem >= 16 ? 0xffffu : (1u << rem) - 1u;
+        const uint32_t b0 = A0[t * ys] & 0xffffu;
+        A0[bofs + t * ys] = (uint32_t)(next0 < 0 ? ZSUM_NONE : next0 - (16 * t + 15));
+        if (b0) next0 = 16 * t + __builtin_ctz(b0);
+        if constexpr (PAIR) {
+            const uint32_t b1 = ~b0 & valid;
+            A1[bofs + t * ys] = (uint32_t)(next1 < 0 ? ZSUM_NONE : next1 - (16 * t + 15));
+            if (b1) next1 = 16 * t + __builtin_ctz(b1);
         }
     }
 }
@@ -775,17 +820,15 @@ int64_t edt_sentinel(int nx, int ny, int nz)
     return (int64_t)(nx - 1) * (nx - 1) + (int64_t)(ny - 1) * (ny - 1) + (int64_t)(nz - 1) * (nz - 1) + 1;
 }
 
-// Tile counts of one volume in the three passes for line tiles of Wy and Wx lines.
-void vol_tiles(int nx, int ny, int nz, int planes, int Wy, int Wx, uint64_t t[3])
+// Tile counts of one volume in the three passes for line tiles of Wy and Wx
+// lines (z pass: blocks of ZSUM_ROWS rows for summaries, else of 4 rows).
+void vol_tiles(int nx, int ny, int nz, int planes, int Wy, int Wx, bool zsum, uint64_t t[3])
 {
-    t[0] = ((uint64_t)nx * ny + 3) / 4;
+    const uint64_t per = zsum ? ZSUM_ROWS : 4;
+    t[0] = ((uint64_t)nx * ny + per - 1) / per;
     t[1] = (uint64_t)((nz + Wy - 1) / Wy) * nx * planes;
     t[2] = (uint64_t)((nz + Wx - 1) / Wx) * ny * planes;
 }
-
-// Rows of at least ZSUM_MIN_NZ voxels take the z-summary path (the summaries
-// of a plane then fit in its nx*ny*nz words).
-#define ZSUM_MIN_NZ 16
 
 // z pass (full planes, or summaries when zsum) into `out`, for a table
 // (nvol > 0, device table) or one volume (nvol == 0).
@@ -797,11 +840,11 @@ void launch_zpass(const uint8_t* masks, int32_t* out, const EdtVol* table, int n
     if (zsum) {
         const uint32_t g8 = round8(tiles);
         if (planes == 2)
-            hipLaunchKernelGGL(k_edt_zsum<true>, dim3(g8), block, 0, st, masks, out, table, nvol, one, site_one,
-                               (uint32_t)tiles, g8);
+            hipLaunchKernelGGL(k_edt_zsum<true>, dim3(g8), dim3(ZSUM_ROWS), 0, st, masks, out, table, nvol, one,
+                               site_one, (uint32_t)tiles, g8);
         else
-            hipLaunchKernelGGL(k_edt_zsum<false>, dim3(g8), block, 0, st, masks, out, table, nvol, one, site_one,
-                               (uint32_t)tiles, g8);
+            hipLaunchKernelGGL(k_edt_zsum<false>, dim3(g8), dim3(ZSUM_ROWS), 0, st, masks, out, table, nvol, one,
+                               site_one, (uint32_t)tiles, g8);
     } else {
         if (planes == 2) hipLaunchKernelGGL(k_edt_zpass<true>, grid, block, 0, st, masks, out, table, nvol, one, site_one);
         else hipLaunchKernelGGL(k_edt_zpass<false>, grid, block, 0, st, masks, out, table, nvol, one, site_one);
@@ -857,9 +900,10 @@ extern "C" int fe_edt_device(const uint8_t* mask, int nx, int ny, int nz, int pl
         return FE_INVALID;
     const EdtVol one = single_vol(nx, ny, nz, (int64_t)nx * ny * nz);
     uint64_t tiles[3];
-    vol_tiles(nx, ny, nz, planes, pick_cfg(ny).W, pick_cfg(nx).W, tiles);
+    const bool zsum = nz >= ZSUM_MIN_NZ;
+    vol_tiles(nx, ny, nz, planes, pick_cfg(ny).W, pick_cfg(nx).W, zsum, tiles);
     const bool compact = nx <= 1024 && ny <= 1024 && nz <= 1024;
-    FE_TRY(run_passes(mask, work, out, nullptr, 0, one, planes, ny, nx, tiles, compact, nz >= ZSUM_MIN_NZ,
+    FE_TRY(run_passes(mask, work, out, nullptr, 0, one, planes, ny, nx, tiles, compact, zsum,
                       static_cast<hipStream_t>(stream)));
     return FE_OK;
 }
@@ -878,8 +922,8 @@ extern "C" int fe_edt_axis_pass(const void* src, int32_t* dst, int nx, int ny, i
     hipStream_t st = static_cast<hipStream_t>(stream);
     const EdtVol one = single_vol(nx, ny, nz, (int64_t)nx * ny * nz);
     uint64_t tiles[3];
-    vol_tiles(nx, ny, nz, planes, pick_cfg(ny).W, pick_cfg(nx).W, tiles);
     const bool zsum = nz >= ZSUM_MIN_NZ;
+    vol_tiles(nx, ny, nz, planes, pick_cfg(ny).W, pick_cfg(nx).W, zsum, tiles);
     if (axis == 2) {
         launch_zpass(static_cast<const uint8_t*>(src), dst, nullptr, 0, one, planes, tiles[0], zsum, st);
         FE_TRY(hipGetLastError());
@@ -945,10 +989,11 @@ extern "C" void* fe_edt_batch_create(const int32_t* dims, const int64_t* mask_of
         for (size_t i = 1; i < ranges.size(); ++i)
             if (ranges[i].first < ranges[i - 1].second) return nullptr;
         const int Wy = pick_cfg(Ly).W, Wx = pick_cfg(Lx).W;
+        const bool zsum = zmin >= ZSUM_MIN_NZ;
         uint64_t tot[3] = {0, 0, 0};
         for (int v = 0; v < nvol; ++v) {
             uint64_t t[3];
-            vol_tiles(host[v].nx, host[v].ny, host[v].nz, planes, Wy, Wx, t);
+            vol_tiles(host[v].nx, host[v].ny, host[v].nz, planes, Wy, Wx, zsum, t);
             for (int p = 0; p < 3; ++p) {
                 host[v].first[p] = (uint32_t)tot[p];
                 tot[p] += t[p];
@@ -963,7 +1008,7 @@ extern "C" void* fe_edt_batch_create(const int32_t* dims, const int64_t* mask_of
         b->Lmax_y = Ly;
         b->Lmax_x = Lx;
         b->compact = Lx <= 1024 && Ly <= 1024 && Lz <= 1024;
-        b->zsum = zmin >= ZSUM_MIN_NZ;
+        b->zsum = zsum;
         for (int p = 0; p < 3; ++p) b->tiles[p] = tot[p];
         hipError_t e = hipMalloc(&b->table, sizeof(EdtVol) * (size_t)nvol);
         if (e == hipSuccess) e = hipMemcpy(b->table, host.data(), sizeof(EdtVol) * (size_t)nvol, hipMemcpyHostToDevice);
