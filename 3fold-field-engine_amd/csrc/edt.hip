@@ -132,10 +132,10 @@ __device__ __forceinline__ uint64_t lane_fetch64(uint64_t v, int src)
 //
 // One lane per row, 64 consecutive rows per one-wave block, so that every
 // word store covers 64 consecutive y.  The rows are staged through LDS 64
-// bytes at a time with coalesced loads (4-byte words when the rows start
-// 4-byte aligned, bytes otherwise); A is written in ascending tile order with
-// the last site carried forward, then B in descending order from the site
-// bits read back from the lane's own A words.
+// bytes at a time with coalesced 4-byte loads (any row alignment); A is
+// written in ascending tile order with the last site carried forward, then B
+// in descending order from the site bits read back from the lane's own A
+// words.
 #define ZSUM_NONE 0x7fff
 #define ZSUM_ROWS 64
 // Rows of at least ZSUM_MIN_NZ voxels take the z-summary path (the summaries
@@ -172,31 +172,53 @@ __global__ __launch_bounds__(ZSUM_ROWS) void k_edt_zsum(const uint8_t* __restric
     uint32_t* A0 = reinterpret_cast<uint32_t*>(outs + V.data) + x * tz * ys + y;
     uint32_t* A1 = reinterpret_cast<uint32_t*>(outs + V.data + V.plane) + x * tz * ys + y;
     const int64_t bofs = rows * tz;
-    const bool words = ((reinterpret_cast<uintptr_t>(masks + V.mask) | (uintptr_t)nz) & 3) == 0;
-    int last0 = -1, last1 = -1;  // last site so far
+    // The block's rows are one contiguous byte span.  Each row's chunk
+    // [c0, c0 + 64) is staged as the 17 aligned words that cover it (only
+    // words that start before the span end, so every load holds a byte of the
+    // span) and shifted into place by the row's byte misalignment.
+    const uintptr_t span0 = reinterpret_cast<uintptr_t>(m0), span1 = span0 + (uintptr_t)nrows * nz;
+    const int mis = (int)((span0 + (uintptr_t)lane * nz) & 3);  // c0 is a multiple of 64
+    int last0 = -1, last1 = -1;                                  // last site so far
     for (int c0 = 0; c0 < nz; c0 += 64) {
-        if (words) {
-            const int d = lane & 15;
-            for (int r = lane >> 4; r < nrows; r += 4) {
+        if (((span0 | (uintptr_t)nz) & 3) == 0) {  // every row 4-byte aligned: 16 words cover a chunk
+            for (int f = lane; f < ZSUM_ROWS * 16; f += 64) {
+                const int r = f >> 4, k = f & 15;
                 uint32_t w = 0;
-                if (c0 + 4 * d < nz) w = *reinterpret_cast<const uint32_t*>(m0 + (int64_t)r * nz + c0 + 4 * d);
-                *reinterpret_cast<uint32_t*>(stage + r * ZSUM_STRIDE + 4 * d) = w;
+                if (r < nrows && c0 + 4 * k < nz)
+                    w = *reinterpret_cast<const uint32_t*>(span0 + (uintptr_t)r * nz + c0 + 4 * k);
+                reinterpret_cast<uint32_t*>(stage + r * ZSUM_STRIDE)[k] = w;
             }
         } else {
-            for (int r = 0; r < nrows; ++r)
-                stage[r * ZSUM_STRIDE + lane] = c0 + lane < nz ? m0[(int64_t)r * nz + c0 + lane] : (uint8_t)0;
+            for (int f = lane; f < ZSUM_ROWS * 17; f += 64) {
+                const int r = f / 17, k = f - 17 * r;
+                uint32_t w = 0;
+                if (r < nrows) {
+                    const uintptr_t at = ((span0 + (uintptr_t)r * nz + c0) & ~(uintptr_t)3) + 4 * k;
+                    if (at < span1) w = *reinterpret_cast<const uint32_t*>(at);
+                }
+                reinterpret_cast<uint32_t*>(stage + r * ZSUM_STRIDE)[k] = w;
+            }
         }
         __syncthreads();
         if (row_ok) {
-            uint4 q[4];
+            uint32_t w[17], q[16];
 #pragma unroll
-            for (int k = 0; k < 4; ++k) q[k] = *reinterpret_cast<const uint4*>(stage + lane * ZSUM_STRIDE + 16 * k);
+            for (int k = 0; k < 4; ++k) {
+                const uint4 u = *reinterpret_cast<const uint4*>(stage + lane * ZSUM_STRIDE + 16 * k);
+                w[4 * k] = u.x;
+                w[4 * k + 1] = u.y;
+                w[4 * k + 2] = u.z;
+                w[4 * k + 3] = u.w;
+            }
+            w[16] = mis ? *reinterpret_cast<const uint32_t*>(stage + lane * ZSUM_STRIDE + 64) : 0u;
+#pragma unroll
+            for (int k = 0; k < 16; ++k) q[k] = __builtin_amdgcn_alignbyte(w[k + 1], w[k], mis);
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
                 const int t = (c0 >> 4) + k;
                 if (t < tz) {
-                    const uint32_t nzb = nonzero_nibble(q[k].x) | nonzero_nibble(q[k].y) << 4 |
-                                         nonzero_nibble(q[k].z) << 8 | nonzero_nibble(q[k].w) << 12;
+                    const uint32_t nzb = nonzero_nibble(q[4 * k]) | nonzero_nibble(q[4 * k + 1]) << 4 |
+                                         nonzero_nibble(q[4 * k + 2]) << 8 | nonzero_nibble(q[4 * k + 3]) << 12;
                     const int rem = nz - 16 * t;
                     const uint32_t valid = rem >= 16 ? 0xffffu : (1u << rem) - 1u;
                     const uint32_t b0 = (site_one ? nzb : ~nzb) & valid;

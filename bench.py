@@ -125,7 +125,8 @@ def bench_edt_heavy(fe, torch, reps=5, plain_sample=16):
     px = event_ms(torch, lambda: L.fe_edt_axis_pass(fe._lib.ptr(pair), fe._lib.ptr(work), *s, 2, 0, st), reps)
     out = {"volume_768": {"occupancy_fraction": occ, "pair_ms": edt_ms, "edt_mvox_s": n / (edt_ms * 1e-3) / 1e6,
                           "pass_ms": {"z": pz, "y": py, "x": px},
-                          "pair_gb_s": 53 * n / (edt_ms * 1e-3) / 1e9}}
+                          # algorithmic bytes of the three passes: z 2n, y 9n, x 16n
+                          "pair_gb_s": 27 * n / (edt_ms * 1e-3) / 1e9}}
     del mask, pair, work
     torch.cuda.empty_cache()
 
@@ -479,8 +480,10 @@ def main():
     kernels = {
         # name: (ms, algorithmic bytes per launch)
         "edt_pass_x": (px, 16 * N),
-        "edt_pass_y": (py, 16 * N),
-        "edt_pass_z": (pz, 9 * N),
+        # y pass: both planes' z summaries (8 bytes per 16 voxels and plane) in, two int32 planes out
+        "edt_pass_y": (py, 9 * N),
+        # z pass: the uint8 occupancy in, the z summaries out
+        "edt_pass_z": (pz, 2 * N),
         "reconstruct": (recon_ms, 5 * N + 4 * B3 * na + 8 * meta.n_blocks),
         "field_build": (build_ms, 8 * meta.n_blocks + (4 + 4 * B3) * na),
     }
