@@ -124,8 +124,9 @@ __device__ __forceinline__ uint64_t lane_fetch64(uint64_t v, int src)
 // tile) - (16 t + 15), both ZSUM_NONE where there is no such site.  Plane pl's
 // words sit at element offset data + pl * plane of the int32 arena, ordered
 // (x, t, y) so that a y-pass tile reads them contiguously: A of (x, y, t) at
-// (x * tz + t) * ny + y, B at nx * ny * tz further (tz = ceil(nz / 16);
-// 2 * nx * ny * tz <= nx * ny * nz for nz >= 2).  The squared z distance at
+// (x * tz + t) * ys + y, B at nx * tz * ys further (tz = ceil(nz / 16), ys =
+// ny rounded up to a multiple of 4 for 16-byte loads; 2 * nx * tz * ys <=
+// nx * ny * nz for nz >= 16).  The squared z distance at
 // z = 16 t + j follows from the in-tile bits, or else from dl / dr; with no
 // site in the row both distances exceed any axis, so the value clamps to the
 // sentinel.
@@ -168,10 +169,10 @@ __global__ __launch_bounds__(ZSUM_ROWS) void k_edt_zsum(const uint8_t* __restric
     const bool row_ok = lane < nrows;
     const int64_t row = row0 + lane;
     const int64_t x = row / V.ny, y = row - x * V.ny;
-    const int64_t ys = V.ny;  // words from one tile of a row to the next
+    const int64_t ys = (V.ny + 3) & ~3;  // words from one tile of a row to the next
     uint32_t* A0 = reinterpret_cast<uint32_t*>(outs + V.data) + x * tz * ys + y;
     uint32_t* A1 = reinterpret_cast<uint32_t*>(outs + V.data + V.plane) + x * tz * ys + y;
-    const int64_t bofs = rows * tz;
+    const int64_t bofs = (int64_t)V.nx * tz * ys;
     // The block's rows are one contiguous byte span.  Each row's chunk
     // [c0, c0 + 64) is staged as the 17 aligned words that cover it (only
     // words that start before the span end, so every load holds a byte of the
@@ -420,9 +421,10 @@ __global__ __launch_bounds__(W* T) void k_edt_lines(const int32_t* __restrict__ 
         if (active) {
             const int z = zt * W + line, j = z & 15;
             const int tz = (nz + 15) >> 4;
+            const int64_t ys = (L + 3) & ~3;
             const uint32_t* pa = reinterpret_cast<const uint32_t*>(src) + V.data + (int64_t)plane * V.plane +
-                                 ((int64_t)outer * tz + (z >> 4)) * L + p0;
-            const int64_t bofs = (int64_t)V.nx * V.ny * tz;
+                                 ((int64_t)outer * tz + (z >> 4)) * ys + p0;
+            const int64_t bofs = (int64_t)V.nx * tz * ys;
             const uint32_t lowm = (2u << j) - 1u;
             auto expand = [&](uint32_t a, uint32_t b) {
                 const uint32_t bits = a & 0xffffu;
