@@ -54,9 +54,12 @@ void *fe_edt_pair_create(int nx, int ny, int nz);
 int fe_edt_device(const uint8_t *mask, int nx, int ny, int nz, int planes, int32_t *work, int32_t *out,
                   void *stream);
 
-/* One axis pass of fe_edt_device (per-kernel timing and profiling): axis 2
- * reads the uint8 mask (z pass), axes 1 and 0 run the lower-envelope pass of
- * edt_native_linear_v1/edt.cu:13-29 over int32 planes. */
+/* One axis pass of fe_edt_device (per-kernel timing and profiling), with the
+ * intermediate layouts fe_edt_device uses: axis 2 reads the uint8 mask and
+ * writes the z pass (z summaries when nz >= 16, else int32 planes), axis 1
+ * reads that and runs the lower-envelope pass of
+ * edt_native_linear_v1/edt.cu:13-29 into int32 planes, axis 0 runs it over
+ * int32 planes.  The z-summary layout is internal to the library. */
 int fe_edt_axis_pass(const void *src, int32_t *dst, int nx, int ny, int nz, int planes, int axis, void *stream);
 
 /* Batched EDT of many uneven volumes (no reference counterpart; it replaces
