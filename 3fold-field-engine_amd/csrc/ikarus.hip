@@ -392,6 +392,173 @@ __global__ __launch_bounds__(IK_T) void k_ik_refine(IkProgram P, double lx, doub
     block_count(in, bd, counts);
 }
 
+// ---------------------------------------------------------------- grid-leaf pairs
+// Hard intersection of two grid leaves, the pair chosen per point
+// (FE/ikarus_v1/eval_warp.py:504-560): max(leaf a, leaf b) with the selection
+// max.  Out-of-range pair or leaf indices give NaN.
+__global__ __launch_bounds__(IK_T) void k_ik_pairs(const IkGrid* __restrict__ grids, int nleaves,
+                                                    const float* __restrict__ gdata, const float* __restrict__ pts,
+                                                    const int32_t* __restrict__ pair, const int32_t* __restrict__ la,
+                                                    const int32_t* __restrict__ lb, int npairs, int64_t n,
+                                                    float* __restrict__ out)
+{
+    const int64_t i = (int64_t)blockIdx.x * IK_T + threadIdx.x;
+    if (i >= n) return;
+    const int p = pair[i];
+    const int a = (p >= 0 && p < npairs) ? la[p] : -1, b = (p >= 0 && p < npairs) ? lb[p] : -1;
+    if (a < 0 || a >= nleaves || b < 0 || b >= nleaves) {
+        out[i] = __builtin_nanf("");
+        return;
+    }
+    const float x = pts[3 * i], y = pts[3 * i + 1], z = pts[3 * i + 2];
+    out[i] = wmax(ik_grid(grids[a], gdata, x, y, z), ik_grid(grids[b], gdata, x, y, z));
+}
+
+// Band tree with the plan as data (FE/ikarus_v1/eval_warp.py:567-700): the
+// minimum over bands i of max(leaf(p with p[axis] := zs[i]), max(p[axis] -
+// z1[i], z0[i] - p[axis])), folded from 1e30 with the selection min/max --
+// the interpreter's arithmetic on the same tree (a unit-axis halfspace adds
+// only signed zeros to p[axis]).
+__global__ __launch_bounds__(IK_T) void k_ik_band(const IkGrid* __restrict__ grids, const float* __restrict__ gdata,
+                                                   int axis, const float* __restrict__ zs, const float* __restrict__ z0,
+                                                   const float* __restrict__ z1, int nbands,
+                                                   const float* __restrict__ pts, int64_t n, float* __restrict__ out)
+{
+    const int64_t i = (int64_t)blockIdx.x * IK_T + threadIdx.x;
+    if (i >= n) return;
+    const float x = pts[3 * i], y = pts[3 * i + 1], z = pts[3 * i + 2];
+    const float pax = axis == 0 ? x : (axis == 1 ? y : z);
+    const IkGrid g = grids[0];
+    float best = 1.0e30f;
+    for (int b = 0; b < nbands; ++b) {
+        const float s = zs[b];
+        const float body = ik_grid(g, gdata, axis == 0 ? s : x, axis == 1 ? s : y, axis == 2 ? s : z);
+        const float slab = wmax(pax - z1[b], z0[b] - pax);
+        best = wmin(best, wmax(body, slab));
+    }
+    out[i] = best;
+}
+
+// One pair of a batched sweep (FE/ikarus_v1/queries.py:281-551): its coarse
+// grid lo + pitch*i (f64, samples rounded to fp32), the first global sample
+// index, the two leaves and the fp32 thresholds of the coarse and fine passes.
+struct IkPair {
+    double lx, ly, lz, pitch;
+    int64_t first;
+    int32_t nx, ny, nz, la, lb;
+    float below, band, fband;
+    int32_t pad[2];
+};
+static_assert(sizeof(IkPair) == 80, "IkPair layout");
+
+// max(leaf a, leaf b) of a pair; NaN (neither inside nor in a band) when a
+// leaf index is out of range.
+__device__ __forceinline__ float pair_value(const IkGrid* __restrict__ grids, int nleaves,
+                                            const float* __restrict__ gdata, const IkPair& P, float x, float y, float z)
+{
+    if (P.la < 0 || P.la >= nleaves || P.lb < 0 || P.lb >= nleaves) return __builtin_nanf("");
+    return wmax(ik_grid(grids[P.la], gdata, x, y, z), ik_grid(grids[P.lb], gdata, x, y, z));
+}
+
+// Pair of global sample t: the last pair whose first sample is <= t.
+__device__ __forceinline__ int find_pair(const IkPair* __restrict__ pairs, int npairs, int64_t t)
+{
+    int lo = 0, hi = npairs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (pairs[mid].first <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Per-pair counts: one atomic pair per wave when the wave's samples share a
+// pair (the usual case: pairs are contiguous sample ranges), else per lane.
+__device__ __forceinline__ void pair_count(int p, bool active, bool in, bool bd, unsigned long long* counts)
+{
+    const int p0 = __builtin_amdgcn_readfirstlane(p);
+    const uint64_t act = __ballot(active);
+    if (__ballot(active && p != p0) == 0) {
+        const int ni = __popcll(__ballot(active && in)), nb = __popcll(__ballot(active && bd));
+        if ((threadIdx.x & (FE_WAVE - 1)) == 0 && act) {
+            if (ni) atomicAdd(counts + 2 * p0, (unsigned long long)ni);
+            if (nb) atomicAdd(counts + 2 * p0 + 1, (unsigned long long)nb);
+        }
+    } else if (active) {
+        if (in) atomicAdd(counts + 2 * p, 1ull);
+        if (bd) atomicAdd(counts + 2 * p + 1, 1ull);
+    }
+}
+
+// Coarse pass of every pair in one launch: sample t of pair p at
+// (float)(lo + pitch*i) per axis (C order), v = max(leaf a, leaf b); counts
+// [p][0] += v < below, [p][1] += |v| <= band; band samples' global indices
+// appended to `cells` through `cursor`.
+__global__ __launch_bounds__(IK_T) void k_ik_pair_grid(const IkPair* __restrict__ pairs, int npairs, int64_t total,
+                                                        const IkGrid* __restrict__ grids, int nleaves,
+                                                        const float* __restrict__ gdata, unsigned long long* counts,
+                                                        int32_t* __restrict__ cells, unsigned int* cursor)
+{
+    const int64_t t = (int64_t)blockIdx.x * IK_T + threadIdx.x;
+    const bool active = t < total;
+    int p = 0;
+    bool in = false, bd = false;
+    if (active) {
+        p = find_pair(pairs, npairs, t);
+        const IkPair& P = pairs[p];
+        const int64_t l = t - P.first;
+        const int i = (int)(l / ((int64_t)P.ny * P.nz)), j = (int)((l / P.nz) % P.ny), k = (int)(l % P.nz);
+        const float x = (float)(P.lx + P.pitch * (double)i), y = (float)(P.ly + P.pitch * (double)j),
+                    z = (float)(P.lz + P.pitch * (double)k);
+        const float v = pair_value(grids, nleaves, gdata, P, x, y, z);
+        in = v < P.below;
+        bd = fabsf(v) <= P.band;
+    }
+    const uint64_t m = __ballot(bd);
+    const int lane = threadIdx.x & (FE_WAVE - 1);
+    unsigned int base = 0;
+    if (lane == 0 && m) base = atomicAdd(cursor, (unsigned int)__popcll(m));
+    base = __shfl(base, 0);
+    if (bd) cells[base + __popcll(m & ((1ull << lane) - 1))] = (int32_t)t;
+    pair_count(p, active, in, bd, counts);
+}
+
+// Refinement of the band samples of every pair in one launch: cell c (global
+// coarse index) gets r^3 samples at centre(c) + ((a + 0.5)/r * pitch - pitch/2)
+// per axis (f64, a-major), rounded to fp32; counts [p][0] += v < 0,
+// [p][1] += |v| <= fband.
+__global__ __launch_bounds__(IK_T) void k_ik_pair_refine(const IkPair* __restrict__ pairs, int npairs,
+                                                          const IkGrid* __restrict__ grids, int nleaves,
+                                                          const float* __restrict__ gdata,
+                                                          const int32_t* __restrict__ cells, int64_t ncells, int r,
+                                                          unsigned long long* counts)
+{
+    const int64_t r3 = (int64_t)r * r * r;
+    const int64_t t = (int64_t)blockIdx.x * IK_T + threadIdx.x;
+    const bool active = t < ncells * r3;
+    int p = 0;
+    bool in = false, bd = false;
+    if (active) {
+        const int64_t c = cells[t / r3];
+        p = find_pair(pairs, npairs, c);
+        const IkPair& P = pairs[p];
+        const int sub = (int)(t % r3);
+        const int a = sub / (r * r), b = (sub / r) % r, e = sub % r;
+        const int64_t l = c - P.first;
+        const int i = (int)(l / ((int64_t)P.ny * P.nz)), j = (int)((l / P.nz) % P.ny), k = (int)(l % P.nz);
+        const double half = P.pitch / 2.0;
+        const double ox = ((double)a + 0.5) / (double)r * P.pitch - half;
+        const double oy = ((double)b + 0.5) / (double)r * P.pitch - half;
+        const double oz = ((double)e + 0.5) / (double)r * P.pitch - half;
+        const float x = (float)((P.lx + P.pitch * (double)i) + ox), y = (float)((P.ly + P.pitch * (double)j) + oy),
+                    z = (float)((P.lz + P.pitch * (double)k) + oz);
+        const float v = pair_value(grids, nleaves, gdata, P, x, y, z);
+        in = v < 0.0f;
+        bd = fabsf(v) <= P.fband;
+    }
+    pair_count(p, active, in, bd, counts);
+}
+
 bool valid_program(const void* ins, int n_ins, const void* par)
 {
     return ins && par && n_ins > 0 && n_ins <= (1 << 20);
@@ -492,6 +659,82 @@ extern "C" int fe_ikarus_refine(const void* ins, int n_ins, const float* params,
     hipLaunchKernelGGL(k_ik_refine, dim3((unsigned)((total + IK_T - 1) / IK_T)), dim3(IK_T), 0,
                        static_cast<hipStream_t>(stream), P, lo[0], lo[1], lo[2], pitch, shape[1], shape[2], cells,
                        ncells, r, pitch / 2.0, band, counts);
+    FE_TRY(hipGetLastError());
+    return FE_OK;
+}
+
+// Grid-leaf pairs at device points (FE/ikarus_v1/eval_warp.py:545-560):
+// out[i] = max(leaf la[pair[i]], leaf lb[pair[i]]) at point i (n x 3 fp32);
+// grids: nleaves 48-byte records over `grid_data`; pair ids index la / lb
+// (npairs each), out-of-range indices give NaN.
+extern "C" int fe_ikarus_pairs_eval(const void* grids, int nleaves, const float* grid_data, const float* points,
+                                    const int32_t* pair_of_point, const int32_t* leaf_a, const int32_t* leaf_b,
+                                    int npairs, int64_t n, float* out, void* stream) noexcept
+{
+    if (!grids || !grid_data || nleaves < 1 || !points || !pair_of_point || !leaf_a || !leaf_b || npairs < 1 ||
+        n < 0 || !out)
+        return FE_INVALID;
+    if (n == 0) return FE_OK;
+    hipLaunchKernelGGL(k_ik_pairs, dim3((unsigned)((n + IK_T - 1) / IK_T)), dim3(IK_T), 0,
+                       static_cast<hipStream_t>(stream), static_cast<const IkGrid*>(grids), nleaves, grid_data, points,
+                       pair_of_point, leaf_a, leaf_b, npairs, n, out);
+    FE_TRY(hipGetLastError());
+    return FE_OK;
+}
+
+// Band tree of one grid leaf (grids[0]) at device points: the plan is nbands
+// fp32 triples (zs, z0, z1) in three device arrays; axis 0..2.
+extern "C" int fe_ikarus_band_eval(const void* grids, const float* grid_data, int axis, const float* zs,
+                                   const float* z0, const float* z1, int nbands, const float* points, int64_t n,
+                                   float* out, void* stream) noexcept
+{
+    if (!grids || !grid_data || axis < 0 || axis > 2 || nbands < 0 || (nbands && (!zs || !z0 || !z1)) || !points ||
+        n < 0 || !out)
+        return FE_INVALID;
+    if (n == 0) return FE_OK;
+    hipLaunchKernelGGL(k_ik_band, dim3((unsigned)((n + IK_T - 1) / IK_T)), dim3(IK_T), 0,
+                       static_cast<hipStream_t>(stream), static_cast<const IkGrid*>(grids), grid_data, axis, zs, z0, z1,
+                       nbands, points, n, out);
+    FE_TRY(hipGetLastError());
+    return FE_OK;
+}
+
+// Coarse pass of a batched pair sweep: pairs = npairs 80-byte records (device)
+// {double lx, ly, lz, pitch; int64 first; int32 nx, ny, nz, leaf_a, leaf_b;
+// float below, band, fine_band; int32 pad[2]} with first ascending from 0 and
+// total = last first + its nx*ny*nz < 2^31; counts: device uint64[npairs][2]
+// (added to); cells: device int32[total] and cursor uint32 (zeroed by the
+// caller) receive the band samples' global indices.
+extern "C" int fe_ikarus_pairs_grid(const void* pairs, int npairs, int64_t total, const void* grids, int nleaves,
+                                    const float* grid_data, unsigned long long* counts, int32_t* cells,
+                                    unsigned int* cursor, void* stream) noexcept
+{
+    if (!pairs || npairs < 1 || total < 0 || total >= INT32_MAX || !grids || nleaves < 1 || !grid_data || !counts ||
+        !cells || !cursor)
+        return FE_INVALID;
+    if (total == 0) return FE_OK;
+    hipLaunchKernelGGL(k_ik_pair_grid, dim3((unsigned)((total + IK_T - 1) / IK_T)), dim3(IK_T), 0,
+                       static_cast<hipStream_t>(stream), static_cast<const IkPair*>(pairs), npairs, total,
+                       static_cast<const IkGrid*>(grids), nleaves, grid_data, counts, cells, cursor);
+    FE_TRY(hipGetLastError());
+    return FE_OK;
+}
+
+// Refinement pass of a batched pair sweep over ncells band samples (global
+// indices from fe_ikarus_pairs_grid) with r^3 samples each.
+extern "C" int fe_ikarus_pairs_refine(const void* pairs, int npairs, const void* grids, int nleaves,
+                                      const float* grid_data, const int32_t* cells, int64_t ncells, int r,
+                                      unsigned long long* counts, void* stream) noexcept
+{
+    if (!pairs || npairs < 1 || !grids || nleaves < 1 || !grid_data || ncells < 0 || (ncells && !cells) || r < 1 ||
+        r > 64 || !counts)
+        return FE_INVALID;
+    const int64_t total = ncells * r * r * r;
+    if (total == 0) return FE_OK;
+    if (total >= ((int64_t)1 << 40)) return FE_INVALID;
+    hipLaunchKernelGGL(k_ik_pair_refine, dim3((unsigned)((total + IK_T - 1) / IK_T)), dim3(IK_T), 0,
+                       static_cast<hipStream_t>(stream), static_cast<const IkPair*>(pairs), npairs,
+                       static_cast<const IkGrid*>(grids), nleaves, grid_data, cells, ncells, r, counts);
     FE_TRY(hipGetLastError());
     return FE_OK;
 }

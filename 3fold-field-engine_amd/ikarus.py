@@ -238,6 +238,16 @@ def _grid_samples(node):
     return arr, [float(x) for x in origin], float(pitch), float(node["iso"]), node["mode"] == "sdf"
 
 
+def _grid_record(node, off):
+    """(48-byte grid record as a tuple, fp32 samples) of a grid leaf whose
+    samples start at element `off` of the grid data."""
+    arr, origin, pitch, iso, is_sdf = _grid_samples(node)
+    if arr.ndim != 3 or min(arr.shape) < 1:
+        raise ValueError("grid leaf needs a 3-D array")
+    return (off, *arr.shape, 1 if is_sdf else 0, *(np.float32(x) for x in origin), np.float32(pitch),
+            np.float32(iso), 0.0), arr
+
+
 def _need(node, memo):
     """Value-stack slots a subtree needs (Sethi-Ullman, the register top counted)."""
     key = id(node)
@@ -328,11 +338,8 @@ def compile_tree(node):
                                      -(ky + kcy), n["hy0"] + n["cy0"] - (ky + kcy) * z0,
                                      -(ky - kcy), n["hy0"] - n["cy0"] - (ky - kcy) * z0])
         elif op == "gridfield":
-            arr, origin, pitch, iso, is_sdf = _grid_samples(n)
-            if arr.ndim != 3 or min(arr.shape) < 1:
-                raise ValueError("grid leaf needs a 3-D array")
-            grids.append((goff[0], *arr.shape, 1 if is_sdf else 0, *(np.float32(x) for x in origin),
-                          np.float32(pitch), np.float32(iso), 0.0))
+            rec, arr = _grid_record(n, goff[0])
+            grids.append(rec)
             chunks.append(arr.reshape(-1))
             goff[0] += arr.size
             leaf(chain, GRID, len(grids) - 1, ())
@@ -449,3 +456,143 @@ def klassificera_och_evaluera_ikarus(meta, node, device="cuda:0"):
         tiles = np.zeros((0, B, B, B), np.float32)
     return SparseField(meta=meta, kind=kind, active_ids=active_ids, tiles=tiles, corner_min_abs=cma,
                        stats=_stats(meta, kind, len(active_ids)))
+
+
+# ------------------------------------------------------------------ grid-leaf pairs and band trees
+_LEAF_TABLES = {}
+_LEAF_TABLES_MAX = 32
+
+
+def _leaf_key(node):
+    """Identity of a grid leaf's data: (path, key, mode, iso, pitch override)
+    for an .npz leaf (the reference's upload-cache key,
+    FE/ikarus_v1/eval_warp.py:441, plus the pitch override), the array object
+    for an in-memory leaf."""
+    if node.get("op") != "gridfield":
+        raise ValueError("a grid leaf (op 'gridfield') is required")
+    if "values" in node:
+        return ("mem", id(node["values"]), tuple(node["origin"]), node["pitch"], node["mode"], node["iso"])
+    return ("npz", node["npz_path"], node["key"], node["mode"], node["iso"], node.get("pitch_override"))
+
+
+def leaf_table(leaves, device="cuda:0"):
+    """Device grid records (uint8 view of 48-byte records) and fp32 samples of
+    a list of grid leaves, uploaded once per (leaves, device) and reused
+    (FE/ikarus_v1/eval_warp.py:433-469)."""
+    import torch
+    dev = torch.device(device)
+    key = (tuple(_leaf_key(n) for n in leaves), str(dev))
+    hit = _LEAF_TABLES.get(key)
+    if hit is not None:
+        return hit[0], hit[1]
+    if not leaves:
+        raise ValueError("at least one grid leaf is required")
+    recs, chunks, off = [], [], 0
+    for n in leaves:
+        rec, arr = _grid_record(n, off)
+        recs.append(rec)
+        chunks.append(arr.reshape(-1))
+        off += arr.size
+    grids = torch.from_numpy(np.array(recs, GRID_REC).view(np.uint8)).to(dev)
+    gdata = torch.from_numpy(np.concatenate(chunks).astype(np.float32)).to(dev)
+    if len(_LEAF_TABLES) >= _LEAF_TABLES_MAX:
+        _LEAF_TABLES.pop(next(iter(_LEAF_TABLES)))
+    # the leaves are kept with the entry so that in-memory arrays keep their ids
+    _LEAF_TABLES[key] = (grids, gdata, list(leaves))
+    return grids, gdata
+
+
+def eval_batch_multi_pair(points_mm, pair_of_point, leaf_a_of_pair, leaf_b_of_pair, grid_leaves_all,
+                          device="cuda:0"):
+    """Hard intersection of two grid leaves per point, the pair chosen per
+    point, in one launch (FE/ikarus_v1/eval_warp.py:545-560): (N,) fp32
+    max(leaf a, leaf b) at the points cast to fp32."""
+    import torch
+    pts = np.ascontiguousarray(np.asarray(points_mm, np.float64).astype(np.float32)).reshape(-1, 3)
+    pop = np.ascontiguousarray(pair_of_point, np.int32).reshape(-1)
+    la = np.ascontiguousarray(leaf_a_of_pair, np.int32).reshape(-1)
+    lb = np.ascontiguousarray(leaf_b_of_pair, np.int32).reshape(-1)
+    nl = len(grid_leaves_all)
+    if pop.shape[0] != pts.shape[0]:
+        raise ValueError("pair_of_point needs one entry per point")
+    if len(la) != len(lb) or len(la) < 1:
+        raise ValueError("leaf_a_of_pair and leaf_b_of_pair need the same, nonzero length")
+    if pop.size and (pop.min() < 0 or pop.max() >= len(la)):
+        raise ValueError("pair index out of range")
+    if la.min() < 0 or lb.min() < 0 or max(la.max(), lb.max()) >= nl:
+        raise ValueError("leaf index out of range")
+    dev = torch.device(device)
+    grids, gdata = leaf_table(grid_leaves_all, dev)
+    n = pts.shape[0]
+    out = torch.empty(n, dtype=torch.float32, device=dev)
+    if n:
+        t_pts = torch.from_numpy(pts).to(dev)
+        t_pop = torch.from_numpy(pop).to(dev)
+        t_la = torch.from_numpy(la).to(dev)
+        t_lb = torch.from_numpy(lb).to(dev)
+        P = _lib.ptr
+        _check(_lib.lib().fe_ikarus_pairs_eval(P(grids), nl, P(gdata), P(t_pts), P(t_pop), P(t_la), P(t_lb), len(la),
+                                               n, P(out), _lib.stream_handle(None)), "ikarus pairs eval")
+    return out.cpu().numpy()
+
+
+def band_tree_expr(leaf_node, axis, bands):
+    """The tree of a band plan (FE/ikarus_v1/eval_warp.py:700-715):
+    union over bands of intersect(axis_project(leaf, axis, zs),
+    intersect(halfspace(+e, z1), halfspace(-e, -z0)))."""
+    def slab(z0, z1):
+        n = [0.0, 0.0, 0.0]
+        n[axis] = 1.0
+        nm = [0.0, 0.0, 0.0]
+        nm[axis] = -1.0
+        return intersect(halfspace(n, z1), halfspace(nm, -z0))
+    node = None
+    for b in bands:
+        piece = intersect(axis_project(leaf_node, axis, b["zs"]), slab(b["z0"], b["z1"]))
+        node = piece if node is None else union(node, piece)
+    return node
+
+
+class BandTreeEvaluator:
+    """A grid leaf and the band-tree pass, the plan swappable per call
+    (FE/ikarus_v1/eval_warp.py:659-697).  The leaf is uploaded once; a plan is
+    three fp32 arrays (zs, z0, z1), cached per plan.  Values equal the
+    interpreter's on band_tree_expr(leaf, axis, bands) bit for bit."""
+
+    def __init__(self, leaf_node, axis, device="cuda:0"):
+        import torch
+        if int(axis) not in (0, 1, 2):
+            raise ValueError("axis must be 0, 1 or 2")
+        self.leaf = leaf_node
+        self.axis = int(axis)
+        self.device = torch.device(device)
+        self.grid_args = leaf_table([leaf_node], self.device)
+        self._plan_cache = {}
+
+    def _plan_arrays(self, bands):
+        import torch
+        key = tuple((float(b["zs"]), float(b["z0"]), float(b["z1"])) for b in bands)
+        got = self._plan_cache.get(key)
+        if got is None:
+            plan = np.array([[b["zs"] for b in bands], [b["z0"] for b in bands], [b["z1"] for b in bands]],
+                            np.float32).reshape(3, len(bands))
+            got = (torch.from_numpy(np.ascontiguousarray(plan)).to(self.device), len(bands))
+            self._plan_cache[key] = got
+        return got
+
+    def eval(self, points_mm, bands):
+        """(N,) fp32 values of the band tree at the points (cast to fp32)."""
+        import torch
+        plan, nb = self._plan_arrays(bands)
+        pts = np.ascontiguousarray(np.asarray(points_mm, np.float64).astype(np.float32)).reshape(-1, 3)
+        n = pts.shape[0]
+        out = torch.empty(n, dtype=torch.float32, device=self.device)
+        if n:
+            t_pts = torch.from_numpy(pts).to(self.device)
+            grids, gdata = self.grid_args
+            P = _lib.ptr
+            base = plan.data_ptr()
+            _check(_lib.lib().fe_ikarus_band_eval(P(grids), P(gdata), self.axis, base, base + 4 * nb, base + 8 * nb,
+                                                  nb, P(t_pts), n, P(out), _lib.stream_handle(None)),
+                   "ikarus band eval")
+        return out.cpu().numpy()

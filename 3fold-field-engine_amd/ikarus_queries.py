@@ -330,3 +330,314 @@ def intrusion_volume(exprA, exprB, budget_ms=200.0, min_pitch_mm=0.25, max_refin
     if log is not None:
         log.append(result["dispatch_log"])
     return result
+
+
+def predicted_wall_ms(n_points, overhead_s, rate_pts_per_s):
+    """FE/ikarus_v1/cost_model_v1_2.py:111-112."""
+    return (overhead_s + n_points / max(rate_pts_per_s, 1e-9)) * 1000.0
+
+
+def total_volume(node, budget_ms=200.0, min_pitch_mm=0.25, device="cuda:0"):
+    """Volume of one tree: intrusion_volume(node, node) (FE/ikarus_v1/queries.py:554-558)."""
+    return intrusion_volume(node, node, budget_ms=budget_ms, min_pitch_mm=min_pitch_mm, device=device)
+
+
+def calibrate_throughput(node, n_probe=200_000, device="cuda:0"):
+    """(points per second, seconds) of one eval_batch call over n_probe seeded
+    uniform points in the tree's box (FE/ikarus_v1/queries.py:96-106)."""
+    lo, hi = bbox_of(node)
+    pts = np.random.default_rng(0).uniform(lo, hi, size=(n_probe, 3))
+    t0 = time.perf_counter()
+    E.eval_batch(node, pts, device=device)
+    dt = time.perf_counter() - t0
+    return n_probe / max(dt, 1e-6), dt
+
+
+def calibrate_throughput_cached(node, device="cuda:0", cache_path=None):
+    """(rate, overhead_s, from_cache) of the two-term model of a tree's grid
+    pass, cached per program shape (FE/ikarus_v1/queries.py:109-122)."""
+    passes = _Passes(node, device)
+    lo, hi = bbox_of(node)
+    key = "ikarus_grid::" + hashlib.sha256(passes.prog.ins[:, :2].tobytes()).hexdigest()[:16] + "::" + str(passes.dev)
+    overhead_s, rate, from_cache = calibrate_two_term(key, passes, lo, hi, cache_path=cache_path)
+    return rate, overhead_s, from_cache
+
+
+def min_clearance(exprA, exprB, budget_ms=200.0, n_samples=None, device="cuda:0"):
+    """Clearance of B from A's surface (FE/ikarus_v1/queries.py:561-597): A's
+    samples within half a cell diagonal of its surface on a budget-sized grid,
+    then the minimum of B's value there.  The near-surface samples are kept on
+    the device (the grid pass's band list) and B is evaluated at them there."""
+    import torch
+    rate, _ = calibrate_throughput(exprA, n_probe=50_000, device=device)
+    budget_points = max(int(budget_ms / 1000.0 * rate * 0.5), 500)
+    lo, hi = bbox_of(exprA)
+    bbox_vol = float(np.prod(hi - lo))
+    pitch = max((bbox_vol / budget_points) ** (1.0 / 3.0), 0.25)
+    t0 = time.perf_counter()
+    shape = grid_shape(lo, hi, pitch)
+    diag_half = 0.5 * pitch * math.sqrt(3.0)
+    passes = _Passes(exprA, device)
+    _, n_surf = passes.grid(lo, pitch, shape, -math.inf, diag_half, keep_cells=True)
+    if n_surf == 0:
+        return {"method": "surface_sample_vs_field", "min_clearance_mm": float("inf"), "n_surface_samples": 0,
+                "wall_ms": (time.perf_counter() - t0) * 1000.0,
+                "note": "expr A empty at this bbox/pitch -- no surface samples found"}
+    idx = passes.cells[:n_surf].to(torch.int64)
+    ny, nz = shape[1], shape[2]
+    ijk = torch.stack([idx // (ny * nz), (idx // nz) % ny, idx % nz], 1).to(torch.float64)
+    lo_t = torch.tensor(np.asarray(lo, np.float64), device=idx.device)
+    # lo + pitch * i in f64 (two roundings, as NumPy's grid), then fp32 as eval_batch casts
+    pts = (lo_t + ijk * pitch).to(torch.float32).contiguous()
+    vals = torch.empty(n_surf, dtype=torch.float32, device=idx.device)
+    E.eval_device(E.compile_tree(exprB), pts, vals)
+    min_clear = float(vals.min().item())
+    wall_s = time.perf_counter() - t0
+    return {"method": "surface_sample_vs_field", "min_clearance_mm": min_clear, "n_surface_samples": int(n_surf),
+            "pitch_mm": pitch, "uncertainty_mm": pitch, "budget_ms": budget_ms, "wall_ms": wall_s * 1000.0,
+            "budget_met_within_20pct": bool(wall_s * 1000.0 <= budget_ms * 1.2)}
+
+
+# ------------------------------------------------------------------ batched pair sweep
+PAIR_REC = np.dtype([("lx", "<f8"), ("ly", "<f8"), ("lz", "<f8"), ("pitch", "<f8"), ("first", "<i8"),
+                     ("nx", "<i4"), ("ny", "<i4"), ("nz", "<i4"), ("la", "<i4"), ("lb", "<i4"),
+                     ("below", "<f4"), ("band", "<f4"), ("fband", "<f4"), ("pad", "<i4", (2,))])
+assert PAIR_REC.itemsize == 80
+
+
+class _PairPasses:
+    """Device state of a batched pair sweep: the leaf table, one pair table
+    per pass, per-pair counters and the band-sample list."""
+
+    def __init__(self, leaves, device):
+        import torch
+        self.torch = torch
+        self.dev = torch.device(device)
+        self.nleaves = len(leaves)
+        self.grids, self.gdata = E.leaf_table(leaves, self.dev)
+        self.cells = None
+        self.cursor = torch.zeros(1, dtype=torch.int32, device=self.dev)
+
+    def upload(self, recs):
+        return self.torch.from_numpy(np.ascontiguousarray(recs).view(np.uint8)).to(self.dev)
+
+    def coarse(self, recs):
+        """Counts [n, 2] (below, band) of the coarse pass over the pairs `recs`;
+        the band samples' global indices stay in self.cells."""
+        t = self.torch
+        P = _lib.ptr
+        total = int(recs["first"][-1]) + int(recs["nx"][-1]) * int(recs["ny"][-1]) * int(recs["nz"][-1])
+        if total >= 2 ** 31 - 1:
+            raise ValueError("a sweep pass holds at most 2^31 - 2 samples")
+        counts = t.zeros((len(recs), 2), dtype=t.int64, device=self.dev)
+        if self.cells is None or self.cells.numel() < max(total, 1):
+            self.cells = t.empty(max(total, 1), dtype=t.int32, device=self.dev)
+        self.cursor.zero_()
+        self.table = self.upload(recs)
+        E._check(_lib.lib().fe_ikarus_pairs_grid(P(self.table), len(recs), total, P(self.grids), self.nleaves,
+                                                 P(self.gdata), P(counts), P(self.cells), P(self.cursor),
+                                                 _lib.stream_handle(None)), "pair sweep coarse pass")
+        return counts.cpu().numpy()
+
+    def refine(self, recs, ncells, r):
+        """Counts [n, 2] (below 0, fine band) of r^3 samples per band sample of
+        the last coarse pass (recs: the same pairs with the fine bands set)."""
+        t = self.torch
+        P = _lib.ptr
+        counts = t.zeros((len(recs), 2), dtype=t.int64, device=self.dev)
+        table = self.upload(recs)
+        E._check(_lib.lib().fe_ikarus_pairs_refine(P(table), len(recs), P(self.grids), self.nleaves, P(self.gdata),
+                                                   P(self.cells), int(ncells), int(r), P(counts),
+                                                   _lib.stream_handle(None)), "pair sweep refine pass")
+        return counts.cpu().numpy()
+
+
+class _OnePair:
+    """The coarse pass of one (leaf, leaf) pair as a `grid` call, for the cost model."""
+
+    def __init__(self, passes, la, lb):
+        self.p, self.la, self.lb = passes, la, lb
+
+    def grid(self, lo, pitch, shape, below, band, keep_cells):
+        rec = np.zeros(1, PAIR_REC)
+        rec["lx"], rec["ly"], rec["lz"] = lo
+        rec["pitch"] = pitch
+        rec["nx"], rec["ny"], rec["nz"] = shape
+        rec["la"], rec["lb"] = self.la, self.lb
+        rec["below"], rec["band"] = np.float32(below), np.float32(band)
+        c = self.p.coarse(rec)
+        return int(c[0, 0]), int(c[0, 1])
+
+
+def scene_sweep_gridfield_pairs(pairs, total_budget_ms=800.0, min_pitch_mm=0.25, max_refine_factor=8,
+                                target_refine_factor=4, min_pair_coarse_points=27, device="cuda:0", cache_path=None,
+                                prior_vol_override=None, max_pair_frac_of_pool=0.05, max_pair_points_abs=150_000,
+                                max_total_refine_points_abs=6_000_000):
+    """Hard-intersection volume and uncertainty of many (grid leaf, grid leaf)
+    pairs under one sweep budget (FE/ikarus_v1/queries.py:281-551): the budget
+    is shared by the pairs' box-overlap priors (clipped per pair), every pair's
+    coarse grid runs in ONE device launch and every pair's band samples are
+    refined in a second.  Returns (results, sweep_log) with the reference's
+    keys.  The grids are generated on the device from (lo, pitch, shape), so
+    no point buffer is built on the host."""
+    n_pairs = len(pairs)
+    if n_pairs == 0:
+        return [], {"n_pairs": 0, "note": "empty pair list"}
+    t_sweep0 = time.perf_counter()
+
+    # stage A: box-overlap priors, the leaf boxes memoised per leaf
+    box_cache = {}
+
+    def leaf_box(node):
+        k = E._leaf_key(node)
+        if k not in box_cache:
+            box_cache[k] = bbox_of(node)
+        return box_cache[k]
+
+    priors = []
+    for la, lb in pairs:
+        loA, hiA = leaf_box(la)
+        loB, hiB = leaf_box(lb)
+        lo = np.maximum(loA, loB)
+        hi = np.minimum(hiA, hiB)
+        empty = bool(np.any(hi <= lo))
+        priors.append({"lo": lo, "hi": hi, "vol": 0.0 if empty else float(np.prod(hi - lo)), "empty": empty})
+    if prior_vol_override is not None:
+        if len(prior_vol_override) != n_pairs:
+            raise ValueError("prior_vol_override needs one value per pair")
+        for i, p in enumerate(priors):
+            p["alloc_vol"] = 0.0 if p["empty"] else float(prior_vol_override[i])
+    else:
+        for p in priors:
+            p["alloc_vol"] = p["vol"]
+    total_prior_vol = sum(p["alloc_vol"] for p in priors)
+    t_prior = time.perf_counter() - t_sweep0
+
+    # the leaf table (distinct leaves) and the device passes
+    all_leaves, leaf_index = [], {}
+
+    def leaf_id(node):
+        k = E._leaf_key(node)
+        if k not in leaf_index:
+            leaf_index[k] = len(all_leaves)
+            all_leaves.append(node)
+        return leaf_index[k]
+
+    leaf_a_of_pair = np.array([leaf_id(la) for la, _ in pairs], np.int32)
+    leaf_b_of_pair = np.array([leaf_id(lb) for _, lb in pairs], np.int32)
+    passes = _PairPasses(all_leaves, device)
+
+    # stage B: two-term cost model of the pair pass (cached per device)
+    lo0, hi0 = leaf_box(pairs[0][0])
+    overhead_s, rate, from_cache = calibrate_two_term("ikarus_pairs::" + str(passes.dev),
+                                                      _OnePair(passes, int(leaf_a_of_pair[0]),
+                                                               int(leaf_a_of_pair[0])),
+                                                      lo0, hi0, cache_path=cache_path)
+    floor_total = n_pairs * min_pair_coarse_points
+    total_points_budget = affordable_points(total_budget_ms, overhead_s, rate, floor_points=floor_total)
+    alloc_pool = max(total_points_budget - floor_total, 0)
+
+    # stage C: per-pair coarse pitch from its clipped share of the pool
+    max_pair_points = max(min(int(alloc_pool * max_pair_frac_of_pool), max_pair_points_abs), min_pair_coarse_points)
+    coarse_pitch_of_pair = [None] * n_pairs
+    n_pairs_capped = 0
+    for i, p in enumerate(priors):
+        if p["empty"]:
+            continue
+        share = (p["alloc_vol"] / total_prior_vol) if total_prior_vol > 0 else (1.0 / n_pairs)
+        n_target = min_pair_coarse_points + int(alloc_pool * share)
+        if n_target > max_pair_points:
+            n_target = max_pair_points
+            n_pairs_capped += 1
+        coarse_pitch_of_pair[i] = max((p["vol"] / max(n_target, 1)) ** (1.0 / 3.0), min_pitch_mm)
+
+    # the pair table of the coarse launch (pairs with a grid pass, in order)
+    t_buf0 = time.perf_counter()
+    live = [i for i in range(n_pairs) if coarse_pitch_of_pair[i] is not None]
+    recs = np.zeros(len(live), PAIR_REC)
+    first = 0
+    for r_i, i in enumerate(live):
+        pitch = coarse_pitch_of_pair[i]
+        shape = grid_shape(priors[i]["lo"], priors[i]["hi"], pitch)
+        dh = 0.5 * pitch * math.sqrt(3.0)
+        recs[r_i]["lx"], recs[r_i]["ly"], recs[r_i]["lz"] = priors[i]["lo"]
+        recs[r_i]["pitch"] = pitch
+        recs[r_i]["first"] = first
+        recs[r_i]["nx"], recs[r_i]["ny"], recs[r_i]["nz"] = shape
+        recs[r_i]["la"], recs[r_i]["lb"] = leaf_a_of_pair[i], leaf_b_of_pair[i]
+        recs[r_i]["below"], recs[r_i]["band"] = np.float32(-dh), np.float32(dh)
+        first += int(np.prod(shape))
+    t_buffer_build = time.perf_counter() - t_buf0
+
+    t0 = time.perf_counter()
+    coarse = passes.coarse(recs) if len(live) else np.zeros((0, 2), np.int64)
+    t_coarse_launch = time.perf_counter() - t0
+
+    # stage D: per-pair volume and band counts of the coarse launch
+    results = [None] * n_pairs
+    n_boundary_of_pair = [0] * n_pairs
+    for i in range(n_pairs):
+        if coarse_pitch_of_pair[i] is None:
+            results[i] = {"method": "sweep_batched_prior_alloc", "volume_mm3": 0.0, "uncertainty_mm3": 0.0,
+                          "dispatch_log": {"note": "AABB-overlap prior volume is 0 -- no grid pass needed",
+                                           "prior_vol_mm3": 0.0}}
+    for r_i, i in enumerate(live):
+        pitch = coarse_pitch_of_pair[i]
+        n_i = int(recs[r_i]["nx"]) * int(recs[r_i]["ny"]) * int(recs[r_i]["nz"])
+        n_in, n_b = int(coarse[r_i, 0]), int(coarse[r_i, 1])
+        cell_vol = pitch ** 3
+        n_boundary_of_pair[i] = n_b
+        results[i] = {"method": "sweep_batched_prior_alloc", "volume_mm3": float(n_in) * cell_vol,
+                      "uncertainty_mm3": n_b * cell_vol,
+                      "dispatch_log": {"prior_vol_mm3": priors[i]["vol"], "coarse_pitch_mm": pitch,
+                                       "n_coarse_points": n_i, "n_fully_inside_coarse": n_in,
+                                       "n_boundary_coarse": n_b}}
+
+    # stage E: one refinement launch over every pair's band samples
+    total_boundary = sum(n_boundary_of_pair)
+    remaining_budget_ms = max(total_budget_ms - t_coarse_launch * 1000.0, 0.0)
+    remaining_points_budget = affordable_points(remaining_budget_ms, overhead_s, rate, floor_points=0)
+    refine_factor = 1
+    t_refine_launch = 0.0
+    t_refine_buffer_build = 0.0
+    if total_boundary > 0 and remaining_points_budget > total_boundary:
+        refine_factor = int(min(max_refine_factor, target_refine_factor,
+                                max((remaining_points_budget / total_boundary) ** (1.0 / 3.0), 1.0)))
+    while refine_factor > 1 and total_boundary * (refine_factor ** 3) > max_total_refine_points_abs:
+        refine_factor -= 1
+    if refine_factor > 1 and total_boundary > 0:
+        t_rb0 = time.perf_counter()
+        for r_i, i in enumerate(live):
+            raw_fine_pitch = coarse_pitch_of_pair[i] / refine_factor
+            recs[r_i]["fband"] = np.float32(0.5 * raw_fine_pitch * math.sqrt(3.0))
+        t_refine_buffer_build = time.perf_counter() - t_rb0
+        t0 = time.perf_counter()
+        fine = passes.refine(recs, total_boundary, refine_factor)
+        t_refine_launch = time.perf_counter() - t0
+        for r_i, i in enumerate(live):
+            if n_boundary_of_pair[i] == 0:
+                continue
+            raw_fp = coarse_pitch_of_pair[i] / refine_factor
+            fine_cell_vol = raw_fp ** 3
+            results[i]["volume_mm3"] += float(int(fine[r_i, 0])) * fine_cell_vol
+            results[i]["uncertainty_mm3"] = int(fine[r_i, 1]) * fine_cell_vol
+            log = results[i]["dispatch_log"]
+            log["refine_factor"] = refine_factor
+            log["fine_pitch_mm"] = max(raw_fp, min_pitch_mm)
+            log["n_fine_points"] = n_boundary_of_pair[i] * refine_factor ** 3
+            log["n_boundary_remaining_at_finest"] = int(fine[r_i, 1])
+
+    sweep_log = {
+        "n_pairs": n_pairs, "total_budget_ms": total_budget_ms,
+        "cost_model_v1_2": {"overhead_s": overhead_s, "rate_pts_per_s": rate, "from_cache": from_cache},
+        "total_prior_overlap_vol_mm3": total_prior_vol,
+        "n_pairs_empty_prior": sum(1 for p in priors if p["empty"]),
+        "total_points_budget": total_points_budget, "n_coarse_points_actual": int(first),
+        "max_pair_points_cap": max_pair_points, "n_pairs_capped": n_pairs_capped,
+        "n_launches": 1 + (1 if (refine_factor > 1 and total_boundary > 0) else 0),
+        "refine_factor_chosen": refine_factor, "total_boundary_cells": total_boundary,
+        "t_prior_stage_s": t_prior, "t_buffer_build_s": t_buffer_build, "t_coarse_launch_s": t_coarse_launch,
+        "t_refine_buffer_build_s": t_refine_buffer_build, "t_refine_launch_s": t_refine_launch,
+        "t_sweep_wall_s": time.perf_counter() - t_sweep0,
+    }
+    return results, sweep_log

@@ -291,6 +291,39 @@ int fe_ikarus_refine(const void *ins, int n_ins, const float *params, const void
                      const double *lo, double pitch, const int32_t *shape, const int32_t *cells, int64_t ncells, int r,
                      float band, unsigned long long *counts, void *stream);
 
+/* Grid-leaf pairs, replacing the fixed-shape multi-pair kernel of
+ * ikarus_v1/eval_warp.py:491-560 (eval_batch_multi_pair): out[i] =
+ * max(leaf leaf_a[pair_of_point[i]], leaf leaf_b[...]) at point i (selection
+ * max), over nleaves grid records as above; out-of-range indices give NaN.
+ * All pointers are device pointers. */
+int fe_ikarus_pairs_eval(const void *grids, int nleaves, const float *grid_data, const float *points,
+                         const int32_t *pair_of_point, const int32_t *leaf_a, const int32_t *leaf_b, int npairs,
+                         int64_t n, float *out, void *stream);
+
+/* Band tree of one grid leaf (grids[0]) with the plan as data, replacing
+ * ikarus_v1/eval_warp.py:567-700 (compile_band_tree_kernel,
+ * BandTreeEvaluator.eval): min over bands of max(leaf(p, p[axis] := zs),
+ * max(p[axis] - z1, z0 - p[axis])), folded from 1e30; zs/z0/z1: nbands fp32
+ * each (device). */
+int fe_ikarus_band_eval(const void *grids, const float *grid_data, int axis, const float *zs, const float *z0,
+                        const float *z1, int nbands, const float *points, int64_t n, float *out, void *stream);
+
+/* The two batched launches of ikarus_v1/queries.py:281-551
+ * (scene_sweep_gridfield_pairs): pairs = npairs 80-byte device records
+ * {double lx, ly, lz, pitch; int64 first; int32 nx, ny, nz, leaf_a, leaf_b;
+ * float below, band, fine_band; int32 pad[2]} (first ascending from 0; total
+ * samples < 2^31).  pairs_grid: coarse samples lo + pitch*i of every pair,
+ * v = max(leaf a, leaf b), counts[p][0] += v < below, counts[p][1] +=
+ * |v| <= band (device uint64[npairs][2]); band samples' global indices
+ * appended to cells through *cursor.  pairs_refine: r^3 samples per listed
+ * cell at centre + ((a+0.5)/r*pitch - pitch/2); counts[p][0] += v < 0,
+ * counts[p][1] += |v| <= fine_band. */
+int fe_ikarus_pairs_grid(const void *pairs, int npairs, int64_t total, const void *grids, int nleaves,
+                         const float *grid_data, unsigned long long *counts, int32_t *cells, unsigned int *cursor,
+                         void *stream);
+int fe_ikarus_pairs_refine(const void *pairs, int npairs, const void *grids, int nleaves, const float *grid_data,
+                           const int32_t *cells, int64_t ncells, int r, unsigned long long *counts, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

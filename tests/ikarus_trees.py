@@ -91,3 +91,35 @@ def balanced(E, n_levels):
             return E.sphere(0.5 + 0.01 * idx, (idx * 0.1, 0.0, 0.0))
         return E.union(build(level - 1, 2 * idx), build(level - 1, 2 * idx + 1))
     return build(n_levels, 0)
+
+
+def sampled_leaf(E, sdf, lo, pitch, shape, mode="sdf", iso=0.5):
+    """A grid leaf holding the samples of a float64 field `sdf(X, Y, Z)` at
+    lo + pitch*i (occupancy mode: 1 inside, 0 outside)."""
+    g = [lo[a] + np.arange(shape[a]) * pitch for a in range(3)]
+    X, Y, Z = np.meshgrid(*g, indexing="ij")
+    v = sdf(X, Y, Z)
+    if mode != "sdf":
+        v = (v < 0).astype(np.float64)
+    return E.grid_leaf(v.astype(np.float32), lo, pitch, mode=mode, iso=iso)
+
+
+def sphere_sdf(c, r):
+    return lambda X, Y, Z: np.sqrt((X - c[0]) ** 2 + (Y - c[1]) ** 2 + (Z - c[2]) ** 2) - r
+
+
+def box_sdf(c, h):
+    def f(X, Y, Z):
+        q = [np.abs(X - c[0]) - h[0], np.abs(Y - c[1]) - h[1], np.abs(Z - c[2]) - h[2]]
+        out = np.sqrt(sum(np.maximum(a, 0.0) ** 2 for a in q))
+        return out + np.minimum(np.maximum(q[0], np.maximum(q[1], q[2])), 0.0)
+    return f
+
+
+def leaf_set(E):
+    """Five grid leaves of different shapes, pitches and modes."""
+    return [sampled_leaf(E, sphere_sdf((0.0, 0.0, 0.0), 5.0), (-7.0, -7.0, -7.0), 0.5, (29, 29, 29)),
+            sampled_leaf(E, box_sdf((4.0, 1.0, 0.5), (3.0, 2.5, 4.0)), (-1.0, -3.0, -5.0), 0.25, (41, 27, 45)),
+            sampled_leaf(E, sphere_sdf((6.0, 0.0, 1.0), 3.0), (2.0, -4.0, -3.0), 0.4, (21, 21, 21), mode="occupancy"),
+            sampled_leaf(E, box_sdf((-3.0, -2.0, 0.0), (2.0, 2.0, 2.0)), (-6.0, -5.0, -3.0), 0.3, (21, 21, 21)),
+            sampled_leaf(E, sphere_sdf((40.0, 0.0, 0.0), 2.0), (37.0, -3.0, -3.0), 0.5, (13, 13, 13))]

@@ -182,3 +182,45 @@ def test_oracle_grid_volume_of_box_overlap(E):
     r = IR.grid_volume(E.intersect(a, b), lo, hi, 0.4, 4)
     assert abs(r["volume_mm3"] - exact) < 0.05 * exact
     assert r["n_boundary"] > 0 and r["n_fine_inside"] > 0
+
+
+def test_oracle_pairs_band_and_sweep_pinned(E):
+    """The pair, band-tree and sweep restatements against the tree-walking
+    oracle: a pair's value is intersect(a, b) (FE/ikarus_v1/eval_warp.py:
+    504-560); the band plan as data equals the literal tree bit for bit, as
+    the reference states for its kernel (FE/ikarus_v1/eval_warp.py:582-594),
+    and that tree has 7*n_bands - 1 nodes (:575); a pair's sweep counts are
+    grid_volume's on intersect(a, b)."""
+    from ikarus_trees import leaf_set
+    leaves = leaf_set(E)
+    rng = np.random.default_rng(31)
+    pts = rng.uniform([-8, -6, -6], [12, 6, 6], (3000, 3))
+    la, lb = [0, 1, 2, 3, 0], [1, 2, 0, 0, 4]
+    pop = rng.integers(0, len(la), len(pts))
+    got = IR.pairs_f32(pts, pop, la, lb, leaves)
+    for k in range(len(la)):
+        sel = pop == k
+        want = IR.eval_f32(E.intersect(leaves[la[k]], leaves[lb[k]]), pts[sel])
+        assert got[sel].tobytes() == want.tobytes(), k
+    bands = [{"zs": float(z), "z0": float(z - 0.6), "z1": float(z + 0.4)} for z in np.linspace(-4.0, 4.5, 9)]
+    for axis in range(3):
+        tree = E.band_tree_expr(leaves[0], axis, bands)
+        assert E.node_count(tree) == 7 * len(bands) - 1
+        assert IR.band_f32(leaves[0], axis, bands, pts).tobytes() == IR.eval_f32(tree, pts).tobytes(), axis
+    lo, hi = np.array([-1.0, -3.0, -5.0]), np.array([5.0, 5.0, 5.0])
+    s = IR.sweep_counts(leaves, [0], [1], [lo], [hi], [0.6], 3)[0]
+    g = IR.grid_volume(E.intersect(leaves[0], leaves[1]), lo, hi, 0.6, 3)
+    for k in ("n_fully_inside", "n_boundary", "n_fine_inside", "n_fine_boundary"):
+        assert s[k] == g[k], k
+    assert IR.sweep_counts(leaves, [0], [1], [lo], [hi], [None], 3) == [None]
+
+
+def test_sweep_helpers(fe, E):
+    Q = fe.ikarus_queries
+    assert Q.predicted_wall_ms(1000, 0.001, 1e6) == pytest.approx(2.0)
+    assert Q.PAIR_REC.itemsize == 80
+    from ikarus_trees import leaf_set
+    leaves = leaf_set(E)
+    assert E._leaf_key(leaves[0]) != E._leaf_key(leaves[1])
+    with pytest.raises(ValueError):
+        E._leaf_key(E.sphere(1.0))
