@@ -16,9 +16,11 @@
 //  * Exact Lipschitz culling per superblock: each primitive is bounded on the
 //    superblock's box by value(centre) +- (half diagonal + fp32 slack); the
 //    ordered min/max fold of those intervals is a composition of clamps, so a
-//    workgroup-wide scan gives the fold interval before every step, and a step
-//    that provably selects the running value everywhere (strict inequality) is
-//    dropped.  Dropped steps are identities, so results do not change.
+//    workgroup-wide scan gives the fold interval at every step, and an
+//    operand that is strictly worse everywhere than another operand of its
+//    run of same-op steps is dropped.  The selection min/max returns the
+//    extreme operand (the last of equal ones), which a dropped operand never
+//    is, so results do not change.
 //  * Corner and voxel loops run the primitive list in the outer loop with
 //    scalar (wave-uniform) parameter loads and several points per lane.
 //  * Compaction: wave ballots + popcount prefix; each chunk of 1024 blocks
@@ -159,6 +161,40 @@ __device__ __forceinline__ void prim_interval(const Prim& p, float cx, float cy,
     hi = v + rad + slack;
 }
 
+// Culling rule.  Consecutive steps of one op form a run, and the fold over a
+// run is a selection over its operands (min for union, max for subtraction
+// and intersection, plus the incoming value), which returns the extreme value
+// and, among operands equal to it, the last.  An operand that is strictly
+// worse than some other operand of its run everywhere in the box is never
+// that value, so dropping it leaves every result bit unchanged; a run's
+// interval after its last step bounds the best other operand.  Runs are cut at
+// wave boundaries (each cut run is itself a run) and primitive 0, which
+// starts the fold, is its own run and always kept.
+
+// Inclusive scan value at the end of this lane's run within the wave.
+__device__ __forceinline__ Clamp2 run_end(const Clamp2& inc, int op, bool first, bool active, bool has_next)
+{
+    const int lane = threadIdx.x & 63;
+    const int me = first ? 3 : op;
+    const int nx = __shfl_down(me, 1);
+    const bool end = active && (lane == 63 || !has_next || nx != me || me == 3);
+    const uint64_t ends = __ballot(end);
+    const uint64_t after = ends >> lane;
+    const int e = after ? lane + __builtin_ctzll(after) : lane;
+    return {__shfl(inc.ll, e), __shfl(inc.lu, e), __shfl(inc.hl, e), __shfl(inc.hu, e)};
+}
+
+// Keep an operand with value interval [lo, hi] whose run ends with the fold
+// interval [rlo, rhi]: union needs lo <= rhi, subtraction -lo >= rlo,
+// intersection hi >= rlo.
+__device__ __forceinline__ bool keep_step(int op, bool first, float lo, float hi, float rlo, float rhi)
+{
+    if (first) return true;
+    if (op == 0) return !(lo > rhi);
+    if (op == 1) return !(-lo < rlo);
+    return !(hi < rlo);
+}
+
 // Builds the ordered list of primitives that can change the fold somewhere in
 // the box.  Called by all 256 threads; returns the list length (list in LDS).
 __device__ int cull_box(const Prim* __restrict__ prims, int nprims, float cx, float cy, float cz, float rad,
@@ -192,17 +228,9 @@ __device__ int cull_box(const Prim* __restrict__ prims, int nprims, float cx, fl
         __syncthreads();
         Clamp2 pre = {-INFINITY, INFINITY, -INFINITY, INFINITY};
         for (int w = 0; w < wid; ++w) pre = ccompose(pre, wtot[w]);
-        Clamp2 exc = cshfl_up(inc, 1);
-        if (lane == 0) exc = {-INFINITY, INFINITY, -INFINITY, INFINITY};
-        const Clamp2 before = ccompose(pre, exc);
-        const float blo = capply(Flo, before.ll, before.lu), bhi = capply(Fhi, before.hl, before.hu);
-        bool keep = false;
-        if (i < nprims) {
-            if (i == 0) keep = true;
-            else if (op == 0) keep = !(lo > bhi);
-            else if (op == 1) keep = !(-lo < blo);
-            else keep = !(hi < blo);
-        }
+        const Clamp2 run = ccompose(pre, run_end(inc, op, i == 0, i < nprims, i + 1 < nprims));
+        const float rlo = capply(Flo, run.ll, run.lu), rhi = capply(Fhi, run.hl, run.hu);
+        const bool keep = i < nprims && keep_step(op, i == 0, lo, hi, rlo, rhi);
         const uint64_t bal = __ballot(keep);
         if (lane == 0) wcnt[wid] = __popcll(bal);
         __syncthreads();
@@ -223,7 +251,8 @@ __device__ int cull_box(const Prim* __restrict__ prims, int nprims, float cx, fl
 // Wave-level refinement of a primitive list for a smaller box (a group of
 // blocks or one block).  src: the parent list (its first entry is primitive 0)
 // or nullptr for all primitives; out: per-wave LDS array.  Same exact rule as
-// cull_box, scanned 64 candidates at a time.
+// cull_box, scanned 64 candidates at a time (runs of the list, which join
+// across dropped steps of other ops).
 __device__ int cull_wave(const Prim* __restrict__ prims, const uint16_t* src, int nsrc, float cx, float cy, float cz,
                          float rad, float slack, uint16_t* out)
 {
@@ -251,16 +280,9 @@ __device__ int cull_wave(const Prim* __restrict__ prims, const uint16_t* src, in
             Clamp2 o = cshfl_up(inc, d);
             if (lane >= d) inc = ccompose(o, inc);
         }
-        Clamp2 exc = cshfl_up(inc, 1);
-        if (lane == 0) exc = {-INFINITY, INFINITY, -INFINITY, INFINITY};
-        const float blo = capply(Flo, exc.ll, exc.lu), bhi = capply(Fhi, exc.hl, exc.hu);
-        bool keep = false;
-        if (j < nsrc) {
-            if (j == 0) keep = true;
-            else if (op == 0) keep = !(lo > bhi);
-            else if (op == 1) keep = !(-lo < blo);
-            else keep = !(hi < blo);
-        }
+        const Clamp2 run = run_end(inc, op, j == 0, j < nsrc, j + 1 < nsrc);
+        const float rlo = capply(Flo, run.ll, run.lu), rhi = capply(Fhi, run.hl, run.hu);
+        const bool keep = j < nsrc && keep_step(op, j == 0, lo, hi, rlo, rhi);
         const uint64_t bal = __ballot(keep);
         if (keep) out[n + __popcll(bal & ((1ull << lane) - 1))] = (uint16_t)idx;
         n += __popcll(bal);
