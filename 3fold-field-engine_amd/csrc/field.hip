@@ -250,9 +250,10 @@ __device__ int cull_box(const Prim* __restrict__ prims, int nprims, float cx, fl
 
 // Wave-level refinement of a primitive list for a smaller box (a group of
 // blocks or one block).  src: the parent list (its first entry is primitive 0)
-// or nullptr for all primitives; out: per-wave LDS array.  Same exact rule as
-// cull_box, scanned 64 candidates at a time (runs of the list, which join
-// across dropped steps of other ops).
+// or nullptr for all primitives; out: per-wave LDS array (may be src: entry j
+// is read before any entry at or past the write position is written).  Same
+// exact rule as cull_box, scanned 64 candidates at a time (runs of the list,
+// which join across dropped steps of other ops).
 __device__ int cull_wave(const Prim* __restrict__ prims, const uint16_t* src, int nsrc, float cx, float cy, float cz,
                          float rad, float slack, uint16_t* out)
 {
@@ -349,6 +350,7 @@ __global__ __launch_bounds__(256) void k_classify(const FieldDesc* __restrict__ 
         int gn = n;
         if (n <= FE_MAX_PRIMS / 2) {
             gn = cull_wave(D.prims, list, n, cx, cy, cz, rad, D.slack, wl);
+            if (gn > 8) gn = cull_wave(D.prims, wl, gn, cx, cy, cz, rad, D.slack, wl);
             gl = wl;
         }
         const int bl = lane >> 3, c = lane & 7;
@@ -464,6 +466,9 @@ __device__ __forceinline__ void tile_eval(const FieldDesc& D, int64_t a, int bl,
             float cx, cy, cz, rad;
             block_box(D, bi, bj, bk, 1, cx, cy, cz, rad);
             n = cull_wave(D.prims, src, nsrc, cx, cy, cz, rad, D.slack, wl);
+            // a second pass over the refined list (in place): its runs join
+            // across dropped steps and are no longer cut every 64 entries
+            if (n > 8) n = cull_wave(D.prims, wl, n, cx, cy, cz, rad, D.slack, wl);
             list = wl;
         } else {
             list = src;
