@@ -109,6 +109,16 @@ __device__ __forceinline__ void row_sites(const uint8_t* __restrict__ m, int nz,
     }
 }
 
+// Orders this wave's LDS writes before its later LDS reads by other lanes
+// (LDS requests of one wave are served in order; the fences keep the
+// compiler from moving accesses across).
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // 64-bit value of lane `src` (per-lane index; every lane of the wave active).
 __device__ __forceinline__ uint64_t lane_fetch64(uint64_t v, int src)
 {
@@ -516,7 +526,13 @@ __global__ __launch_bounds__(W* T) void k_edt_lines(const int32_t* __restrict__ 
     }
     blo[me] = 0;
     bhi[me] = (uint8_t)(k + 1);
-    __syncthreads();
+    // A wave holds CPW consecutive chunks of each of its lines, so the merge
+    // rounds whose groups of 2h chunks fit in one wave read only data of their
+    // own wave: those rounds (and the hull stacks before them) need a wave
+    // sync, not a block barrier.
+    constexpr int CPW = FE_WAVE / W;
+    static_assert(CPW >= 2, "a wave holds at least two chunks of a line");
+    wave_sync();
 
     // 2. tree merge of the chunk hulls
     for (int h = 1; h < T; h <<= 1) {
@@ -582,7 +598,8 @@ __global__ __launch_bounds__(W* T) void k_edt_lines(const int32_t* __restrict__ 
                 blo[slot(hb)] = (uint8_t)hl;
             }
         }
-        __syncthreads();
+        if (4 * h <= CPW) wave_sync();  // the next round (2h) stays within the wave
+        else __syncthreads();
     }
     if constexpr (COMPACT) {
         // 3. Each thread computes, for the surviving entries of its chunk, the
