@@ -769,10 +769,11 @@ __global__ __launch_bounds__(W* T) void k_edt_lines(const int32_t* __restrict__ 
         }
     }
     if (active) {
+        // streaming stores: a pass's output is far larger than the caches
         int32_t* pd = dst + lbase + (int64_t)p0 * axis_stride;
 #pragma unroll
         for (int i = 0; i < CMAX; ++i) {
-            if (p0 + i < p1) *pd = v[i];
+            if (p0 + i < p1) __builtin_nontemporal_store(v[i], pd);
             pd += axis_stride;
         }
     }
@@ -789,7 +790,7 @@ __global__ __launch_bounds__(256) void k_edt_signed(const int32_t* __restrict__ 
     const float d = o - in;
     const float sg = d > 0.0f ? 1.0f : (d < 0.0f ? -1.0f : 0.0f);
     const float half = (float)(0.5 * pitch);
-    out[i] = d - sg * half;
+    __builtin_nontemporal_store(d - sg * half, out + i);  // written once, read by the caller
 }
 
 struct LineCfg {

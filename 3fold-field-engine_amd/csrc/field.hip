@@ -528,38 +528,52 @@ __global__ __launch_bounds__(256) void k_tiles_ids(FieldDesc D, const int32_t* _
 __global__ __launch_bounds__(256) void k_reconstruct(const FieldDesc* __restrict__ descs, int r, float* __restrict__ dense,
                                                      uint8_t* __restrict__ occ)
 {
+    // One row (x, y) per wave at a time: the row's block row and in-tile
+    // offset are wave-uniform (scalar), the lanes walk z four voxels apiece
+    // with 32-bit arithmetic.
     const FieldDesc& D = descs[r];
     const int nz4 = (D.nz + 3) >> 2;
-    const int64_t n4 = (int64_t)D.nx * D.ny * nz4;
+    const int64_t rows = (int64_t)D.nx * D.ny;
     const int B = D.B;
     const bool vec = (D.nz & 3) == 0;
-    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n4; i += (int64_t)gridDim.x * 256) {
-        const int z0 = (int)(i % nz4) * 4;
-        const int64_t xy = i / nz4;
-        const int y = (int)(xy % D.ny), x = (int)(xy / D.ny);
-        const int64_t b = ((int64_t)(x / B) * D.nby + y / B) * D.nbz + z0 / B;
-        const int s = D.slot[b];
-        float v[4];
-        if (s >= 0 && s < D.tile_cap) {
-            const float* t = D.tiles + (int64_t)s * B * B * B + ((x % B) * B + (y % B)) * B + (z0 % B);
-            const float4 q = *reinterpret_cast<const float4*>(t);
-            v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
-        } else {
-            const float bg = D.kind[b] < 0 ? -FE_BIG : FE_BIG;
-            v[0] = v[1] = v[2] = v[3] = bg;
-        }
-        const int64_t o = xy * D.nz + z0;
-        if (vec) {
-            *reinterpret_cast<float4*>(dense + o) = make_float4(v[0], v[1], v[2], v[3]);
-            if (occ) {
-                const uint32_t m = (v[0] < 0.0f ? 1u : 0u) | (v[1] < 0.0f ? 1u << 8 : 0u) |
-                                   (v[2] < 0.0f ? 1u << 16 : 0u) | (v[3] < 0.0f ? 1u << 24 : 0u);
-                *reinterpret_cast<uint32_t*>(occ + o) = m;
+    const int lane = threadIdx.x & 63;
+    const int64_t first = __builtin_amdgcn_readfirstlane((int)(((int64_t)blockIdx.x * 256 + threadIdx.x) >> 6));
+    const int64_t stride = ((int64_t)gridDim.x * 256) >> 6;
+    for (int64_t xy = first; xy < rows; xy += stride) {
+        const int x = (int)(xy / D.ny), y = (int)(xy - (int64_t)x * D.ny);
+        const int64_t brow = ((int64_t)(x / B) * D.nby + y / B) * D.nbz;
+        const int toff = ((x % B) * B + (y % B)) * B;
+        const int64_t o0 = xy * D.nz;
+        for (int z4 = lane; z4 < nz4; z4 += 64) {
+            const int z0 = z4 * 4, bz = z0 / B;
+            const int64_t b = brow + bz;
+            const int s = D.slot[b];
+            float v[4];
+            if (s >= 0 && s < D.tile_cap) {
+                const float* t = D.tiles + (int64_t)s * B * B * B + toff + (z0 - bz * B);
+                const float4 q = *reinterpret_cast<const float4*>(t);
+                v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
+            } else {
+                const float bg = D.kind[b] < 0 ? -FE_BIG : FE_BIG;
+                v[0] = v[1] = v[2] = v[3] = bg;
             }
-        } else {
-            for (int k = 0; k < 4 && z0 + k < D.nz; ++k) {
-                dense[o + k] = v[k];
-                if (occ) occ[o + k] = v[k] < 0.0f ? 1 : 0;
+            const int64_t o = o0 + z0;
+            if (vec) {
+                // the dense field streams past the caches; the occupancy, which
+                // the EDT reads next, is stored normally and may stay cached
+                typedef float f4 __attribute__((ext_vector_type(4)));
+                const f4 w = {v[0], v[1], v[2], v[3]};
+                __builtin_nontemporal_store(w, reinterpret_cast<f4*>(dense + o));
+                if (occ) {
+                    const uint32_t m = (v[0] < 0.0f ? 1u : 0u) | (v[1] < 0.0f ? 1u << 8 : 0u) |
+                                       (v[2] < 0.0f ? 1u << 16 : 0u) | (v[3] < 0.0f ? 1u << 24 : 0u);
+                    *reinterpret_cast<uint32_t*>(occ + o) = m;
+                }
+            } else {
+                for (int k = 0; k < 4 && z0 + k < D.nz; ++k) {
+                    dense[o + k] = v[k];
+                    if (occ) occ[o + k] = v[k] < 0.0f ? 1 : 0;
+                }
             }
         }
     }
