@@ -315,11 +315,14 @@ __device__ __forceinline__ void block_box(const FieldDesc& D, int bi, int bj, in
 // superblock box (stored for the tile pass), then each wave takes groups of
 // 2x2x2 blocks, refines the list for the group's box and evaluates the
 // group's 64 (block, corner) points, one per lane.
-__global__ __launch_bounds__(256) void k_classify(const FieldDesc* __restrict__ descs, const int32_t* __restrict__ sb_prefix,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_classify(const FieldDesc* __restrict__ descs, const int32_t* __restrict__ sb_prefix,
                                                   int R, int total, uint32_t grid)
 {
+    // group lists hold up to FE_MAX_PRIMS / 4 entries (longer superblock lists
+    // are evaluated unrefined), which keeps the block's LDS small enough for
+    // eight waves per SIMD
     __shared__ uint16_t list[FE_MAX_PRIMS];
-    __shared__ uint16_t wlist[4][FE_MAX_PRIMS / 2];
+    __shared__ uint16_t wlist[4][FE_MAX_PRIMS / 4];
     __shared__ Clamp2 wtot[4];
     __shared__ int wcnt[4];
     const uint32_t item = xcd_tile(blockIdx.x, grid);
@@ -348,7 +351,7 @@ __global__ __launch_bounds__(256) void k_classify(const FieldDesc* __restrict__ 
         // would not fit the per-wave buffer
         const uint16_t* gl = list;
         int gn = n;
-        if (n <= FE_MAX_PRIMS / 2) {
+        if (n <= FE_MAX_PRIMS / 4) {
             gn = cull_wave(D.prims, list, n, cx, cy, cz, rad, D.slack, wl);
             if (gn > 8) gn = cull_wave(D.prims, wl, gn, cx, cy, cz, rad, D.slack, wl);
             gl = wl;
@@ -497,7 +500,7 @@ __device__ __forceinline__ void tile_eval(const FieldDesc& D, int64_t a, int bl,
 
 // Persistent: every wave walks all recipes' active blocks with a global stride.
 template <int KV>
-__global__ __launch_bounds__(256) void k_tiles(const FieldDesc* __restrict__ descs, int R)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_tiles(const FieldDesc* __restrict__ descs, int R)
 {
     __shared__ uint16_t wlist[4][FE_MAX_PRIMS / 2];
     const int64_t wave = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
