@@ -315,16 +315,18 @@ __device__ __forceinline__ void block_box(const FieldDesc& D, int bi, int bj, in
 // superblock box (stored for the tile pass), then each wave takes groups of
 // 2x2x2 blocks, refines the list for the group's box and evaluates the
 // group's 64 (block, corner) points, one per lane.
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_classify(const FieldDesc* __restrict__ descs, const int32_t* __restrict__ sb_prefix,
-                                                  int R, int total, uint32_t grid)
+// The lists live in dynamic LDS sized to the launch's longest recipe: the
+// superblock list holds lcap entries (>= every recipe's primitive count), each
+// wave's group list min(lcap, FE_MAX_PRIMS / 2) (a longer superblock list is
+// evaluated unrefined), so small recipes keep the block's LDS small.
+__global__ __launch_bounds__(256) void k_classify(const FieldDesc* __restrict__ descs, const int32_t* __restrict__ sb_prefix,
+                                                  int R, int total, uint32_t grid, int lcap)
 {
-    // group lists hold up to FE_MAX_PRIMS / 4 entries (longer superblock lists
-    // are evaluated unrefined), which keeps the block's LDS small enough for
-    // eight waves per SIMD
-    __shared__ uint16_t list[FE_MAX_PRIMS];
-    __shared__ uint16_t wlist[4][FE_MAX_PRIMS / 4];
+    extern __shared__ uint16_t lists_lds[];
     __shared__ Clamp2 wtot[4];
     __shared__ int wcnt[4];
+    const int wcap = min(lcap, FE_MAX_PRIMS / 2);
+    uint16_t* list = lists_lds;
     const uint32_t item = xcd_tile(blockIdx.x, grid);
     if (item >= (uint32_t)total) return;
     const int r = find_item(sb_prefix, R, item);
@@ -335,23 +337,26 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
     const int bx0 = sbx * SB, by0 = sby * SB, bz0 = sbz * SB;
     float cx, cy, cz, rad;
     block_box(D, bx0, by0, bz0, SB, cx, cy, cz, rad);
-    const int n = cull_box(D.prims, D.nprims, cx, cy, cz, rad, D.slack, list, wtot, wcnt);
+    // a recipe longer than the LDS lists (a launch given too small a max_prims)
+    // is evaluated without culling: every list then means all primitives
+    const bool fits = D.nprims <= lcap;
+    const int n = fits ? cull_box(D.prims, D.nprims, cx, cy, cz, rad, D.slack, list, wtot, wcnt) : D.nprims;
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-    if (tid == 0) D.list_len[sb] = n <= D.cap ? n : -1;
-    if (n <= D.cap)
+    if (tid == 0) D.list_len[sb] = (fits && n <= D.cap) ? n : -1;
+    if (fits && n <= D.cap)
         for (int j = tid; j < n; j += 256) D.lists[(int64_t)sb * D.cap + j] = list[j];
 
     const int G = SB / 2, ngroups = G * G * G;
     const float side = (float)B * D.pitch;
-    uint16_t* wl = wlist[wid];
+    uint16_t* wl = lists_lds + lcap + wid * wcap;
     for (int g = wid; g < ngroups; g += 4) {
         const int gx = bx0 + 2 * (g / (G * G)), gy = by0 + 2 * ((g / G) % G), gz = bz0 + 2 * (g % G);
         block_box(D, gx, gy, gz, 2, cx, cy, cz, rad);
         // a group list never exceeds the superblock list; fall back to it if it
         // would not fit the per-wave buffer
-        const uint16_t* gl = list;
+        const uint16_t* gl = fits ? list : nullptr;
         int gn = n;
-        if (n <= FE_MAX_PRIMS / 4) {
+        if (fits && n <= wcap) {
             gn = cull_wave(D.prims, list, n, cx, cy, cz, rad, D.slack, wl);
             if (gn > 8) gn = cull_wave(D.prims, wl, gn, cx, cy, cz, rad, D.slack, wl);
             gl = wl;
@@ -677,16 +682,23 @@ __global__ __launch_bounds__(256) void k_tiles_dense(const float* __restrict__ d
 // device descriptor table.  sb_prefix / ch_prefix: device int32[R+1]
 // exclusive prefixes of superblock and 1024-block chunk counts; chunk_count:
 // device int32[ch_prefix[R]].  Every recipe in one call shares one block edge.
+// LDS list capacity (entries) of a classify launch whose longest recipe has
+// max_prims primitives, and the dynamic LDS bytes it takes.
+static int classify_lcap(int max_prims) { return (max_prims + 7) & ~7; }
+static size_t classify_lds(int lcap) { return (size_t)2 * (lcap + 4 * std::min(lcap, FE_MAX_PRIMS / 2)); }
+
 extern "C" int fe_field_build(const void* descs, int R, int B, const int32_t* sb_prefix, int total_sb,
                               const int32_t* ch_prefix, int total_chunks, int32_t* chunk_count, int tile_waves,
-                              void* stream) noexcept
+                              int max_prims, void* stream) noexcept
 {
-    if (!descs || R < 1 || (B != 4 && B != 8 && B != 16) || total_sb < 1 || total_chunks < 1 || !chunk_count)
+    if (!descs || R < 1 || (B != 4 && B != 8 && B != 16) || total_sb < 1 || total_chunks < 1 || !chunk_count ||
+        max_prims < 1 || max_prims > FE_MAX_PRIMS)
         return FE_INVALID;
     hipStream_t st = static_cast<hipStream_t>(stream);
     const FieldDesc* d = static_cast<const FieldDesc*>(descs);
     const uint32_t g = round8(total_sb);
-    hipLaunchKernelGGL(k_classify, dim3(g), dim3(256), 0, st, d, sb_prefix, R, total_sb, g);
+    const int lcap = classify_lcap(max_prims);
+    hipLaunchKernelGGL(k_classify, dim3(g), dim3(256), classify_lds(lcap), st, d, sb_prefix, R, total_sb, g, lcap);
     FE_TRY(hipGetLastError());
     hipLaunchKernelGGL(k_compact_count, dim3(total_chunks), dim3(256), 0, st, d, ch_prefix, R, total_chunks, chunk_count);
     FE_TRY(hipGetLastError());
@@ -701,13 +713,16 @@ extern "C" int fe_field_build(const void* descs, int R, int B, const int32_t* sb
 
 // Classification only (the first launch of the plain per-call path).
 extern "C" int fe_field_classify(const void* descs, int R, int B, const int32_t* sb_prefix, int total_sb,
-                                 void* stream) noexcept
+                                 int max_prims, void* stream) noexcept
 {
-    if (!descs || R < 1 || (B != 4 && B != 8 && B != 16) || total_sb < 1) return FE_INVALID;
+    if (!descs || R < 1 || (B != 4 && B != 8 && B != 16) || total_sb < 1 || max_prims < 1 ||
+        max_prims > FE_MAX_PRIMS)
+        return FE_INVALID;
     hipStream_t st = static_cast<hipStream_t>(stream);
     const FieldDesc* d = static_cast<const FieldDesc*>(descs);
     const uint32_t g = round8(total_sb);
-    hipLaunchKernelGGL(k_classify, dim3(g), dim3(256), 0, st, d, sb_prefix, R, total_sb, g);
+    const int lcap = classify_lcap(max_prims);
+    hipLaunchKernelGGL(k_classify, dim3(g), dim3(256), classify_lds(lcap), st, d, sb_prefix, R, total_sb, g, lcap);
     FE_TRY(hipGetLastError());
     return FE_OK;
 }

@@ -120,6 +120,7 @@ class FieldPlan:
                 raise ValueError("A plan shares one block edge")
         self.B, self.SB = B, SUPER[B]
         self.items = [(m, R.validate(r)) for m, r in items]
+        self.max_prims = max(len(r) for _, r in self.items)
         n = len(items)
         caps = tile_caps if tile_caps is not None else [m.n_blocks for m, _ in items]
         b3 = B ** 3
@@ -183,7 +184,8 @@ class FieldPlan:
         """classify -> compaction -> tiles for every item, asynchronous on `stream`."""
         _check(_lib.lib().fe_field_build(_lib.ptr(self.desc), len(self.items), self.B, _lib.ptr(self.sb_prefix),
                                          self.total_sb, _lib.ptr(self.ch_prefix), self.total_ch,
-                                         _lib.ptr(self.chunk_count), self.tile_waves, _lib.stream_handle(stream)),
+                                         _lib.ptr(self.chunk_count), self.tile_waves, self.max_prims,
+                                         _lib.stream_handle(stream)),
                "field build")
 
     def _slice(self, pool, i, name):
@@ -261,7 +263,8 @@ def klassificera_och_evaluera_recept(meta, recipe, corrupt_index=False, device="
     plan = FieldPlan([(meta, recipe)], tile_caps=[0], device=device, corrupt=corrupt_index)
     L = _lib.lib()
     st = _lib.stream_handle(None)
-    _check(L.fe_field_classify(_lib.ptr(plan.desc), 1, plan.B, _lib.ptr(plan.sb_prefix), plan.total_sb, st),
+    _check(L.fe_field_classify(_lib.ptr(plan.desc), 1, plan.B, _lib.ptr(plan.sb_prefix), plan.total_sb,
+                               plan.max_prims, st),
            "classify")
     torch.cuda.synchronize(plan.device)
     kind = plan.kind(0).cpu().numpy()

@@ -123,11 +123,14 @@ int fe_octree_sample(const uint8_t *packed, const int32_t *branch_prefix, const 
  * exclusive prefixes of superblock and 1024-block chunk counts; chunk_count:
  * device int32[total_chunks] scratch. */
 int fe_field_build(const void *descs, int R, int B, const int32_t *sb_prefix, int total_sb, const int32_t *ch_prefix,
-                   int total_chunks, int32_t *chunk_count, int tile_waves, void *stream);
+                   int total_chunks, int32_t *chunk_count, int tile_waves, int max_prims, void *stream);
 
 /* k_classify_testfalt launch of faltkarna_v1.py:269-273 (kinds, corner_min_abs,
- * culled primitive lists) without compaction or tiles. */
-int fe_field_classify(const void *descs, int R, int B, const int32_t *sb_prefix, int total_sb, void *stream);
+ * culled primitive lists) without compaction or tiles.  max_prims (both
+ * calls): the longest recipe's primitive count (<= 4096), which sizes the
+ * culling lists in LDS. */
+int fe_field_classify(const void *descs, int R, int B, const int32_t *sb_prefix, int total_sb, int max_prims,
+                      void *stream);
 
 /* k_eval_tile_testfalt launch of faltkarna_v1.py:283-287 over host-compacted ids;
  * host_desc is a host copy of one descriptor whose tiles field names the output. */
