@@ -97,6 +97,59 @@ def test_random_recipes_bit_exact(fe, gpu):
         assert_same(fe.klassificera_och_evaluera_recept(meta, rec), ref, name + " plain")
 
 
+def op_order_recipe(fe, rng, n_prims, dims, pattern):
+    """Unions and subtractions of 3..15-voxel primitives in the central 40 % of
+    the box, intersections with large central spheres (they trim the solid
+    rather than empty it), in alternating single steps or in runs of random
+    length."""
+    R = fe.recipes
+    if pattern == "alternate":
+        ops = list(rng.choice([0, 1, 2], n_prims, p=[0.55, 0.3, 0.15]))
+    else:
+        ops = []
+        while len(ops) < n_prims:
+            ops += [int(rng.choice([0, 1, 2], p=[0.55, 0.3, 0.15]))] * int(rng.integers(1, 90))
+        ops = ops[:n_prims]
+    ops[0] = 0
+    d = np.array(dims, float)
+    recs = []
+    for op in ops:
+        if op == 2:
+            r = float(rng.uniform(0.22, 0.3)) * float(d.max())
+            recs.append((R.SPHERE, 2, tuple(d / 2 + rng.uniform(-3, 3, 3)), (r,)))
+        else:
+            k = int(rng.integers(0, 6))
+            c = tuple(rng.uniform(0.3 * d, 0.7 * d))
+            if k == R.SPHERE:
+                recs.append((k, op, c, (float(rng.uniform(3, 15)),)))
+            elif k == R.BOX:
+                recs.append((k, op, c, tuple(rng.uniform(2, 8, 3))))
+            elif k == R.TORUS:
+                big = float(rng.uniform(4, 12))
+                recs.append((k, op, c, (big, big * float(rng.uniform(0.2, 0.5)))))
+            else:
+                recs.append((k, op, c, (float(rng.uniform(2, 7)), float(rng.uniform(2, 10)))))
+    return R.make(recs)
+
+
+def test_random_op_orders_bit_exact(fe, gpu):
+    """Culling is by runs of same-op steps: recipes whose folds alternate
+    union, subtraction and intersection (single steps and runs of random
+    length) against the oracle's unculled fold."""
+    rng = np.random.default_rng(81)
+    for B, n_prims, pattern in ((8, 300, "runs"), (4, 200, "alternate"), (16, 400, "runs"), (8, 257, "alternate")):
+        dims = tuple(int(x) for x in rng.integers(110, 170, 3))
+        rec = op_order_recipe(fe, rng, n_prims, dims, pattern)
+        meta = fe.GridMeta(0.0, 0.0, 0.0, 1.0, *dims, block=B)
+        name = f"B{B}_P{n_prims}_{pattern}_{dims}"
+        print("case", name, flush=True)
+        ref = oracle_build(meta, rec)
+        assert 0 < len(ref[2]) < ref[0].size, name
+        plan = fe.FieldPlan([(meta, rec)])
+        plan.launch()
+        assert_same(plan.sparse_field(0), ref, name)
+
+
 def test_batched_plan_equals_single(fe, gpu):
     cases = [c for c in random_cases(fe) if c[1].block == 8]
     plan = fe.FieldPlan([(m, r) for _, m, r in cases])
