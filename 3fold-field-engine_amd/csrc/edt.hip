@@ -367,7 +367,8 @@ __device__ __forceinline__ bool keep_mid(int sa, int32_t Fa, int sb, int32_t Fb,
 // contributes sentinel + (p-s)^2 >= sentinel, with equality only at its own
 // position, so the result is min(envelope of the other sites, sentinel).
 // LDS: kmax*W*T packed entries (kmax = the largest chunk length of the launch)
-// plus four bytes per thread and twelve bytes per line.  ZSUM: the y pass
+// plus four bytes per thread, eight bytes per line and (COMPACT) a start bit
+// per position of the longest line.  ZSUM: the y pass
 // reading z summaries (k_edt_zsum) instead of int32 planes.
 template <int W, int T, int CMAX, bool COMPACT, bool ZSUM>
 __global__ __launch_bounds__(W* T) void k_edt_lines(const int32_t* __restrict__ src, int32_t* __restrict__ dst,
@@ -402,10 +403,14 @@ __global__ __launch_bounds__(W* T) void k_edt_lines(const int32_t* __restrict__ 
     uint8_t* rsb = lsb + NT;                                  // [NT] nearest nonempty chunk >= c
     int32_t* lval = reinterpret_cast<int32_t*>(rsb + NT);     // [W] first value of the line
     int32_t* lflat = lval + W;                                // [W] line is constant
+    uint32_t* smask = reinterpret_cast<uint32_t*>(lflat + W);  // [W][MW] start positions (COMPACT)
 
     const int tid = threadIdx.x;
     const int line = tid % W, c = tid / W;
     const int C = (L + T - 1) / T;
+    const int MW = (L + 31) >> 5;  // mask words per line
+    if constexpr (COMPACT)
+        for (int i = tid; i < W * MW; i += NT) smask[i] = 0u;  // published by the barriers below
     const int p0 = min(c * C, L), p1 = min(p0 + C, L);
     const bool active = line < nlines;
     // slot of chunk b of this line: rotated within the W-slot row so that the
@@ -604,9 +609,10 @@ __global__ __launch_bounds__(W* T) void k_edt_lines(const int32_t* __restrict__ 
     if constexpr (COMPACT) {
         // 3. Each thread computes, for the surviving entries of its chunk, the
         // first integer position where the entry is the envelope minimum (one
-        // exact floor division with the neighbouring envelope entry) and
-        // writes the entry there; every position then takes the last entry
-        // written at or before it.  No data-dependent loops over positions.
+        // exact floor division with the neighbouring envelope entry), writes
+        // the entry there and marks the position in the line's start mask;
+        // every position then takes the entry at the last start at or before
+        // it.  No data-dependent loops over positions.
         const int lo = blo[me], cnt = bhi[me] - lo;
         uint32_t pred = 0xffffffffu, succ = 0xffffffffu;
         if (busy) {
@@ -621,15 +627,9 @@ __global__ __launch_bounds__(W* T) void k_edt_lines(const int32_t* __restrict__ 
             if (b < T) succ = stk[blo[slot(b)] * NT + slot(b)];
         }
         __syncthreads();  // the hull stacks are free from here on
-        // at[p*W + line]: entry whose interval starts at position p
+        // at[p*W + line]: entry whose interval starts at position p; the
+        // line's start positions are bits of smask (zeroed at kernel start)
         uint32_t* at = stk;
-        uint32_t* tail = reinterpret_cast<uint32_t*>(blo);  // [NT] last entry at or before a chunk's end
-        if (busy) {
-#pragma unroll
-            for (int i = 0; i < CMAX; ++i)
-                if (p0 + i < p1) at[(p0 + i) * W + line] = 0xffffffffu;
-        }
-        __syncthreads();
         // first integer p where entry b is strictly below entry a (a left of b):
         // floor((Fb - Fa) / (2 (sb - sa))) + 1, exact: |Fb - Fa| < 2^22 and the
         // reciprocal estimate is within 0.4 of the quotient, so one integer
@@ -642,6 +642,7 @@ __global__ __launch_bounds__(W* T) void k_edt_lines(const int32_t* __restrict__ 
             q += r < 0 ? -1 : (r >= d ? 1 : 0);
             return q + 1;
         };
+        uint32_t* lmask = smask + line * MW;
         if (busy && cnt > 0) {
             int xlo = pred == 0xffffffffu ? 0 : min(max(start(pred, (uint32_t)v[0]), 0), L);
 #pragma unroll
@@ -650,33 +651,30 @@ __global__ __launch_bounds__(W* T) void k_edt_lines(const int32_t* __restrict__ 
                     const uint32_t e = (uint32_t)v[i];
                     const uint32_t nx = i + 1 < cnt ? (uint32_t)v[min(i + 1, CMAX - 1)] : succ;
                     const int xhi = nx == 0xffffffffu ? L : min(max(start(e, nx), 0), L);
-                    if (xlo < xhi) at[xlo * W + line] = e;
+                    if (xlo < xhi) {
+                        at[xlo * W + line] = e;
+                        atomicOr(lmask + (xlo >> 5), 1u << (xlo & 31));
+                    }
                     xlo = max(xlo, xhi);
                 }
         }
         __syncthreads();
-        uint32_t last = 0xffffffffu;
-        if (busy) {
-#pragma unroll
-            for (int i = 0; i < CMAX; ++i)
-                if (p0 + i < p1) {
-                    const uint32_t e = at[(p0 + i) * W + line];
-                    last = e != 0xffffffffu ? e : last;
-                    v[i] = (int32_t)last;
-                }
-        }
-        tail[me] = last;
-        __syncthreads();
         if (busy && p0 < p1) {
-            uint32_t carry = 0xffffffffu;  // last entry before p0 (position 0 always holds one)
-            if ((uint32_t)v[0] == 0xffffffffu)
-                for (int b = c - 1; b >= 0 && carry == 0xffffffffu; --b) carry = tail[slot(b)];
+            // the entry in force at p0 is the one starting at the last start
+            // at or before p0 (a busy line's envelope is nonempty and its first
+            // entry starts at 0); later starts within the chunk come from a
+            // 64-bit window of the mask (chunks hold <= 32 positions)
+            int w = p0 >> 5;
+            uint32_t bits = lmask[w] & ((2u << (p0 & 31)) - 1u);
+            while (!bits && w > 0) bits = lmask[--w];
+            uint32_t e = bits ? at[((w << 5) + 31 - __builtin_clz(bits)) * W + line] : 0xffffffffu;
+            const int w0 = p0 >> 5;
+            const uint64_t win = (uint64_t)lmask[w0] | ((uint64_t)(w0 + 1 < MW ? lmask[w0 + 1] : 0u) << 32);
 #pragma unroll
             for (int i = 0; i < CMAX; ++i) {
                 const int p = p0 + i;
                 if (p < p1) {
-                    uint32_t e = (uint32_t)v[i];
-                    if (e == 0xffffffffu) e = carry;
+                    if (i > 0 && ((win >> (p - (w0 << 5))) & 1ull)) e = at[p * W + line];
                     const int es = (int)(e & 1023u);
                     const int32_t eF = (int32_t)(e >> 10);
                     const int32_t g = eF - (int32_t)__umul24(2u * p, es) + (int32_t)__umul24(p, p);
@@ -818,7 +816,8 @@ hipError_t launch_lines_t(const int32_t* src, int32_t* dst, const EdtVol* table,
     const int kmax = (Lmax + T - 1) / T;  // <= CMAX by the choice of shape
     // hull stacks; with COMPACT the same words later hold Lmax rows of W start entries
     const int region = kmax * W * T;
-    const size_t lds = (size_t)region * 4 + (size_t)W * T * 4 + (size_t)W * 12;
+    const size_t lds = (size_t)region * 4 + (size_t)W * T * 4 + (size_t)W * 8 +
+                       (COMPACT ? (size_t)W * ((Lmax + 31) / 32) * 4 : 0);
     const void* fn = reinterpret_cast<const void*>(&k_edt_lines<W, T, CMAX, COMPACT, ZSUM>);
     if (lds > 65536) {
         hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
