@@ -295,6 +295,22 @@ __device__ int cull_wave(const Prim* __restrict__ prims, const uint16_t* src, in
     return n;
 }
 
+// Box (centre, half diagonal) of blocks [bi, bi+nx) x [bj, bj+ny) x [bk, bk+nz),
+// corners included.
+__device__ __forceinline__ void block_box3(const FieldDesc& D, int bi, int bj, int bk, int nx, int ny, int nz,
+                                           float& cx, float& cy, float& cz, float& rad)
+{
+    const int B = D.B;
+    const float lx = D.x0 + (float)(bi * B) * D.pitch, ux = D.x0 + (float)((bi + nx) * B) * D.pitch;
+    const float ly = D.y0 + (float)(bj * B) * D.pitch, uy = D.y0 + (float)((bj + ny) * B) * D.pitch;
+    const float lz = D.z0 + (float)(bk * B) * D.pitch, uz = D.z0 + (float)((bk + nz) * B) * D.pitch;
+    cx = 0.5f * (lx + ux);
+    cy = 0.5f * (ly + uy);
+    cz = 0.5f * (lz + uz);
+    const float ex = ux - lx, ey = uy - ly, ez = uz - lz;
+    rad = 0.5f * sqrtf(ex * ex + ey * ey + ez * ez) * 1.0001f;
+}
+
 // Box (centre, half diagonal) of blocks [b0, b0+nb) per axis, corners included.
 __device__ __forceinline__ void block_box(const FieldDesc& D, int bi, int bj, int bk, int nb, float& cx, float& cy,
                                           float& cz, float& rad)
@@ -313,8 +329,8 @@ __device__ __forceinline__ void block_box(const FieldDesc& D, int bi, int bj, in
 // ---------------------------------------------------------------- classify
 // One workgroup per superblock: the workgroup culls the recipe for the
 // superblock box (stored for the tile pass), then each wave takes groups of
-// 2x2x2 blocks, refines the list for the group's box and evaluates the
-// group's 64 (block, corner) points, one per lane.
+// 4x2x2 blocks, refines the list for the group's box and evaluates the
+// group's corner points (shared lattice points once when exact).
 // The lists live in dynamic LDS sized to the launch's longest recipe: the
 // superblock list holds lcap entries (>= every recipe's primitive count), each
 // wave's group list min(lcap, FE_MAX_PRIMS / 2) (a longer superblock list is
@@ -346,12 +362,18 @@ __global__ __launch_bounds__(256) void k_classify(const FieldDesc* __restrict__ 
     if (fits && n <= D.cap)
         for (int j = tid; j < n; j += 256) D.lists[(int64_t)sb * D.cap + j] = list[j];
 
-    const int G = SB / 2, ngroups = G * G * G;
+    // Groups of 4 x 2 x 2 blocks, one per wave per pass.  The corner "+1" of a
+    // block, origin + B*h, is usually the next block's origin bit for bit;
+    // the wave checks that for its group and then evaluates the 5 x 3 x 3
+    // lattice of block origins once (lane l = point l) and gathers each
+    // block's eight corners by lane shuffles.  Otherwise (origin or pitch
+    // rounding) every corner is evaluated as its own point, 8 blocks per pass.
+    const int GX = SB / 4, GY = SB / 2, GZ = SB / 2, ngroups = GX * GY * GZ;
     const float side = (float)B * D.pitch;
     uint16_t* wl = lists_lds + lcap + wid * wcap;
     for (int g = wid; g < ngroups; g += 4) {
-        const int gx = bx0 + 2 * (g / (G * G)), gy = by0 + 2 * ((g / G) % G), gz = bz0 + 2 * (g % G);
-        block_box(D, gx, gy, gz, 2, cx, cy, cz, rad);
+        const int gx = bx0 + 4 * (g / (GY * GZ)), gy = by0 + 2 * ((g / GZ) % GY), gz = bz0 + 2 * (g % GZ);
+        block_box3(D, gx, gy, gz, 4, 2, 2, cx, cy, cz, rad);
         // a group list never exceeds the superblock list; fall back to it if it
         // would not fit the per-wave buffer
         const uint16_t* gl = fits ? list : nullptr;
@@ -361,27 +383,52 @@ __global__ __launch_bounds__(256) void k_classify(const FieldDesc* __restrict__ 
             if (gn > 8) gn = cull_wave(D.prims, wl, gn, cx, cy, cz, rad, D.slack, wl);
             gl = wl;
         }
-        const int bl = lane >> 3, c = lane & 7;
-        const int bi = gx + ((bl >> 2) & 1), bj = gy + ((bl >> 1) & 1), bk = gz + (bl & 1);
-        const float ox = D.x0 + (float)(bi * B) * D.pitch;
-        const float oy = D.y0 + (float)(bj * B) * D.pitch;
-        const float oz = D.z0 + (float)(bk * B) * D.pitch;
-        float x = ox + (float)(c & 1) * side, y = oy + (float)((c >> 1) & 1) * side, z = oz + (float)((c >> 2) & 1) * side;
-        float v;
-        eval_points<1>(D.prims, gl, gn, &x, &y, &z, &v);
-        float mn = fabsf(v);
-        int pos = v >= 0.0f, neg = !pos;
-#pragma unroll
-        for (int s = 1; s < 8; s <<= 1) {
-            mn = sel_min(mn, __shfl_xor(mn, s));
-            pos |= __shfl_xor(pos, s);
-            neg |= __shfl_xor(neg, s);
+        const int la = lane / 9, lb = (lane / 3) % 3, lc = lane % 3;
+        const float lx = D.x0 + (float)((gx + la) * B) * D.pitch;
+        const float ly = D.y0 + (float)((gy + lb) * B) * D.pitch;
+        const float lz = D.z0 + (float)((gz + lc) * B) * D.pitch;
+        auto same = [](float p, float q) { return __float_as_uint(p) == __float_as_uint(q); };
+        const bool ok_x = la == 0 || same((D.x0 + (float)((gx + la - 1) * B) * D.pitch) + side, lx);
+        const bool ok_y = lb == 0 || same((D.y0 + (float)((gy + lb - 1) * B) * D.pitch) + side, ly);
+        const bool ok_z = lc == 0 || same((D.z0 + (float)((gz + lc - 1) * B) * D.pitch) + side, lz);
+        const uint64_t lat = (1ull << 45) - 1;
+        const bool lattice = (__ballot(ok_x && ok_y && ok_z) & lat) == lat;
+        float lv = 0.0f;
+        if (lattice) {
+            float x = lx, y = ly, z = lz;
+            eval_points<1>(D.prims, gl, gn, &x, &y, &z, &lv);
         }
-        mn = sel_min(1.0e30f, mn);
-        if (c == 0 && bi < D.nbx && bj < D.nby && bk < D.nbz) {
-            const int64_t b = ((int64_t)bi * D.nby + bj) * D.nbz + bk;
-            D.cma[b] = mn;
-            D.kind[b] = (pos && neg) ? 0 : (mn <= D.margin ? 0 : (neg ? -1 : 1));
+        const int c = lane & 7;
+        for (int half = 0; half < 2; ++half) {
+            const int bl = half * 8 + (lane >> 3);
+            const int ba = bl >> 2, bb = (bl >> 1) & 1, bc = bl & 1;
+            const int bi = gx + ba, bj = gy + bb, bk = gz + bc;
+            float v;
+            if (lattice) {
+                const int idx = ((ba + (c & 1)) * 3 + bb + ((c >> 1) & 1)) * 3 + bc + ((c >> 2) & 1);
+                v = __shfl(lv, idx);
+            } else {
+                const float ox = D.x0 + (float)(bi * B) * D.pitch;
+                const float oy = D.y0 + (float)(bj * B) * D.pitch;
+                const float oz = D.z0 + (float)(bk * B) * D.pitch;
+                float x = ox + (float)(c & 1) * side, y = oy + (float)((c >> 1) & 1) * side,
+                      z = oz + (float)((c >> 2) & 1) * side;
+                eval_points<1>(D.prims, gl, gn, &x, &y, &z, &v);
+            }
+            float mn = fabsf(v);
+            int pos = v >= 0.0f, neg = !pos;
+#pragma unroll
+            for (int s = 1; s < 8; s <<= 1) {
+                mn = sel_min(mn, __shfl_xor(mn, s));
+                pos |= __shfl_xor(pos, s);
+                neg |= __shfl_xor(neg, s);
+            }
+            mn = sel_min(1.0e30f, mn);
+            if (c == 0 && bi < D.nbx && bj < D.nby && bk < D.nbz) {
+                const int64_t b = ((int64_t)bi * D.nby + bj) * D.nbz + bk;
+                D.cma[b] = mn;
+                D.kind[b] = (pos && neg) ? 0 : (mn <= D.margin ? 0 : (neg ? -1 : 1));
+            }
         }
     }
 }
@@ -503,17 +550,23 @@ __device__ __forceinline__ void tile_eval(const FieldDesc& D, int64_t a, int bl,
     }
 }
 
-// Persistent: every wave walks all recipes' active blocks with a global stride.
+// Persistent: the active blocks of all recipes, concatenated in recipe order,
+// are dealt round-robin over the waves (tile g of the concatenation goes to
+// wave g mod nwaves), so a recipe with fewer tiles than waves does not land
+// on the same low-numbered waves as every other recipe.
 template <int KV>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_tiles(const FieldDesc* __restrict__ descs, int R)
 {
     __shared__ uint16_t wlist[4][FE_MAX_PRIMS / 2];
     const int64_t wave = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     const int64_t nwaves = (int64_t)gridDim.x * 4;
+    int64_t base = 0;  // tiles of the recipes before r, mod nwaves
     for (int r = 0; r < R; ++r) {
         const FieldDesc& D = descs[r];
         const int64_t na = min((int64_t)D.counts[0], D.tile_cap);
-        for (int64_t a = wave; a < na; a += nwaves) tile_eval<KV>(D, a, D.ids[a], wlist[threadIdx.x >> 6]);
+        const int64_t a0 = wave >= base ? wave - base : wave - base + nwaves;
+        for (int64_t a = a0; a < na; a += nwaves) tile_eval<KV>(D, a, D.ids[a], wlist[threadIdx.x >> 6]);
+        base = (base + na) % nwaves;
     }
 }
 
