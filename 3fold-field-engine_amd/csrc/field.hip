@@ -109,18 +109,46 @@ __device__ __forceinline__ float fold(int op, float f, float v)
 }
 
 // Evaluates K points over a primitive list (list == nullptr: all primitives).
-// Primitive parameters are wave-uniform scalar loads shared by the K points.
+// Primitive parameters are wave-uniform scalar loads shared by the K points:
+// the list (wave-uniform, n entries) is read 64 entries at a time, one per
+// lane, and broadcast with readlane; the records are read through the
+// constant address space, so uniform indices become scalar loads and the
+// kind switch scalar branches, and the next record's load is issued before
+// the current record is evaluated.
+typedef const __attribute__((address_space(4))) Prim ConstPrim;
+__device__ __forceinline__ Prim load_prim(ConstPrim* cp, int idx)
+{
+    ConstPrim* q = cp + idx;
+    Prim p;
+    p.kind = q->kind;
+    p.op = q->op;
+    p.cx = q->cx;
+    p.cy = q->cy;
+    p.cz = q->cz;
+    p.a = q->a;
+    p.b = q->b;
+    p.c = q->c;
+    return p;
+}
 template <int K>
 __device__ __forceinline__ void eval_points(const Prim* __restrict__ prims, const uint16_t* list, int n, const float* x,
                                             const float* y, const float* z, float* f)
 {
-    for (int j = 0; j < n; ++j) {
-        int idx = list ? __builtin_amdgcn_readfirstlane((int)list[j]) : j;
-        const Prim p = prims[idx];
+    ConstPrim* cp = (ConstPrim*)prims;
+    const int lane = threadIdx.x & 63;
+    n = __builtin_amdgcn_readfirstlane(n);
+    for (int jb = 0; jb < n; jb += 64) {
+        const int cnt = min(64, n - jb);
+        const int mine = list ? (lane < cnt ? (int)list[jb + lane] : 0) : jb + lane;
+        Prim pn = load_prim(cp, __builtin_amdgcn_readlane(mine, 0));
+        for (int t = 0; t < cnt; ++t) {
+            const Prim p = pn;
+            if (t + 1 < cnt) pn = load_prim(cp, __builtin_amdgcn_readlane(mine, t + 1));
 #pragma unroll
-        for (int k = 0; k < K; ++k) {
-            float v = prim_eval(p.kind, p.cx, p.cy, p.cz, p.a, p.b, p.c, x[k], y[k], z[k]);
-            f[k] = j == 0 ? v : fold(p.op, f[k], v);
+            for (int k = 0; k < K; ++k) {
+                const float v = prim_eval(p.kind, p.cx, p.cy, p.cz, p.a, p.b, p.c, x[k], y[k], z[k]);
+                f[k] = jb + t == 0 ? v : fold(p.op, f[k], v);
+            }
         }
     }
 }
@@ -357,7 +385,7 @@ __global__ __launch_bounds__(256) void k_classify(const FieldDesc* __restrict__ 
     // is evaluated without culling: every list then means all primitives
     const bool fits = D.nprims <= lcap;
     const int n = fits ? cull_box(D.prims, D.nprims, cx, cy, cz, rad, D.slack, list, wtot, wcnt) : D.nprims;
-    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63, wid = __builtin_amdgcn_readfirstlane(tid >> 6);
     if (tid == 0) D.list_len[sb] = (fits && n <= D.cap) ? n : -1;
     if (fits && n <= D.cap)
         for (int j = tid; j < n; j += 256) D.lists[(int64_t)sb * D.cap + j] = list[j];
@@ -558,14 +586,15 @@ template <int KV>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_tiles(const FieldDesc* __restrict__ descs, int R)
 {
     __shared__ uint16_t wlist[4][FE_MAX_PRIMS / 2];
-    const int64_t wave = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
+    const int64_t wave = (int64_t)blockIdx.x * 4 + wv;
     const int64_t nwaves = (int64_t)gridDim.x * 4;
     int64_t base = 0;  // tiles of the recipes before r, mod nwaves
     for (int r = 0; r < R; ++r) {
         const FieldDesc& D = descs[r];
         const int64_t na = min((int64_t)D.counts[0], D.tile_cap);
         const int64_t a0 = wave >= base ? wave - base : wave - base + nwaves;
-        for (int64_t a = a0; a < na; a += nwaves) tile_eval<KV>(D, a, D.ids[a], wlist[threadIdx.x >> 6]);
+        for (int64_t a = a0; a < na; a += nwaves) tile_eval<KV>(D, a, D.ids[a], wlist[wv]);
         base = (base + na) % nwaves;
     }
 }
@@ -576,9 +605,10 @@ template <int KV>
 __global__ __launch_bounds__(256) void k_tiles_ids(FieldDesc D, const int32_t* __restrict__ ids, int64_t na)
 {
     __shared__ uint16_t wlist[4][FE_MAX_PRIMS / 2];
-    const int64_t a = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
+    const int64_t a = (int64_t)blockIdx.x * 4 + wv;
     if (a >= na) return;
-    tile_eval<KV>(D, a, ids[a], wlist[threadIdx.x >> 6]);
+    tile_eval<KV>(D, a, ids[a], wlist[wv]);
 }
 
 // ---------------------------------------------------------------- reconstruct / checksum
