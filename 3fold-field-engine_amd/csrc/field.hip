@@ -789,7 +789,9 @@ extern "C" int fe_field_build(const void* descs, int R, int B, const int32_t* sb
     FE_TRY(hipGetLastError());
     const int blocks = tile_waves > 0 ? (tile_waves + 3) / 4 : 2048;
     if (B == 4) hipLaunchKernelGGL(k_tiles<1>, dim3(blocks), dim3(256), 0, st, d, R);
-    else hipLaunchKernelGGL(k_tiles<4>, dim3(blocks), dim3(256), 0, st, d, R);  // 4 voxels per lane: 8 waves/SIMD
+    // 4 voxels per lane per pass fits 8 waves/SIMD without spills (8 voxels
+    // per lane spills VGPRs to scratch under that cap)
+    else hipLaunchKernelGGL(k_tiles<4>, dim3(blocks), dim3(256), 0, st, d, R);
     FE_TRY(hipGetLastError());
     return FE_OK;
 }

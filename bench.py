@@ -477,10 +477,11 @@ def main():
     ap.add_argument("--quick", action="store_true", help="single-field step only")
     ap.add_argument("--batch-count", type=int, default=256)
     ap.add_argument("--serve-count", type=int, default=48)
-    ap.add_argument("--only", choices=["edt-heavy", "kernels", "mesh", "ikarus"], default=None,
+    ap.add_argument("--only", choices=["edt-heavy", "kernels", "mesh", "ikarus", "batch"], default=None,
                     help="edt-heavy: the thin-feature EDT section only; kernels: each hot kernel of the "
                          "512^3 step three times plus a 1 GiB copy (for rocprofv3 counter runs); "
-                         "mesh: the mesh-stage section only; ikarus: the IKARUS-tree section only")
+                         "mesh: the mesh-stage section only; ikarus: the IKARUS-tree section only; "
+                         "batch: the batched uneven recipes only")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU: the block-sparse build and the EDT run on one "
@@ -500,6 +501,9 @@ def main():
         return 0
     if args.only == "ikarus":
         print(json.dumps({"ikarus": bench_ikarus(fe, torch, cpu=not args.no_cpu)}))
+        return 0
+    if args.only == "batch":
+        print(json.dumps({"batched": bench_batch(fe, torch, args.batch_count, 32)}))
         return 0
     n = args.n
     rec = fe.recipes.large_field_512(seed=2) if n == 512 else fe.recipes.mixed_recipe(np.random.default_rng(2), 1024, n)
