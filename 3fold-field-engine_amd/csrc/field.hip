@@ -529,12 +529,14 @@ __global__ __launch_bounds__(256) void k_compact_write(const FieldDesc* __restri
 // One wave evaluates one B^3 tile: it refines the superblock list for the
 // block's own box (exact, as above) and evaluates KV voxels per lane per pass.
 // `lin` is the block whose voxels are evaluated (the planted fault shifts it,
-// and then evaluates the whole recipe), `bl` the listed block.
-template <int KV>
+// and then evaluates the whole recipe), `bl` the listed block.  B (= D.B) is a
+// compile-time edge: a lane's KV points of a pass then differ only in x (B = 8,
+// 16) and the compiler shares their y and z terms in every primitive formula.
+template <int B, int KV>
 __device__ __forceinline__ void tile_eval(const FieldDesc& D, int64_t a, int bl, uint16_t* wl)
 {
     const int lane = threadIdx.x & 63;
-    const int B = D.B, b3 = B * B * B;
+    constexpr int b3 = B * B * B;
     const int lin = bl + (D.corrupt ? 1 : 0);
     const int bk = lin % D.nbz, bj = (lin / D.nbz) % D.nby, bi = lin / (D.nbz * D.nby);
     const uint16_t* list = nullptr;
@@ -582,7 +584,7 @@ __device__ __forceinline__ void tile_eval(const FieldDesc& D, int64_t a, int bl,
 // are dealt round-robin over the waves (tile g of the concatenation goes to
 // wave g mod nwaves), so a recipe with fewer tiles than waves does not land
 // on the same low-numbered waves as every other recipe.
-template <int KV>
+template <int B, int KV>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_tiles(const FieldDesc* __restrict__ descs, int R)
 {
     __shared__ uint16_t wlist[4][FE_MAX_PRIMS / 2];
@@ -594,21 +596,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
         const FieldDesc& D = descs[r];
         const int64_t na = min((int64_t)D.counts[0], D.tile_cap);
         const int64_t a0 = wave >= base ? wave - base : wave - base + nwaves;
-        for (int64_t a = a0; a < na; a += nwaves) tile_eval<KV>(D, a, D.ids[a], wlist[wv]);
+        for (int64_t a = a0; a < na; a += nwaves) tile_eval<B, KV>(D, a, D.ids[a], wlist[wv]);
         base = (base + na) % nwaves;
     }
 }
 
 // Tiles for an explicit host-provided id list (the plain per-call path); uses
 // the culled lists that the classify launch stored.
-template <int KV>
+template <int B, int KV>
 __global__ __launch_bounds__(256) void k_tiles_ids(FieldDesc D, const int32_t* __restrict__ ids, int64_t na)
 {
     __shared__ uint16_t wlist[4][FE_MAX_PRIMS / 2];
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
     const int64_t a = (int64_t)blockIdx.x * 4 + wv;
     if (a >= na) return;
-    tile_eval<KV>(D, a, ids[a], wlist[wv]);
+    tile_eval<B, KV>(D, a, ids[a], wlist[wv]);
 }
 
 // ---------------------------------------------------------------- reconstruct / checksum
@@ -788,10 +790,12 @@ extern "C" int fe_field_build(const void* descs, int R, int B, const int32_t* sb
     hipLaunchKernelGGL(k_compact_write, dim3(total_chunks), dim3(256), 0, st, d, ch_prefix, R, total_chunks, chunk_count);
     FE_TRY(hipGetLastError());
     const int blocks = tile_waves > 0 ? (tile_waves + 3) / 4 : 2048;
-    if (B == 4) hipLaunchKernelGGL(k_tiles<1>, dim3(blocks), dim3(256), 0, st, d, R);
-    // 4 voxels per lane per pass fits 8 waves/SIMD without spills (8 voxels
-    // per lane spills VGPRs to scratch under that cap)
-    else hipLaunchKernelGGL(k_tiles<4>, dim3(blocks), dim3(256), 0, st, d, R);
+    if (B == 4) hipLaunchKernelGGL((k_tiles<4, 1>), dim3(blocks), dim3(256), 0, st, d, R);
+    // B = 8: one pass of 8 voxels per lane; with the edge a compile-time
+    // constant the 8 points share y and z and fit 64 VGPRs (8 waves/SIMD)
+    // without spills
+    else if (B == 8) hipLaunchKernelGGL((k_tiles<8, 8>), dim3(blocks), dim3(256), 0, st, d, R);
+    else hipLaunchKernelGGL((k_tiles<16, 4>), dim3(blocks), dim3(256), 0, st, d, R);
     FE_TRY(hipGetLastError());
     return FE_OK;
 }
@@ -822,8 +826,10 @@ extern "C" int fe_field_tiles_ids(const void* host_desc, const int32_t* ids, int
     std::memcpy(&D, host_desc, sizeof D);
     hipStream_t st = static_cast<hipStream_t>(stream);
     const unsigned blocks = (unsigned)((n_active + 3) / 4);
-    if (D.B == 4) hipLaunchKernelGGL(k_tiles_ids<1>, dim3(blocks), dim3(256), 0, st, D, ids, n_active);
-    else hipLaunchKernelGGL(k_tiles_ids<4>, dim3(blocks), dim3(256), 0, st, D, ids, n_active);
+    if (D.B == 4) hipLaunchKernelGGL((k_tiles_ids<4, 1>), dim3(blocks), dim3(256), 0, st, D, ids, n_active);
+    else if (D.B == 8) hipLaunchKernelGGL((k_tiles_ids<8, 4>), dim3(blocks), dim3(256), 0, st, D, ids, n_active);
+    else if (D.B == 16) hipLaunchKernelGGL((k_tiles_ids<16, 4>), dim3(blocks), dim3(256), 0, st, D, ids, n_active);
+    else return FE_INVALID;
     FE_TRY(hipGetLastError());
     return FE_OK;
 }
