@@ -66,6 +66,12 @@ static_assert(sizeof(FieldDesc) == 160, "descriptor layout");
 // ---------------------------------------------------------------- primitives
 // Same operation order as oracle/fe_oracle.c or_prim_eval (IKARUS emitted
 // fp32 source, FE/ikarus_v1/eval_warp.py:182-224; FE/faltkarna_v1.py:91-104).
+// FAST: square roots by the bare v_sqrt_f32 (within one ulp) instead of the
+// correctly rounded sqrtf; used only for the centres of culling intervals,
+// whose slack is far wider than that error.
+template <bool FAST = false>
+__device__ __forceinline__ float root(float x) { return FAST ? __builtin_amdgcn_sqrtf(x) : sqrtf(x); }
+template <bool FAST = false>
 __device__ __forceinline__ float prim_eval(int kind, float cx, float cy, float cz, float a, float b, float c, float x,
                                            float y, float z)
 {
@@ -74,14 +80,14 @@ __device__ __forceinline__ float prim_eval(int kind, float cx, float cy, float c
     case 0: {
         float s = lx * lx + ly * ly;
         s = s + lz * lz;
-        return sqrtf(s) - a;
+        return root<FAST>(s) - a;
     }
     case 1: {
         float qx = fabsf(lx) - a, qy = fabsf(ly) - b, qz = fabsf(lz) - c;
         float ox = sel_max(qx, 0.0f), oy = sel_max(qy, 0.0f), oz = sel_max(qz, 0.0f);
         float s = ox * ox + oy * oy;
         s = s + oz * oz;
-        return sqrtf(s) + sel_min(sel_max(qx, sel_max(qy, qz)), 0.0f);
+        return root<FAST>(s) + sel_min(sel_max(qx, sel_max(qy, qz)), 0.0f);
     }
     case 2:
     case 3:
@@ -90,16 +96,16 @@ __device__ __forceinline__ float prim_eval(int kind, float cx, float cy, float c
         if (kind == 4) { r2 = lx * lx + ly * ly; ax = lz; }
         else if (kind == 3) { r2 = lx * lx + lz * lz; ax = ly; }
         else { r2 = ly * ly + lz * lz; ax = lx; }
-        float qx = sqrtf(r2) - a, qy = fabsf(ax) - b;
+        float qx = root<FAST>(r2) - a, qy = fabsf(ax) - b;
         float ox = sel_max(qx, 0.0f), oy = sel_max(qy, 0.0f);
-        return sqrtf(ox * ox + oy * oy) + sel_min(sel_max(qx, qy), 0.0f);
+        return root<FAST>(ox * ox + oy * oy) + sel_min(sel_max(qx, qy), 0.0f);
     }
     case 5: {
-        float qx = sqrtf(lx * lx + ly * ly) - a;
-        return sqrtf(qx * qx + lz * lz) - b;
+        float qx = root<FAST>(lx * lx + ly * ly) - a;
+        return root<FAST>(qx * qx + lz * lz) - b;
     }
     default:  // 6: infinite z-cylinder
-        return sqrtf(lx * lx + ly * ly) - a;
+        return root<FAST>(lx * lx + ly * ly) - a;
     }
 }
 
@@ -180,11 +186,13 @@ __device__ __forceinline__ Clamp2 cshfl_up(const Clamp2& v, int d)
 }
 
 // Interval of one primitive's fp32 values over a box given by centre and
-// half diagonal; `slack` covers fp32 rounding of both evaluations.
+// half diagonal; `slack` (1e-4 of the largest coordinate or size magnitude)
+// covers fp32 rounding of both evaluations and the few-ulp error of the
+// centre value evaluated with bare hardware square roots.
 __device__ __forceinline__ void prim_interval(const Prim& p, float cx, float cy, float cz, float rad, float slack,
                                               float& lo, float& hi)
 {
-    float v = prim_eval(p.kind, p.cx, p.cy, p.cz, p.a, p.b, p.c, cx, cy, cz);
+    float v = prim_eval<true>(p.kind, p.cx, p.cy, p.cz, p.a, p.b, p.c, cx, cy, cz);
     lo = v - rad - slack;
     hi = v + rad + slack;
 }
