@@ -485,35 +485,7 @@ def scene_sweep_gridfield_pairs(pairs, total_budget_ms=800.0, min_pitch_mm=0.25,
         return [], {"n_pairs": 0, "note": "empty pair list"}
     t_sweep0 = time.perf_counter()
 
-    # stage A: box-overlap priors, the leaf boxes memoised per leaf
-    box_cache = {}
-
-    def leaf_box(node):
-        k = E._leaf_key(node)
-        if k not in box_cache:
-            box_cache[k] = bbox_of(node)
-        return box_cache[k]
-
-    priors = []
-    for la, lb in pairs:
-        loA, hiA = leaf_box(la)
-        loB, hiB = leaf_box(lb)
-        lo = np.maximum(loA, loB)
-        hi = np.minimum(hiA, hiB)
-        empty = bool(np.any(hi <= lo))
-        priors.append({"lo": lo, "hi": hi, "vol": 0.0 if empty else float(np.prod(hi - lo)), "empty": empty})
-    if prior_vol_override is not None:
-        if len(prior_vol_override) != n_pairs:
-            raise ValueError("prior_vol_override needs one value per pair")
-        for i, p in enumerate(priors):
-            p["alloc_vol"] = 0.0 if p["empty"] else float(prior_vol_override[i])
-    else:
-        for p in priors:
-            p["alloc_vol"] = p["vol"]
-    total_prior_vol = sum(p["alloc_vol"] for p in priors)
-    t_prior = time.perf_counter() - t_sweep0
-
-    # the leaf table (distinct leaves) and the device passes
+    # the distinct leaves (the device leaf table) and each pair's two of them
     all_leaves, leaf_index = [], {}
 
     def leaf_id(node):
@@ -525,10 +497,30 @@ def scene_sweep_gridfield_pairs(pairs, total_budget_ms=800.0, min_pitch_mm=0.25,
 
     leaf_a_of_pair = np.array([leaf_id(la) for la, _ in pairs], np.int32)
     leaf_b_of_pair = np.array([leaf_id(lb) for _, lb in pairs], np.int32)
+
+    # stage A: box-overlap priors, over arrays of pairs (one box per distinct
+    # leaf; the volume is (dx*dy)*dz, as a product over the three extents)
+    boxes = [bbox_of(leaf) for leaf in all_leaves]
+    leaf_lo = np.array([b[0] for b in boxes], np.float64)
+    leaf_hi = np.array([b[1] for b in boxes], np.float64)
+    pr_lo = np.maximum(leaf_lo[leaf_a_of_pair], leaf_lo[leaf_b_of_pair])
+    pr_hi = np.minimum(leaf_hi[leaf_a_of_pair], leaf_hi[leaf_b_of_pair])
+    pr_empty = np.any(pr_hi <= pr_lo, axis=1)
+    ext = pr_hi - pr_lo
+    pr_vol = np.where(pr_empty, 0.0, ext[:, 0] * ext[:, 1] * ext[:, 2])
+    if prior_vol_override is not None:
+        if len(prior_vol_override) != n_pairs:
+            raise ValueError("prior_vol_override needs one value per pair")
+        alloc_vol = [0.0 if e else float(v) for e, v in zip(pr_empty.tolist(), prior_vol_override)]
+    else:
+        alloc_vol = pr_vol.tolist()
+    vol_list, empty_list = pr_vol.tolist(), pr_empty.tolist()
+    total_prior_vol = sum(alloc_vol)  # left to right, like the per-pair loop
+    t_prior = time.perf_counter() - t_sweep0
     passes = _PairPasses(all_leaves, device)
 
     # stage B: two-term cost model of the pair pass (cached per device)
-    lo0, hi0 = leaf_box(pairs[0][0])
+    lo0, hi0 = boxes[int(leaf_a_of_pair[0])]
     overhead_s, rate, from_cache = calibrate_two_term("ikarus_pairs::" + str(passes.dev),
                                                       _OnePair(passes, int(leaf_a_of_pair[0]),
                                                                int(leaf_a_of_pair[0])),
@@ -541,32 +533,37 @@ def scene_sweep_gridfield_pairs(pairs, total_budget_ms=800.0, min_pitch_mm=0.25,
     max_pair_points = max(min(int(alloc_pool * max_pair_frac_of_pool), max_pair_points_abs), min_pair_coarse_points)
     coarse_pitch_of_pair = [None] * n_pairs
     n_pairs_capped = 0
-    for i, p in enumerate(priors):
-        if p["empty"]:
+    for i in range(n_pairs):
+        if empty_list[i]:
             continue
-        share = (p["alloc_vol"] / total_prior_vol) if total_prior_vol > 0 else (1.0 / n_pairs)
+        share = (alloc_vol[i] / total_prior_vol) if total_prior_vol > 0 else (1.0 / n_pairs)
         n_target = min_pair_coarse_points + int(alloc_pool * share)
         if n_target > max_pair_points:
             n_target = max_pair_points
             n_pairs_capped += 1
-        coarse_pitch_of_pair[i] = max((p["vol"] / max(n_target, 1)) ** (1.0 / 3.0), min_pitch_mm)
+        coarse_pitch_of_pair[i] = max((vol_list[i] / max(n_target, 1)) ** (1.0 / 3.0), min_pitch_mm)
 
-    # the pair table of the coarse launch (pairs with a grid pass, in order)
+    # the pair table of the coarse launch (pairs with a grid pass, in order),
+    # filled column by column: grid_shape's ceil((hi - lo) / pitch) + 1 (at
+    # least 1) per axis, band half-width 0.5 * pitch * sqrt(3)
     t_buf0 = time.perf_counter()
     live = [i for i in range(n_pairs) if coarse_pitch_of_pair[i] is not None]
+    live_ix = np.array(live, np.int64)
     recs = np.zeros(len(live), PAIR_REC)
     first = 0
-    for r_i, i in enumerate(live):
-        pitch = coarse_pitch_of_pair[i]
-        shape = grid_shape(priors[i]["lo"], priors[i]["hi"], pitch)
-        dh = 0.5 * pitch * math.sqrt(3.0)
-        recs[r_i]["lx"], recs[r_i]["ly"], recs[r_i]["lz"] = priors[i]["lo"]
-        recs[r_i]["pitch"] = pitch
-        recs[r_i]["first"] = first
-        recs[r_i]["nx"], recs[r_i]["ny"], recs[r_i]["nz"] = shape
-        recs[r_i]["la"], recs[r_i]["lb"] = leaf_a_of_pair[i], leaf_b_of_pair[i]
-        recs[r_i]["below"], recs[r_i]["band"] = np.float32(-dh), np.float32(dh)
-        first += int(np.prod(shape))
+    if len(live):
+        pitch_arr = np.array([coarse_pitch_of_pair[i] for i in live], np.float64)
+        lo_l, hi_l = pr_lo[live_ix], pr_hi[live_ix]
+        shape_arr = np.maximum(np.ceil((hi_l - lo_l) / pitch_arr[:, None]).astype(np.int64) + 1, 1)
+        npts = shape_arr[:, 0] * shape_arr[:, 1] * shape_arr[:, 2]
+        dh = 0.5 * pitch_arr * math.sqrt(3.0)
+        recs["lx"], recs["ly"], recs["lz"] = lo_l[:, 0], lo_l[:, 1], lo_l[:, 2]
+        recs["pitch"] = pitch_arr
+        recs["first"] = np.concatenate(([0], np.cumsum(npts)[:-1]))
+        recs["nx"], recs["ny"], recs["nz"] = shape_arr[:, 0], shape_arr[:, 1], shape_arr[:, 2]
+        recs["la"], recs["lb"] = leaf_a_of_pair[live_ix], leaf_b_of_pair[live_ix]
+        recs["below"], recs["band"] = (-dh).astype(np.float32), dh.astype(np.float32)
+        first = int(npts.sum())
     t_buffer_build = time.perf_counter() - t_buf0
 
     t0 = time.perf_counter()
@@ -581,15 +578,17 @@ def scene_sweep_gridfield_pairs(pairs, total_budget_ms=800.0, min_pitch_mm=0.25,
             results[i] = {"method": "sweep_batched_prior_alloc", "volume_mm3": 0.0, "uncertainty_mm3": 0.0,
                           "dispatch_log": {"note": "AABB-overlap prior volume is 0 -- no grid pass needed",
                                            "prior_vol_mm3": 0.0}}
+    npts_list = npts.tolist() if len(live) else []
+    coarse_list = coarse.tolist()
     for r_i, i in enumerate(live):
         pitch = coarse_pitch_of_pair[i]
-        n_i = int(recs[r_i]["nx"]) * int(recs[r_i]["ny"]) * int(recs[r_i]["nz"])
-        n_in, n_b = int(coarse[r_i, 0]), int(coarse[r_i, 1])
+        n_i = npts_list[r_i]
+        n_in, n_b = coarse_list[r_i]
         cell_vol = pitch ** 3
         n_boundary_of_pair[i] = n_b
         results[i] = {"method": "sweep_batched_prior_alloc", "volume_mm3": float(n_in) * cell_vol,
                       "uncertainty_mm3": n_b * cell_vol,
-                      "dispatch_log": {"prior_vol_mm3": priors[i]["vol"], "coarse_pitch_mm": pitch,
+                      "dispatch_log": {"prior_vol_mm3": vol_list[i], "coarse_pitch_mm": pitch,
                                        "n_coarse_points": n_i, "n_fully_inside_coarse": n_in,
                                        "n_boundary_coarse": n_b}}
 
@@ -607,31 +606,30 @@ def scene_sweep_gridfield_pairs(pairs, total_budget_ms=800.0, min_pitch_mm=0.25,
         refine_factor -= 1
     if refine_factor > 1 and total_boundary > 0:
         t_rb0 = time.perf_counter()
-        for r_i, i in enumerate(live):
-            raw_fine_pitch = coarse_pitch_of_pair[i] / refine_factor
-            recs[r_i]["fband"] = np.float32(0.5 * raw_fine_pitch * math.sqrt(3.0))
+        recs["fband"] = (0.5 * (pitch_arr / refine_factor) * math.sqrt(3.0)).astype(np.float32)
         t_refine_buffer_build = time.perf_counter() - t_rb0
         t0 = time.perf_counter()
         fine = passes.refine(recs, total_boundary, refine_factor)
         t_refine_launch = time.perf_counter() - t0
+        fine_list = fine.tolist()
         for r_i, i in enumerate(live):
             if n_boundary_of_pair[i] == 0:
                 continue
             raw_fp = coarse_pitch_of_pair[i] / refine_factor
             fine_cell_vol = raw_fp ** 3
-            results[i]["volume_mm3"] += float(int(fine[r_i, 0])) * fine_cell_vol
-            results[i]["uncertainty_mm3"] = int(fine[r_i, 1]) * fine_cell_vol
+            results[i]["volume_mm3"] += float(int(fine_list[r_i][0])) * fine_cell_vol
+            results[i]["uncertainty_mm3"] = int(fine_list[r_i][1]) * fine_cell_vol
             log = results[i]["dispatch_log"]
             log["refine_factor"] = refine_factor
             log["fine_pitch_mm"] = max(raw_fp, min_pitch_mm)
             log["n_fine_points"] = n_boundary_of_pair[i] * refine_factor ** 3
-            log["n_boundary_remaining_at_finest"] = int(fine[r_i, 1])
+            log["n_boundary_remaining_at_finest"] = int(fine_list[r_i][1])
 
     sweep_log = {
         "n_pairs": n_pairs, "total_budget_ms": total_budget_ms,
         "cost_model_v1_2": {"overhead_s": overhead_s, "rate_pts_per_s": rate, "from_cache": from_cache},
         "total_prior_overlap_vol_mm3": total_prior_vol,
-        "n_pairs_empty_prior": sum(1 for p in priors if p["empty"]),
+        "n_pairs_empty_prior": int(np.count_nonzero(pr_empty)),
         "total_points_budget": total_points_budget, "n_coarse_points_actual": int(first),
         "max_pair_points_cap": max_pair_points, "n_pairs_capped": n_pairs_capped,
         "n_launches": 1 + (1 if (refine_factor > 1 and total_boundary > 0) else 0),
