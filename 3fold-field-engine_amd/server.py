@@ -32,13 +32,16 @@ def _torch():
 
 
 class ResidentField:
-    """A built field kept on the device: SparseField data plus a descriptor."""
+    """A built field kept on the device: SparseField data plus a descriptor.
+    n_active: the build's active-block count when the caller has read it
+    already; upload=False leaves the descriptor to the caller (see
+    resident_fields)."""
 
-    def __init__(self, plan, j, stream):
+    def __init__(self, plan, j, stream, n_active=None, upload=True):
         torch = _torch()
         meta, rec = plan.items[j]
         self.meta, self.recipe = meta, rec
-        na = plan.n_active(j)
+        na = plan.n_active(j) if n_active is None else n_active
         self.n_active = na
         B = plan.B
         with torch.cuda.stream(stream):
@@ -52,19 +55,26 @@ class ResidentField:
             n_prims = len(rec)
             off = sum(len(r) for _, r in plan.items[:j])
             self.prims = plan.prims[off * 32:(off + n_prims) * 32].clone()
-            self.counts = torch.tensor([na, 0, 0, 0], dtype=torch.int32, device=plan.device)
+            # [n_active, 0, 0, 0], filled on the device (no host round trip)
+            self.counts = torch.zeros(4, dtype=torch.int32, device=plan.device)
+            self.counts[:1].copy_(plan.counts(j)[:1])
         d = plan.desc_host[j:j + 1].copy()
         d["prims"], d["kind"], d["cma"] = self.prims.data_ptr(), self.kind.data_ptr(), self.cma.data_ptr()
         d["lists"], d["list_len"], d["ids"] = self.lists.data_ptr(), self.list_len.data_ptr(), self.ids.data_ptr()
         d["slot"], d["tiles"], d["counts"] = self.slot.data_ptr(), self.tiles.data_ptr(), self.counts.data_ptr()
         d["tile_cap"] = na
         self.desc_host = d
-        with torch.cuda.stream(stream):
-            self.desc = torch.from_numpy(d.view(np.uint8).copy()).to(plan.device, non_blocking=False)
+        self.B = B
+        self.desc = None
+        if upload:
+            with torch.cuda.stream(stream):
+                self._set_desc(torch.from_numpy(d.view(np.uint8).copy()).to(plan.device, non_blocking=False))
+
+    def _set_desc(self, desc):
+        self.desc = desc
         self.nbytes = sum(t.numel() * t.element_size() for t in
                           (self.kind, self.ids, self.tiles, self.cma, self.slot, self.list_len, self.lists,
                            self.prims, self.counts, self.desc))
-        self.B = B
 
     def query_device(self, points, out=None, stream=None):
         """Point SDF at a device float32 (n,3) tensor."""
@@ -80,6 +90,24 @@ class ResidentField:
         return F.SparseField(meta=self.meta, kind=kind, active_ids=self.ids.cpu().numpy(),
                              tiles=self.tiles.cpu().numpy(), corner_min_abs=self.cma.cpu().numpy(),
                              stats=F._stats(self.meta, kind, self.n_active))
+
+
+def resident_fields(plan, stream):
+    """ResidentFields of every build of a finished plan, with one device-to-host
+    read of the active-block counts and one upload of the descriptors, both on
+    `stream` (so they wait for this plan's work only)."""
+    torch = _torch()
+    n = len(plan.items)
+    with torch.cuda.stream(stream):
+        nas = torch.cat([plan.counts(j)[:1] for j in range(n)]).cpu().tolist()
+    fields = [ResidentField(plan, j, stream, n_active=nas[j], upload=False) for j in range(n)]
+    descs = np.concatenate([f.desc_host for f in fields])
+    size = descs.dtype.itemsize
+    with torch.cuda.stream(stream):
+        dev = torch.from_numpy(descs.view(np.uint8).copy()).to(plan.device, non_blocking=False)
+    for j, f in enumerate(fields):
+        f._set_desc(dev[j * size:(j + 1) * size])
+    return fields
 
 
 def workspace_bytes(meta, recipe):
@@ -265,8 +293,7 @@ class FieldServer:
             plan, batch, need, stream, ev = item
             fields = []
             try:
-                for j in range(len(batch)):
-                    fields.append(ResidentField(plan, j, stream))
+                fields = resident_fields(plan, stream)
                 stream.synchronize()
                 err = None
             except Exception as e:
