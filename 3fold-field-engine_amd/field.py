@@ -106,9 +106,12 @@ class FieldPlan:
     """Device buffers and descriptor table for R builds with one block edge.
 
     items: [(GridMeta, recipe)] (+ optional per-item tile capacity in blocks;
-    the default is every block, which can never overflow)."""
+    the default is every block, which can never overflow).  pools: optional
+    {"i32", "f32", "u16"} device tensors of an earlier plan whose work is
+    finished; a pool at least as large as this plan needs is reused (sliced)
+    instead of allocating.  self.pools holds the tensors this plan uses."""
 
-    def __init__(self, items, tile_caps=None, device=None, corrupt=False):
+    def __init__(self, items, tile_caps=None, device=None, corrupt=False, pools=None):
         torch = _torch()
         if not items:
             raise ValueError("At least one build required")
@@ -135,9 +138,18 @@ class FieldPlan:
         f32_sizes = [cma_len[i] + caps[i] * b3 for i in range(n)]
         u16_sizes = [nsb[i] * lcap[i] for i in range(n)]
         prim_counts = [len(r) for _, r in items]
-        self.i32 = torch.empty(sum(i32_sizes), dtype=torch.int32, device=self.device)
-        self.f32 = torch.empty(sum(f32_sizes), dtype=torch.float32, device=self.device)
-        self.u16 = torch.empty(max(1, sum(u16_sizes)), dtype=torch.int16, device=self.device)
+        self.pools = {}
+
+        def pool(name, size, dtype):
+            t = (pools or {}).get(name)
+            if t is None or t.dtype != dtype or t.device != self.device or t.numel() < size:
+                t = torch.empty(size, dtype=dtype, device=self.device)
+            self.pools[name] = t
+            return t[:size]
+
+        self.i32 = pool("i32", sum(i32_sizes), torch.int32)
+        self.f32 = pool("f32", sum(f32_sizes), torch.float32)
+        self.u16 = pool("u16", max(1, sum(u16_sizes)), torch.int16)
         host_prims = np.concatenate([r for _, r in items])
         self.prims = torch.from_numpy(host_prims.view(np.uint8).copy()).to(self.device)
         desc = np.zeros(n, FIELD_DESC)

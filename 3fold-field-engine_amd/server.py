@@ -140,7 +140,9 @@ class FieldServer:
 
     hbm_budget: bytes of device memory the server may hold (workspaces of
     in-flight batches plus resident fields).  slots: batches in flight at once
-    (one stream each).  max_batch: requests per batch."""
+    (one stream each); each slot also keeps the largest workspace it has used
+    between its batches (reused, so steady serving allocates nothing per
+    batch).  max_batch: requests per batch."""
 
     def __init__(self, hbm_budget=200e9, slots=2, max_batch=8, device=None, start=True):
         torch = _torch()
@@ -155,6 +157,7 @@ class FieldServer:
         self.fields = {}
         self.requests = {}
         self.reserved = 0.0  # workspaces in flight + resident fields
+        self.slot_pools = {}  # id(stream) -> the FieldPlan pools last used on that stream
         self.ids = itertools.count()
         self.lock = threading.Condition()
         self.running = True
@@ -214,6 +217,7 @@ class FieldServer:
             self.lock.notify_all()
         if self.worker.is_alive():
             self.worker.join()
+        self.slot_pools.clear()
 
     def stats(self):
         b = np.array([(r.t_done - r.t_arrival) * 1e3 for r in self.completed]) if self.completed else np.zeros(1)
@@ -260,7 +264,11 @@ class FieldServer:
                     r.t_admit = now
                 try:
                     with torch.cuda.stream(stream):
-                        plan = F.FieldPlan([(r.meta, r.recipe) for r in batch], device=self.device)
+                        # the slot's workspace of its earlier batches (finished and
+                        # harvested before the slot was freed) is reused when large enough
+                        plan = F.FieldPlan([(r.meta, r.recipe) for r in batch], device=self.device,
+                                           pools=self.slot_pools.get(id(stream)))
+                        self.slot_pools[id(stream)] = plan.pools
                         plan.launch(stream)
                         ev = torch.cuda.Event()
                         ev.record(stream)
