@@ -84,3 +84,27 @@ def test_server_overlapping_arrivals(fe, gpu):
             assert same(srv.result(rid).sparse_field(), fe.klassificera_och_evaluera_recept(m, r))
     finally:
         srv.close()
+
+
+def test_plan_on_pools_of_a_finished_plan(fe, gpu):
+    """A plan built on the pools of a finished plan (smaller than those pools,
+    then larger) equals the plain per-call builds bit for bit; the server's
+    slots reuse workspaces this way."""
+    import torch
+    items = mixed_items(fe, 9, seed=13)
+    metas = [(fe.batch.make_meta(s, 8), r) for s, r in items]
+    plain = fe.batch.plain_builds(items, block=8)
+    first = fe.FieldPlan(metas[:6])
+    first.launch()
+    torch.cuda.synchronize()
+    small = fe.FieldPlan(metas[6:8], pools=first.pools)
+    assert small.f32.data_ptr() == first.pools["f32"].data_ptr()  # the fitting pool is reused
+    small.launch()
+    torch.cuda.synchronize()
+    for j in range(2):
+        assert same(small.sparse_field(j), plain[6 + j]), j
+    large = fe.FieldPlan(metas, pools=small.pools)
+    large.launch()
+    torch.cuda.synchronize()
+    for j in range(len(metas)):
+        assert same(large.sparse_field(j), plain[j]), j
