@@ -29,6 +29,28 @@ void fe_note_error(hipError_t e, const char* where);
 __host__ __device__ __forceinline__ float sel_min(float a, float b) { return a < b ? a : b; }
 __host__ __device__ __forceinline__ float sel_max(float a, float b) { return a > b ? a : b; }
 
+// Correctly rounded fp32 square root (the same bits as sqrtf), branch-free:
+// inputs below 2^-96 are scaled by 2^32 (result by 2^-16); v_sqrt_f32 is
+// within one ulp, so the correctly rounded root is s or a neighbouring float,
+// and the exact fma residuals x - s_dn*s <= 0 and x - s_up*s > 0 pick it (the
+// root lies below the midpoint (s_dn + s)/2 exactly when x - s_dn*s <= 0, as
+// that residual is a multiple of ulp^2).  This is the residual test of the
+// compiler's own correctly rounded lowering without its final class check,
+// which returns x for +-0 and +inf: those already come out as x here (for 0
+// the lower neighbour is a NaN pattern and the upper residual is +0; for +inf
+// both residuals are NaN), and NaN stays NaN.
+__device__ __forceinline__ float sqrt_rn(float x)
+{
+    const bool scale = x < 0x1p-96f;
+    const float xs = scale ? x * 0x1p32f : x;
+    const float s = __builtin_amdgcn_sqrtf(xs);
+    const float s_dn = __uint_as_float(__float_as_uint(s) - 1u);
+    const float s_up = __uint_as_float(__float_as_uint(s) + 1u);
+    float r = __builtin_fmaf(-s_dn, s, xs) <= 0.0f ? s_dn : s;
+    r = __builtin_fmaf(-s_up, s, xs) > 0.0f ? s_up : r;
+    return scale ? r * 0x1p-16f : r;
+}
+
 // Uniform 64-bit lane read (idx must be wave-uniform).
 __device__ __forceinline__ uint64_t readlane64(uint64_t v, int idx)
 {

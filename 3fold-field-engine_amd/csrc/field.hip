@@ -66,11 +66,11 @@ static_assert(sizeof(FieldDesc) == 160, "descriptor layout");
 // ---------------------------------------------------------------- primitives
 // Same operation order as oracle/fe_oracle.c or_prim_eval (IKARUS emitted
 // fp32 source, FE/ikarus_v1/eval_warp.py:182-224; FE/faltkarna_v1.py:91-104).
-// FAST: square roots by the bare v_sqrt_f32 (within one ulp) instead of the
-// correctly rounded sqrtf; used only for the centres of culling intervals,
-// whose slack is far wider than that error.
+// Square roots are correctly rounded (sqrt_rn, the bits of sqrtf).  FAST:
+// the bare v_sqrt_f32 (within one ulp) instead; used only for the centres of
+// culling intervals, whose slack is far wider than that error.
 template <bool FAST = false>
-__device__ __forceinline__ float root(float x) { return FAST ? __builtin_amdgcn_sqrtf(x) : sqrtf(x); }
+__device__ __forceinline__ float root(float x) { return FAST ? __builtin_amdgcn_sqrtf(x) : sqrt_rn(x); }
 template <bool FAST = false>
 __device__ __forceinline__ float prim_eval(int kind, float cx, float cy, float cz, float a, float b, float c, float x,
                                            float y, float z)
