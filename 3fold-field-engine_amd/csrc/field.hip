@@ -415,8 +415,9 @@ __global__ __launch_bounds__(256) void k_classify(const FieldDesc* __restrict__ 
         const uint16_t* gl = fits ? list : nullptr;
         int gn = n;
         if (fits && n <= wcap) {
+            // one pass: a second one (as the tiles run) costs more than the
+            // few entries it would drop from a 45-point lattice's list
             gn = cull_wave(D.prims, list, n, cx, cy, cz, rad, D.slack, wl);
-            if (gn > 8) gn = cull_wave(D.prims, wl, gn, cx, cy, cz, rad, D.slack, wl);
             gl = wl;
         }
         const int la = lane / 9, lb = (lane / 3) % 3, lc = lane % 3;
