@@ -121,7 +121,10 @@ int fe_octree_sample(const uint8_t *packed, const int32_t *branch_prefix, const 
  * eight-corner classification (faltkarna_v1.py:128-164), ascending compaction
  * (:277), tile evaluation (:166-190).  sb_prefix/ch_prefix: device int32[R+1]
  * exclusive prefixes of superblock and 1024-block chunk counts; chunk_count:
- * device int32[total_chunks] scratch. */
+ * device int32[total_chunks] scratch.  Every descriptor's block edge equals
+ * B (4, 8 or 16).  tile_waves: waves of the persistent tile launch; the
+ * active blocks of all R recipes, concatenated in order, are dealt to them
+ * round-robin. */
 int fe_field_build(const void *descs, int R, int B, const int32_t *sb_prefix, int total_sb, const int32_t *ch_prefix,
                    int total_chunks, int32_t *chunk_count, int tile_waves, int max_prims, void *stream);
 
