@@ -12,7 +12,6 @@ from .edt import (EDTBatch, NativeEDT, NativeEDTLarge, NativeEDTVolume, Persiste
                   edt_device, signed_distance, signed_from_pair_device, squared_distance)
 
 from . import recipes, field, batch, server, blocks, csg, octree, mesh, mesh_service, ikarus, ikarus_queries  # noqa: E402
-from .mesh_service import MeshFieldService  # noqa: E402
 from .mesh import PreparedMeshField, surface_raster_and_flood_cuda  # noqa: E402
 from .blocks import NODE, AdaptiveSampleBlocks, NativeAdaptiveBlocks  # noqa: E402
 from .csg import evaluate, compose, gather_tile, NativeTileReader  # noqa: E402
@@ -22,7 +21,7 @@ from .field import (GridMeta, SparseField, FieldPlan, klassificera_och_evaluera_
                     reconstruct_solid_from_sparse, volym_via_gpu_reduktion, determinism_cert, konservering_check,
                     klassificera_och_evaluera_fran_tatt_falt)
 
-__all__ = ["recipes", "field", "batch", "server", "blocks", "csg", "octree", "mesh", "mesh_service", "MeshFieldService", "ikarus", "ikarus_queries",
+__all__ = ["recipes", "field", "batch", "server", "blocks", "csg", "octree", "mesh", "mesh_service", "ikarus", "ikarus_queries",
            "PreparedMeshField",
            "surface_raster_and_flood_cuda", "NODE", "AdaptiveSampleBlocks",
            "NativeAdaptiveBlocks", "evaluate", "compose", "gather_tile", "NativeTileReader", "GridMeta", "SparseField", "FieldPlan", "klassificera_och_evaluera_testfalt",

@@ -106,6 +106,14 @@ def large_field_512(seed=2):
     return mixed_recipe(np.random.default_rng(seed), 1024, 512)
 
 
+def served_requests(seed=5, count=48):
+    """`count` recipes of the large-field distribution (1024 mixed primitives,
+    512^3), drawn one after another from one seeded generator: the request
+    stream of the field server."""
+    rng = np.random.default_rng(seed)
+    return [mixed_recipe(rng, 1024, 512) for _ in range(count)]
+
+
 def batch_recipes(seed=3, count=256):
     """count recipes: primitive counts log-uniform in [4,2048], cubic domains
     uniform in {64,96,...,384}; returns [(recipe, n)]."""

@@ -229,3 +229,26 @@ def test_abi_host_refusals(fe):
     assert lib.fused_query(None, None, None, None) == 1
     assert lib.fused_destroy(ct.byref(P())) == 0
     assert h.value is None
+
+
+def test_result_region_layout(fe):
+    """FE/mesh_field_shared_service_v1.py:16-19: solid, surface, padding to 64
+    bytes, float32 distance."""
+    L = fe.mesh_service
+    assert L.layout((4, 3, 2)) == (24, 64, 64 + 96)
+    assert L.layout((8, 8, 8)) == (512, 1024, 1024 + 2048)
+    assert L.layout((1, 1, 1)) == (1, 64, 68)
+    assert L.layout((5, 7, 9)) == (315, 640, 640 + 1260)
+    region = bytearray(L.layout((5, 7, 9))[2])
+    solid, surface, distance = L.result_views(region, (5, 7, 9))
+    solid[...] = True
+    surface[1, 2, 3] = True
+    distance[...] = np.arange(315, dtype=np.float32).reshape(5, 7, 9)
+    raw = np.frombuffer(bytes(region), np.uint8)
+    assert raw[:315].all() and raw[315:630].sum() == 1 and raw[315 + (1 * 7 + 2) * 9 + 3] == 1
+    assert not raw[630:640].any()
+    assert np.array_equal(np.frombuffer(bytes(region[640:]), np.float32), np.arange(315, dtype=np.float32))
+    ro = L.result_views(bytes(region), (5, 7, 9))
+    assert not ro[2].flags.writeable
+    with pytest.raises(ValueError):
+        L.result_views(bytearray(100), (5, 7, 9))

@@ -370,18 +370,17 @@ def test_device_replays(fe, gpu):
         assert all(p for p in ptrs[:7]) and ptrs[7] == 64
 
 
-def test_mesh_service_reference_case(fe, gpu, fx):
-    """The spawned service (shared-memory results) on a reference case."""
+def test_result_region_reference_case(fe, gpu, fx):
+    """A prepared stage's query written into a region of the service layout."""
     meta, arrays = fx
     v, f, pitch, lo, _ = reference_case(meta, arrays, "rotated_0.0")
     want = meta["cases"]["rotated_0.0"]["digests"]
-    with fe.MeshFieldService(timeout=180) as service:
-        shape = service.prepare(v, f, pitch, lo)
-        assert list(shape) == meta["cases"]["rotated_0.0"]["shape"]
-        assert service.metadata["pinned"] is False
+    field = fe.PreparedMeshField(v, f, pitch, lo)
+    try:
+        region = bytearray(fe.mesh_service.layout(field.shape)[2])
         for leg in range(2):
-            with service.query() as lease:
-                gmin, _, surface, solid, distance = lease.result
-                got = [digest(x) for x in (gmin, surface, solid, distance)]
-                assert got == [want["gmin"], want["surface"], want["solid"], want["distance"]], leg
-                assert lease.worker_ms > 0
+            solid, surface, distance = fe.mesh_service.fill_region(field, region)
+            got = [digest(x) for x in (surface, solid, distance)]
+            assert got == [want["surface"], want["solid"], want["distance"]], leg
+    finally:
+        field.close()
