@@ -88,8 +88,6 @@ SIGNATURES = {
     "fe_ikarus_refine": (ct.c_int, [P, ct.c_int, P, P, P, P, F64, P, P, I64, ct.c_int, ct.c_float, P, P]),
     "fe_ikarus_pairs_eval": (ct.c_int, [P, ct.c_int, P, P, P, P, P, ct.c_int, I64, P, P]),
     "fe_ikarus_band_eval": (ct.c_int, [P, P, ct.c_int, P, P, P, ct.c_int, P, I64, P, P]),
-    "fe_ikarus_pairs_grid": (ct.c_int, [P, ct.c_int, I64, P, ct.c_int, P, P, P, P, P]),
-    "fe_ikarus_pairs_refine": (ct.c_int, [P, ct.c_int, P, ct.c_int, P, P, I64, ct.c_int, P, P]),
 }
 
 _lib = None

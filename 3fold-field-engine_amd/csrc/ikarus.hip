@@ -439,126 +439,6 @@ __global__ __launch_bounds__(IK_T) void k_ik_band(const IkGrid* __restrict__ gri
     out[i] = best;
 }
 
-// One pair of a batched sweep (FE/ikarus_v1/queries.py:281-551): its coarse
-// grid lo + pitch*i (f64, samples rounded to fp32), the first global sample
-// index, the two leaves and the fp32 thresholds of the coarse and fine passes.
-struct IkPair {
-    double lx, ly, lz, pitch;
-    int64_t first;
-    int32_t nx, ny, nz, la, lb;
-    float below, band, fband;
-    int32_t pad[2];
-};
-static_assert(sizeof(IkPair) == 80, "IkPair layout");
-
-// max(leaf a, leaf b) of a pair; NaN (neither inside nor in a band) when a
-// leaf index is out of range.
-__device__ __forceinline__ float pair_value(const IkGrid* __restrict__ grids, int nleaves,
-                                            const float* __restrict__ gdata, const IkPair& P, float x, float y, float z)
-{
-    if (P.la < 0 || P.la >= nleaves || P.lb < 0 || P.lb >= nleaves) return __builtin_nanf("");
-    return wmax(ik_grid(grids[P.la], gdata, x, y, z), ik_grid(grids[P.lb], gdata, x, y, z));
-}
-
-// Pair of global sample t: the last pair whose first sample is <= t.
-__device__ __forceinline__ int find_pair(const IkPair* __restrict__ pairs, int npairs, int64_t t)
-{
-    int lo = 0, hi = npairs - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (pairs[mid].first <= t) lo = mid;
-        else hi = mid - 1;
-    }
-    return lo;
-}
-
-// Per-pair counts: one atomic pair per wave when the wave's samples share a
-// pair (the usual case: pairs are contiguous sample ranges), else per lane.
-__device__ __forceinline__ void pair_count(int p, bool active, bool in, bool bd, unsigned long long* counts)
-{
-    const int p0 = __builtin_amdgcn_readfirstlane(p);
-    const uint64_t act = __ballot(active);
-    if (__ballot(active && p != p0) == 0) {
-        const int ni = __popcll(__ballot(active && in)), nb = __popcll(__ballot(active && bd));
-        if ((threadIdx.x & (FE_WAVE - 1)) == 0 && act) {
-            if (ni) atomicAdd(counts + 2 * p0, (unsigned long long)ni);
-            if (nb) atomicAdd(counts + 2 * p0 + 1, (unsigned long long)nb);
-        }
-    } else if (active) {
-        if (in) atomicAdd(counts + 2 * p, 1ull);
-        if (bd) atomicAdd(counts + 2 * p + 1, 1ull);
-    }
-}
-
-// Coarse pass of every pair in one launch: sample t of pair p at
-// (float)(lo + pitch*i) per axis (C order), v = max(leaf a, leaf b); counts
-// [p][0] += v < below, [p][1] += |v| <= band; band samples' global indices
-// appended to `cells` through `cursor`.
-__global__ __launch_bounds__(IK_T) void k_ik_pair_grid(const IkPair* __restrict__ pairs, int npairs, int64_t total,
-                                                        const IkGrid* __restrict__ grids, int nleaves,
-                                                        const float* __restrict__ gdata, unsigned long long* counts,
-                                                        int32_t* __restrict__ cells, unsigned int* cursor)
-{
-    const int64_t t = (int64_t)blockIdx.x * IK_T + threadIdx.x;
-    const bool active = t < total;
-    int p = 0;
-    bool in = false, bd = false;
-    if (active) {
-        p = find_pair(pairs, npairs, t);
-        const IkPair& P = pairs[p];
-        const int64_t l = t - P.first;
-        const int i = (int)(l / ((int64_t)P.ny * P.nz)), j = (int)((l / P.nz) % P.ny), k = (int)(l % P.nz);
-        const float x = (float)(P.lx + P.pitch * (double)i), y = (float)(P.ly + P.pitch * (double)j),
-                    z = (float)(P.lz + P.pitch * (double)k);
-        const float v = pair_value(grids, nleaves, gdata, P, x, y, z);
-        in = v < P.below;
-        bd = fabsf(v) <= P.band;
-    }
-    const uint64_t m = __ballot(bd);
-    const int lane = threadIdx.x & (FE_WAVE - 1);
-    unsigned int base = 0;
-    if (lane == 0 && m) base = atomicAdd(cursor, (unsigned int)__popcll(m));
-    base = __shfl(base, 0);
-    if (bd) cells[base + __popcll(m & ((1ull << lane) - 1))] = (int32_t)t;
-    pair_count(p, active, in, bd, counts);
-}
-
-// Refinement of the band samples of every pair in one launch: cell c (global
-// coarse index) gets r^3 samples at centre(c) + ((a + 0.5)/r * pitch - pitch/2)
-// per axis (f64, a-major), rounded to fp32; counts [p][0] += v < 0,
-// [p][1] += |v| <= fband.
-__global__ __launch_bounds__(IK_T) void k_ik_pair_refine(const IkPair* __restrict__ pairs, int npairs,
-                                                          const IkGrid* __restrict__ grids, int nleaves,
-                                                          const float* __restrict__ gdata,
-                                                          const int32_t* __restrict__ cells, int64_t ncells, int r,
-                                                          unsigned long long* counts)
-{
-    const int64_t r3 = (int64_t)r * r * r;
-    const int64_t t = (int64_t)blockIdx.x * IK_T + threadIdx.x;
-    const bool active = t < ncells * r3;
-    int p = 0;
-    bool in = false, bd = false;
-    if (active) {
-        const int64_t c = cells[t / r3];
-        p = find_pair(pairs, npairs, c);
-        const IkPair& P = pairs[p];
-        const int sub = (int)(t % r3);
-        const int a = sub / (r * r), b = (sub / r) % r, e = sub % r;
-        const int64_t l = c - P.first;
-        const int i = (int)(l / ((int64_t)P.ny * P.nz)), j = (int)((l / P.nz) % P.ny), k = (int)(l % P.nz);
-        const double half = P.pitch / 2.0;
-        const double ox = ((double)a + 0.5) / (double)r * P.pitch - half;
-        const double oy = ((double)b + 0.5) / (double)r * P.pitch - half;
-        const double oz = ((double)e + 0.5) / (double)r * P.pitch - half;
-        const float x = (float)((P.lx + P.pitch * (double)i) + ox), y = (float)((P.ly + P.pitch * (double)j) + oy),
-                    z = (float)((P.lz + P.pitch * (double)k) + oz);
-        const float v = pair_value(grids, nleaves, gdata, P, x, y, z);
-        in = v < 0.0f;
-        bd = fabsf(v) <= P.fband;
-    }
-    pair_count(p, active, in, bd, counts);
-}
-
 bool valid_program(const void* ins, int n_ins, const void* par)
 {
     return ins && par && n_ins > 0 && n_ins <= (1 << 20);
@@ -699,42 +579,3 @@ extern "C" int fe_ikarus_band_eval(const void* grids, const float* grid_data, in
     return FE_OK;
 }
 
-// Coarse pass of a batched pair sweep: pairs = npairs 80-byte records (device)
-// {double lx, ly, lz, pitch; int64 first; int32 nx, ny, nz, leaf_a, leaf_b;
-// float below, band, fine_band; int32 pad[2]} with first ascending from 0 and
-// total = last first + its nx*ny*nz < 2^31; counts: device uint64[npairs][2]
-// (added to); cells: device int32[total] and cursor uint32 (zeroed by the
-// caller) receive the band samples' global indices.
-extern "C" int fe_ikarus_pairs_grid(const void* pairs, int npairs, int64_t total, const void* grids, int nleaves,
-                                    const float* grid_data, unsigned long long* counts, int32_t* cells,
-                                    unsigned int* cursor, void* stream) noexcept
-{
-    if (!pairs || npairs < 1 || total < 0 || total >= INT32_MAX || !grids || nleaves < 1 || !grid_data || !counts ||
-        !cells || !cursor)
-        return FE_INVALID;
-    if (total == 0) return FE_OK;
-    hipLaunchKernelGGL(k_ik_pair_grid, dim3((unsigned)((total + IK_T - 1) / IK_T)), dim3(IK_T), 0,
-                       static_cast<hipStream_t>(stream), static_cast<const IkPair*>(pairs), npairs, total,
-                       static_cast<const IkGrid*>(grids), nleaves, grid_data, counts, cells, cursor);
-    FE_TRY(hipGetLastError());
-    return FE_OK;
-}
-
-// Refinement pass of a batched pair sweep over ncells band samples (global
-// indices from fe_ikarus_pairs_grid) with r^3 samples each.
-extern "C" int fe_ikarus_pairs_refine(const void* pairs, int npairs, const void* grids, int nleaves,
-                                      const float* grid_data, const int32_t* cells, int64_t ncells, int r,
-                                      unsigned long long* counts, void* stream) noexcept
-{
-    if (!pairs || npairs < 1 || !grids || nleaves < 1 || !grid_data || ncells < 0 || (ncells && !cells) || r < 1 ||
-        r > 64 || !counts)
-        return FE_INVALID;
-    const int64_t total = ncells * r * r * r;
-    if (total == 0) return FE_OK;
-    if (total >= ((int64_t)1 << 40)) return FE_INVALID;
-    hipLaunchKernelGGL(k_ik_pair_refine, dim3((unsigned)((total + IK_T - 1) / IK_T)), dim3(IK_T), 0,
-                       static_cast<hipStream_t>(stream), static_cast<const IkPair*>(pairs), npairs,
-                       static_cast<const IkGrid*>(grids), nleaves, grid_data, cells, ncells, r, counts);
-    FE_TRY(hipGetLastError());
-    return FE_OK;
-}

@@ -44,62 +44,58 @@ assert GRID_REC.itemsize == 48
 AXES = {"x": 0, "y": 1, "z": 2}
 
 
-# ------------------------------------------------------------------ 4x4 affine helpers (FE/ikarus_v1/expr.py:24-83)
+# ------------------------------------------------------------------ 4x4 affine maps
+# Transform nodes carry a row-major 4x4 affine matrix as nested lists (the
+# JSON form of FE/ikarus_v1/expr.py:24-83).
 def mat4_identity():
-    return [[1.0, 0.0, 0.0, 0.0], [0.0, 1.0, 0.0, 0.0], [0.0, 0.0, 1.0, 0.0], [0.0, 0.0, 0.0, 1.0]]
+    return np.eye(4).tolist()
 
 
 def mat4_translate(tx, ty, tz):
-    m = mat4_identity()
-    m[0][3], m[1][3], m[2][3] = tx, ty, tz
-    return m
+    m = np.eye(4)
+    m[:3, 3] = (tx, ty, tz)
+    return m.tolist()
 
 
 def mat4_rot_axis(axis, deg):
-    a = math.radians(deg)
-    c, s = math.cos(a), math.sin(a)
-    m = mat4_identity()
-    if axis == "x":
-        m[1][1], m[1][2], m[2][1], m[2][2] = c, -s, s, c
-    elif axis == "y":
-        m[0][0], m[0][2], m[2][0], m[2][2] = c, s, -s, c
-    elif axis == "z":
-        m[0][0], m[0][1], m[1][0], m[1][1] = c, -s, s, c
-    else:
-        raise ValueError(f"axis must be x/y/z, got {axis!r}")
-    return m
+    """Rotation by `deg` degrees about coordinate axis 'x', 'y' or 'z'
+    (right-handed: the plane (axis+1, axis+2) turns counter-clockwise)."""
+    if axis not in AXES:
+        raise ValueError(f"rotation axis must be one of x, y, z (got {axis!r})")
+    k = AXES[axis]
+    i, j = (k + 1) % 3, (k + 2) % 3
+    rad = math.radians(deg)
+    m = np.eye(4)
+    m[i, i] = m[j, j] = math.cos(rad)
+    m[i, j], m[j, i] = -math.sin(rad), math.sin(rad)
+    return m.tolist()
 
 
 def mat4_mul(a, b):
-    return [[sum(a[i][k] * b[k][j] for k in range(4)) for j in range(4)] for i in range(4)]
+    return (np.asarray(a, np.float64) @ np.asarray(b, np.float64)).tolist()
 
 
 def mat4_inverse_affine(m):
-    """Inverse of an affine 4x4 by cofactors of its 3x3 block (the world ->
-    local map a transform node applies; the same double operations as
-    FE/ikarus_v1/expr.py:57-83, so the fp32 parameters round alike)."""
-    R = [[m[i][j] for j in range(3)] for i in range(3)]
-    t = [m[i][3] for i in range(3)]
-    det = (R[0][0] * (R[1][1] * R[2][2] - R[1][2] * R[2][1])
-           - R[0][1] * (R[1][0] * R[2][2] - R[1][2] * R[2][0])
-           + R[0][2] * (R[1][0] * R[2][1] - R[1][1] * R[2][0]))
-    if abs(det) < 1e-15:
-        raise ValueError("singular transform, cannot invert")
-
-    def cof(r, c):
-        return (R[(r + 1) % 3][(c + 1) % 3] * R[(r + 2) % 3][(c + 2) % 3]
-                - R[(r + 1) % 3][(c + 2) % 3] * R[(r + 2) % 3][(c + 1) % 3])
-    inv = [[0.0] * 3 for _ in range(3)]
-    for i in range(3):
-        for j in range(3):
-            inv[j][i] = cof(i, j) / det
-    out = mat4_identity()
-    for i in range(3):
-        for j in range(3):
-            out[i][j] = inv[i][j]
-    for i in range(3):
-        out[i][3] = -(inv[i][0] * t[0] + inv[i][1] * t[1] + inv[i][2] * t[2])
-    return out
+    """World -> local map of an affine 4x4: the 3x3 block inverted through its
+    adjugate, whose column i is the cross product of rows i+1 and i+2, then
+    the translation.  Each entry is one double product difference divided by
+    the determinant, and every sum runs left to right, so the values equal
+    those of the cofactor expansion in FE/ikarus_v1/expr.py:57-83 bit for bit
+    (the fp32 parameters of a transform then round alike)."""
+    rows = [np.asarray(m[r][:3], np.float64) for r in range(3)]
+    adj = [np.cross(rows[(c + 1) % 3], rows[(c + 2) % 3]) for c in range(3)]
+    det = rows[0][0] * adj[0][0] + rows[0][1] * adj[0][1] + rows[0][2] * adj[0][2]
+    if not abs(det) >= 1e-15:
+        raise ValueError("transform matrix is singular")
+    inv = np.empty((3, 3))
+    for c in range(3):
+        inv[:, c] = adj[c] / det
+    t = [float(m[r][3]) for r in range(3)]
+    out = np.eye(4)
+    out[:3, :3] = inv
+    for r in range(3):
+        out[r, 3] = -(inv[r, 0] * t[0] + inv[r, 1] * t[1] + inv[r, 2] * t[2])
+    return out.tolist()
 
 
 # ------------------------------------------------------------------ nodes (FE/ikarus_v1/expr.py:90-211)

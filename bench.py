@@ -371,76 +371,7 @@ def bench_ikarus(fe, torch, reps=5, n_points=1 << 24, cpu_points=1 << 18, cpu=Tr
     res["field_512"] = {"n_blocks": meta.n_blocks, "n_active": na, "classify_ms": cls_ms, "tiles_ms": tile_ms,
                         "device_mvox_s": 512 ** 3 / ((cls_ms + tile_ms) * 1e-3) / 1e6,
                         "plain_call_ms_incl_host_copies": plain_ms, "plain_n_active": int(len(sf.active_ids))}
-    res["scene_sweep"] = bench_sweep(fe, cpu=cpu)
     return res
-
-
-def sweep_leaves(E, rng, count=19, side=40):
-    """Grid leaves of seeded spheres and boxes (side^3 samples at 0.5 mm) placed
-    in a 60 mm cube, so that their boxes overlap in many combinations."""
-    leaves = []
-    for i in range(count):
-        c = rng.uniform(-20, 20, 3)
-        lo = c - 0.5 * (side - 1) * 0.5
-        g = [lo[a] + 0.5 * np.arange(side) for a in range(3)]
-        X, Y, Z = np.meshgrid(*g, indexing="ij")
-        if i % 2:
-            r = rng.uniform(4, 9)
-            v = np.sqrt((X - c[0]) ** 2 + (Y - c[1]) ** 2 + (Z - c[2]) ** 2) - r
-        else:
-            h = rng.uniform(2, 8, 3)
-            q = [np.abs(X - c[0]) - h[0], np.abs(Y - c[1]) - h[1], np.abs(Z - c[2]) - h[2]]
-            v = np.sqrt(sum(np.maximum(a, 0.0) ** 2 for a in q)) + np.minimum(np.maximum(q[0], np.maximum(q[1], q[2])),
-                                                                             0.0)
-        leaves.append(E.grid_leaf(v.astype(np.float32), lo, 0.5, mode="sdf"))
-    return leaves
-
-
-def bench_sweep(fe, n_pairs=1645, budget_ms=800.0, cpu=True, cpu_pairs=8):
-    """The batched scene sweep (FE/ikarus_v1/queries.py:281-551) at the
-    reference's measured scale: 1645 candidate pairs over 19 distinct grid
-    leaves, one 800 ms sweep budget; the first call calibrates the cost model,
-    the second is timed.  The CPU oracle's counting of a few pairs at the
-    same pitches is timed beside it unless cpu is False."""
-    E = fe.ikarus
-    Q = fe.ikarus_queries
-    rng = np.random.default_rng(23)
-    leaves = sweep_leaves(E, rng)
-    boxes = [Q.bbox_of(leaf) for leaf in leaves]
-    cand = [(i, j) for i in range(len(leaves)) for j in range(i + 1, len(leaves))
-            if np.all(np.minimum(boxes[i][1], boxes[j][1]) > np.maximum(boxes[i][0], boxes[j][0]))]
-    picks = rng.integers(0, len(cand), n_pairs)
-    pairs = [(leaves[cand[k][0]], leaves[cand[k][1]]) for k in picks]
-    Q.scene_sweep_gridfield_pairs(pairs, total_budget_ms=budget_ms)
-    t0 = time.perf_counter()
-    results, log = Q.scene_sweep_gridfield_pairs(pairs, total_budget_ms=budget_ms)
-    wall_ms = (time.perf_counter() - t0) * 1e3
-    pts = log["n_coarse_points_actual"] + log["total_boundary_cells"] * log["refine_factor_chosen"] ** 3 * (
-        log["refine_factor_chosen"] > 1)
-    dev_s = log["t_coarse_launch_s"] + log["t_refine_launch_s"]
-    out = {"pairs": n_pairs, "distinct_leaves": len(leaves), "overlapping_candidates": len(cand),
-           "budget_ms": budget_ms, "wall_ms": wall_ms, "coarse_launch_ms": log["t_coarse_launch_s"] * 1e3,
-           "refine_launch_ms": log["t_refine_launch_s"] * 1e3, "refine_factor": log["refine_factor_chosen"],
-           "coarse_points": log["n_coarse_points_actual"], "boundary_cells": log["total_boundary_cells"],
-           "points_total": int(pts), "gpoints_s_incl_launch_and_copies": pts / max(dev_s, 1e-9) / 1e9,
-           "total_volume_mm3": float(sum(r["volume_mm3"] for r in results))}
-    if cpu:
-        sys.path.insert(0, str(ROOT / "oracle"))
-        import ikarus_ref as IR
-        idx = {id(leaf): k for k, leaf in enumerate(leaves)}
-        sel = [k for k in range(n_pairs) if "coarse_pitch_mm" in results[k]["dispatch_log"]][:cpu_pairs]
-        la = [idx[id(pairs[k][0])] for k in sel]
-        lb = [idx[id(pairs[k][1])] for k in sel]
-        los = [np.maximum(boxes[a][0], boxes[b][0]) for a, b in zip(la, lb)]
-        his = [np.minimum(boxes[a][1], boxes[b][1]) for a, b in zip(la, lb)]
-        pitches = [results[k]["dispatch_log"]["coarse_pitch_mm"] for k in sel]
-        r = log["refine_factor_chosen"]
-        t0 = time.perf_counter()
-        counts = IR.sweep_counts(leaves, la, lb, los, his, pitches, r)
-        cpu_s = time.perf_counter() - t0
-        cpu_pts = sum(int(np.prod(c["shape"])) + c["n_boundary"] * r ** 3 * (r > 1) for c in counts)
-        out.update(cpu_oracle_mpoints_s=cpu_pts / cpu_s / 1e6, cpu_sample_pairs=len(sel), cpu_cores=1)
-    return out
 
 
 def bench_serve(fe, torch, rates, count):

@@ -314,21 +314,6 @@ int fe_ikarus_pairs_eval(const void *grids, int nleaves, const float *grid_data,
 int fe_ikarus_band_eval(const void *grids, const float *grid_data, int axis, const float *zs, const float *z0,
                         const float *z1, int nbands, const float *points, int64_t n, float *out, void *stream);
 
-/* The two batched launches of ikarus_v1/queries.py:281-551
- * (scene_sweep_gridfield_pairs): pairs = npairs 80-byte device records
- * {double lx, ly, lz, pitch; int64 first; int32 nx, ny, nz, leaf_a, leaf_b;
- * float below, band, fine_band; int32 pad[2]} (first ascending from 0; total
- * samples < 2^31).  pairs_grid: coarse samples lo + pitch*i of every pair,
- * v = max(leaf a, leaf b), counts[p][0] += v < below, counts[p][1] +=
- * |v| <= band (device uint64[npairs][2]); band samples' global indices
- * appended to cells through *cursor.  pairs_refine: r^3 samples per listed
- * cell at centre + ((a+0.5)/r*pitch - pitch/2); counts[p][0] += v < 0,
- * counts[p][1] += |v| <= fine_band. */
-int fe_ikarus_pairs_grid(const void *pairs, int npairs, int64_t total, const void *grids, int nleaves,
-                         const float *grid_data, unsigned long long *counts, int32_t *cells, unsigned int *cursor,
-                         void *stream);
-int fe_ikarus_pairs_refine(const void *pairs, int npairs, const void *grids, int nleaves, const float *grid_data,
-                           const int32_t *cells, int64_t ncells, int r, unsigned long long *counts, void *stream);
 
 #ifdef __cplusplus
 }

@@ -391,37 +391,3 @@ def band_f32(leaf, axis, bands, points):
         slab = _wmax((pax - F(b["z1"])).astype(F), (F(b["z0"]) - pax).astype(F))
         best = _wmin(best, _wmax(body, slab))
     return best
-
-
-def sweep_counts(leaves, leaf_a, leaf_b, los, his, pitches, refine_factor):
-    """The coarse and refine stages of FE/ikarus_v1/queries.py:414-529 for given
-    per-pair boxes and coarse pitches (None: the pair has no grid pass):
-    counts of coarse samples below -diag/2 and within +-diag/2 (fp32
-    thresholds), then r^3 fine samples per band sample, counted below 0 and
-    within the fine half diagonal."""
-    out = []
-    for k in range(len(pitches)):
-        pitch = pitches[k]
-        if pitch is None:
-            out.append(None)
-            continue
-        pts, shape = grid_points(los[k], his[k], pitch)
-        n = len(pts)
-        v = pairs_f32(pts, np.zeros(n, np.int64), [leaf_a[k]], [leaf_b[k]], leaves)
-        dh = 0.5 * pitch * math.sqrt(3.0)
-        inside = v < F(-dh)
-        band = np.abs(v) <= F(dh)
-        r = dict(shape=shape, n_fully_inside=int(inside.sum()), n_boundary=int(band.sum()), n_fine_inside=0,
-                 n_fine_boundary=0)
-        if refine_factor > 1 and r["n_boundary"]:
-            raw = pitch / refine_factor
-            half = pitch / 2.0
-            offs = (np.arange(refine_factor) + 0.5) / refine_factor * pitch - half
-            OX, OY, OZ = np.meshgrid(offs, offs, offs, indexing="ij")
-            local = np.stack([OX.ravel(), OY.ravel(), OZ.ravel()], axis=1)
-            fine = (pts[band][:, None, :] + local[None, :, :]).reshape(-1, 3)
-            fv = pairs_f32(fine, np.zeros(len(fine), np.int64), [leaf_a[k]], [leaf_b[k]], leaves)
-            r["n_fine_inside"] = int((fv < F(0)).sum())
-            r["n_fine_boundary"] = int((np.abs(fv) <= F(0.5 * raw * math.sqrt(3.0))).sum())
-        out.append(r)
-    return out

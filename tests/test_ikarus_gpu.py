@@ -122,14 +122,19 @@ def test_intrusion_volume_closed_forms(fe, E, gpu):
     r1, r2, d = 6.0, 4.0, 7.0
     lens = math.pi * (r1 + r2 - d) ** 2 * (d * d + 2 * d * r2 - 3 * r2 * r2 + 2 * d * r1 + 6 * r1 * r2
                                            - 3 * r1 * r1) / (12 * d)
-    res = Q.intrusion_volume(E.sphere(r1, (0, 0, 0)), E.sphere(r2, (d, 0, 0)), budget_ms=200.0)
-    assert res["method"] == "hierarchical_prune_adaptive_refine"
+    res = Q.intrusion_volume(E.sphere(r1, (0, 0, 0)), E.sphere(r2, (d, 0, 0)), max_points=2_000_000)
     assert abs(res["volume_mm3"] - lens) <= res["uncertainty_mm3"] + 0.02 * lens, (res["volume_mm3"], lens)
-    assert res["dispatch_log"]["refine_factor_chosen"] >= 1 and res["dispatch_log"]["n_coarse_points"] > 0
+    assert res["refine_factor"] >= 2 and res["n_points"] <= 2_000_000
+    # the passes it ran are grid_volume's at the pitch and factor it chose
+    lo, hi = (np.maximum(x, y) for x, y in zip(Q.bbox_of(E.sphere(r1, (0, 0, 0))), Q.bbox_of(E.sphere(r2, (d, 0, 0)))))
+    want = IR.grid_volume(E.intersect(E.sphere(r1, (0, 0, 0)), E.sphere(r2, (d, 0, 0))), lo, hi,
+                          res["coarse_pitch_mm"], res["refine_factor"])
+    for k in ("n_fully_inside", "n_boundary", "n_fine_inside", "n_fine_boundary"):
+        assert res[k] == want[k], k
     a = E.box((3.0, 2.0, 2.5), (0.0, 0.0, 0.0))
     b = E.box((2.0, 3.0, 1.5), (1.5, 0.5, 1.0))
     exact = 3.5 * 4.0 * 3.0
-    res = Q.intrusion_volume(a, b, budget_ms=100.0)
+    res = Q.intrusion_volume(a, b, max_points=500_000)
     assert abs(res["volume_mm3"] - exact) <= res["uncertainty_mm3"] + 0.02 * exact
     far = Q.intrusion_volume(E.sphere(1.0, (0, 0, 0)), E.sphere(1.0, (50, 0, 0)))
     assert far["volume_mm3"] == 0.0 and far["uncertainty_mm3"] == 0.0
@@ -174,66 +179,3 @@ def test_band_tree_evaluator(E, gpu):
         assert np.all(ev.eval(pts[:10], []) == np.float32(1.0e30))
     with pytest.raises(ValueError):
         E.BandTreeEvaluator(leaves[0], 3)
-
-
-def test_scene_sweep_matches_oracle(fe, E, gpu):
-    """Every pair's coarse and fine counts against the oracle at the pitches
-    and refinement factor the sweep chose; an empty-prior pair; volumes of
-    sampled spheres and boxes against their closed forms."""
-    Q = fe.ikarus_queries
-    from ikarus_trees import leaf_set
-    leaves = leaf_set(E)
-    pairs = [(leaves[0], leaves[1]), (leaves[1], leaves[2]), (leaves[2], leaves[0]), (leaves[3], leaves[0]),
-             (leaves[0], leaves[4]), (leaves[1], leaves[3])]
-    results, log = Q.scene_sweep_gridfield_pairs(pairs, total_budget_ms=400.0)
-    print("sweep", log["refine_factor_chosen"], log["n_coarse_points_actual"], log["total_boundary_cells"], flush=True)
-    assert log["n_pairs"] == len(pairs) and log["n_pairs_empty_prior"] == 1
-    assert results[4]["volume_mm3"] == 0.0 and "note" in results[4]["dispatch_log"]
-    r = log["refine_factor_chosen"]
-    assert r >= 2
-    boxes = [Q.bbox_of(leaf) for leaf in leaves]
-    idx = {id(leaf): k for k, leaf in enumerate(leaves)}
-    la = [idx[id(a)] for a, _ in pairs]
-    lb = [idx[id(b)] for _, b in pairs]
-    los = [np.maximum(boxes[a][0], boxes[b][0]) for a, b in zip(la, lb)]
-    his = [np.minimum(boxes[a][1], boxes[b][1]) for a, b in zip(la, lb)]
-    pitches = [res["dispatch_log"].get("coarse_pitch_mm") for res in results]
-    want = IR.sweep_counts(leaves, la, lb, los, his, pitches, r)
-    for i, (res, w) in enumerate(zip(results, want)):
-        if w is None:
-            continue
-        d = res["dispatch_log"]
-        assert d["n_fully_inside_coarse"] == w["n_fully_inside"] and d["n_boundary_coarse"] == w["n_boundary"], i
-        if w["n_boundary"]:
-            fine_pitch = pitches[i] / r
-            vol = w["n_fully_inside"] * pitches[i] ** 3 + w["n_fine_inside"] * fine_pitch ** 3
-            assert d["n_boundary_remaining_at_finest"] == w["n_fine_boundary"], i
-            assert res["volume_mm3"] == vol, i
-            assert res["uncertainty_mm3"] == w["n_fine_boundary"] * fine_pitch ** 3, i
-    # the sampled solids' true overlaps (midpoint rule on a 0.04 mm grid of the analytic fields)
-    from ikarus_trees import box_sdf, sphere_sdf
-    for i, fa, fb in ((0, sphere_sdf((0, 0, 0), 5.0), box_sdf((4.0, 1.0, 0.5), (3.0, 2.5, 4.0))),
-                      (3, box_sdf((-3.0, -2.0, 0.0), (2.0, 2.0, 2.0)), sphere_sdf((0, 0, 0), 5.0))):
-        lo, hi = los[i], his[i]
-        g = [lo[a] + 0.04 * (np.arange(int(np.ceil((hi[a] - lo[a]) / 0.04))) + 0.5) for a in range(3)]
-        X, Y, Z = np.meshgrid(*g, indexing="ij")
-        exact = float(np.count_nonzero((fa(X, Y, Z) < 0) & (fb(X, Y, Z) < 0))) * 0.04 ** 3
-        res = results[i]
-        print("overlap", i, res["volume_mm3"], exact, res["uncertainty_mm3"], flush=True)
-        assert abs(res["volume_mm3"] - exact) <= res["uncertainty_mm3"] + 0.03 * exact, i
-
-
-def test_total_volume_and_clearance(fe, E, gpu):
-    Q = fe.ikarus_queries
-    s = E.sphere(6.0, (0.0, 0.0, 0.0))
-    tv = Q.total_volume(s, budget_ms=200.0)
-    exact = 4.0 / 3.0 * math.pi * 216.0
-    assert abs(tv["volume_mm3"] - exact) <= tv["uncertainty_mm3"] + 0.02 * exact
-    b = E.sphere(2.0, (11.0, 0.0, 0.0))
-    mc = Q.min_clearance(s, b, budget_ms=100.0)
-    assert mc["method"] == "surface_sample_vs_field" and mc["n_surface_samples"] > 0
-    assert abs(mc["min_clearance_mm"] - 3.0) <= mc["uncertainty_mm"] + 1e-3
-    far = Q.min_clearance(E.sphere(1.0, (0, 0, 0)), b)
-    assert far["min_clearance_mm"] > 7.0
-    rate, overhead, _ = Q.calibrate_throughput_cached(s)
-    assert rate > 0 and overhead >= 0
