@@ -25,6 +25,7 @@ SIGNATURES = {
     "fe_edt_squared_host": (ct.c_int, [P, ct.c_int, ct.c_int, ct.c_int, P]),
     "fe_edt_pair_create": (P, [ct.c_int, ct.c_int, ct.c_int]),
     "fe_edt_device": (ct.c_int, [P, ct.c_int, ct.c_int, ct.c_int, ct.c_int, P, P, P]),
+    "fe_edt_device_static": (ct.c_int, [P, ct.c_int, ct.c_int, ct.c_int, ct.c_int, P, P]),
     "fe_edt_signed_device": (ct.c_int, [P, I64, F64, P, P]),
     "fe_edt_axis_pass": (ct.c_int, [P, P, ct.c_int, ct.c_int, ct.c_int, ct.c_int, ct.c_int, P]),
     "fe_edt_batch_create": (P, [P, P, P, P, ct.c_int, ct.c_int, I64, I64, P]),

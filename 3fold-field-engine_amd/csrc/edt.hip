@@ -31,6 +31,7 @@
 #include <cstdio>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <utility>
 #include <vector>
 
@@ -947,6 +948,57 @@ extern "C" int fe_edt_device(const uint8_t* mask, int nx, int ny, int nz, int pl
     FE_TRY(run_passes(mask, work, out, nullptr, 0, one, planes, ny, nx, tiles, compact, zsum,
                       static_cast<hipStream_t>(stream)));
     return FE_OK;
+}
+
+// ---------------------------------------------------------------- no-allocation variant
+// fe_edt_device with its `work` planes in a file-scope device array instead of
+// a caller workspace, for callers that cannot allocate (graph capture, serving
+// loops).  The array holds planes * N <= FE_EDT_SCRATCH_WORDS int32 words
+// (8 MiB: both planes of the reference EDT's 2^20-voxel cap); every GPU the
+// library runs on keeps its own copy until the process ends.  Host code uses
+// the per-device address from hipGetSymbolAddress, never the array's name.
+// Nothing in it must start at zero (the y pass writes every word it reads
+// back).  Calls that overlap on one GPU share the array and race.
+#define FE_EDT_SCRATCH_WORDS (1 << 21)
+#define FE_MAX_DEVICES 64
+static __device__ __align__(256) int32_t scratch[FE_EDT_SCRATCH_WORDS];
+
+namespace {
+std::mutex scratch_lock;
+int32_t* scratch_ptr[FE_MAX_DEVICES];  // per device, set once with that device current
+
+hipError_t scratch_words(int32_t** out)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= FE_MAX_DEVICES) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> hold(scratch_lock);
+    if (!scratch_ptr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        scratch_ptr[dev] = static_cast<int32_t*>(p);
+    }
+    *out = scratch_ptr[dev];
+    return hipSuccess;
+}
+}  // namespace
+
+extern "C" int fe_edt_device_static(const uint8_t* mask, int nx, int ny, int nz, int planes, int32_t* out,
+                                    void* stream) noexcept
+{
+    if (!mask || !out || nx < 1 || ny < 1 || nz < 1 || nx > FE_EDT_MAX_AXIS || ny > FE_EDT_MAX_AXIS ||
+        nz > FE_EDT_MAX_AXIS || (planes != 1 && planes != 2))
+        return FE_INVALID;
+    if ((int64_t)planes * nx * ny * nz > FE_EDT_SCRATCH_WORDS) {
+        snprintf(fe_err_buf, sizeof fe_err_buf, "fe_edt_device_static: %d plane(s) of %lld voxels exceed the %d-word "
+                 "scratch", planes, (long long)nx * ny * nz, FE_EDT_SCRATCH_WORDS);
+        return FE_INVALID;
+    }
+    int32_t* work = nullptr;
+    FE_TRY(scratch_words(&work));
+    return fe_edt_device(mask, nx, ny, nz, planes, work, out, stream);
 }
 
 // One axis pass of fe_edt_device, for profiling and per-kernel timing, with

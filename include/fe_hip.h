@@ -54,6 +54,15 @@ void *fe_edt_pair_create(int nx, int ny, int nz);
 int fe_edt_device(const uint8_t *mask, int nx, int ny, int nz, int planes, int32_t *work, int32_t *out,
                   void *stream);
 
+/* fe_edt_device without the `work` argument, allocating nothing: its work
+ * planes live in a static device array of 2^21 int32 words (8 MiB, both
+ * planes of the reference EDT's 2^20-voxel cap) that each GPU keeps until the
+ * process ends.  No counterpart in the reference; it is for callers that
+ * cannot allocate (code under HIP graph capture, tight serving loops).
+ * Returns 1 when planes*nx*ny*nz exceeds 2^21.  Calls that overlap on one
+ * GPU share the array and race. */
+int fe_edt_device_static(const uint8_t *mask, int nx, int ny, int nz, int planes, int32_t *out, void *stream);
+
 /* One axis pass of fe_edt_device (per-kernel timing and profiling), with the
  * intermediate layouts fe_edt_device uses: axis 2 reads the uint8 mask and
  * writes the z pass (z summaries when nz >= 16, else int32 planes), axis 1
