@@ -62,6 +62,9 @@ SIGNATURES = {
     "fe_unpack": (ct.c_int, [P, I64, P, ct.c_int, ct.c_int, ct.c_int, P, P]),
     "csg_gather": (ct.c_int, [P, I64, P, I64, P, P, P]),
     "csg_execute": (ct.c_int, [P, P, P, P, I32, P, P, I32, P]),
+    "fe_csg_check": (ct.c_int, [P, I64, P, I64, P, P, P]),
+    "fe_csg_workspace_bytes": (I64, [ct.c_int32, P, P, ct.c_int32]),
+    "fe_csg_run": (ct.c_int, [P, P, P, P, ct.c_int32, P, P, ct.c_int32, P, P, P, I64, P]),
     "fe_csg_device": (ct.c_int, [P, P, P, P, I32, P, P, I32, P, P]),
     # mesh stage (reference-shaped)
     "columns_create": (ct.c_int, [ct.c_char_p, P, U32, U32, U32, P, ct.POINTER(P)]),

@@ -91,7 +91,9 @@ class AdaptiveSampleBlocks:
         else:
             if samples.dtype != torch.float32 or samples.dim() != 3 or samples.numel() == 0:
                 raise ValueError("Nonempty three-dimensional float32 samples required")
-            if max(samples.shape) > 4096 or samples.numel() > 16777216:
+            # a device field keeps the format's 4096-per-axis bound only; the
+            # reference's 2^24-sample cap applies to host arrays (drop-in path)
+            if max(samples.shape) > 4096:
                 raise ValueError("Finite bounded field required")
             dense = samples.contiguous()
         self.d_nodes, self.d_payload = pack_device(dense, leaf_samples)

@@ -73,24 +73,60 @@ def _program(fields, instructions):
     return fields, enc
 
 
+_WORKSPACE = {}  # device -> uint8 tensor, grown as programs need
+
+
+def _first_bad(field, shape):
+    """fe_csg_check of a field's device table for a grid of `shape`, run once
+    per (table, shape) and cached on the field (uploading new host arrays
+    invalidates it)."""
+    d_nodes, d_payload = field._device()
+    key = (field._dev_key, tuple(shape))
+    cached = getattr(field, "_csg_check", None)
+    if cached is not None and cached[0] == key:
+        return cached[1]
+    sh = np.array(shape, np.int32)
+    out = ct.c_int32(0)
+    code = _lib.lib().fe_csg_check(d_nodes.data_ptr(), d_nodes.shape[0], d_payload.data_ptr(), d_payload.numel(),
+                                   sh.ctypes.data, ct.byref(out), _lib.stream_handle(None))
+    if code == 2:
+        raise RuntimeError("Native CSG operand check failed: " + _lib.lib().fe_last_error().decode())
+    value = out.value if code == 0 else 0
+    field._csg_check = (key, value)
+    return value
+
+
 def evaluate_dense(fields, instructions):
-    """Runs the program on the device; returns the dense device float32 result."""
+    """Runs the program on the device; returns the dense device float32 result.
+    Operands are checked once (fe_csg_check, cached per field); the program
+    then runs in a cached device workspace (fe_csg_run)."""
     torch = _torch()
     fields, enc = _program(fields, instructions)
     if sys.byteorder != 'little':
         raise ValueError('Little-endian host required')
+    shape = np.array(fields[0].shape, np.int32)
+    bad = np.array([_first_bad(f, fields[0].shape) for f in fields], np.int32)
     dev = [f._device() for f in fields]
     n = len(fields)
     nodes = (ct.c_void_p * n)(*[d[0].data_ptr() for d in dev])
     pays = (ct.c_void_p * n)(*[d[1].data_ptr() for d in dev])
-    counts = np.array([len(f.nodes) for f in fields], np.int64)
-    lengths = np.array([len(f.payload) for f in fields], np.int64)
-    shape = np.array(fields[0].shape, np.int32)
-    out = torch.empty(fields[0].shape, dtype=torch.float32, device=dev[0][0].device)
-    code = _lib.lib().fe_csg_device(nodes, counts.ctypes.data, pays, lengths.ctypes.data, n, shape.ctypes.data,
-                                    enc.ctypes.data, len(enc), _lib.ptr(out), _lib.stream_handle(None))
+    counts = np.array([d[0].shape[0] for d in dev], np.int64)
+    lengths = np.array([d[1].numel() for d in dev], np.int64)
+    L = _lib.lib()
+    need = L.fe_csg_workspace_bytes(n, shape.ctypes.data, enc.ctypes.data, len(enc))
+    if need < 0:
+        raise ValueError('Native CSG validation failed: program')
+    device = dev[0][0].device
+    ws = _WORKSPACE.get(device)
+    if ws is None or ws.numel() < need:
+        ws = _WORKSPACE[device] = torch.empty(max(need, 1), dtype=torch.uint8, device=device)
+    out = torch.empty(fields[0].shape, dtype=torch.float32, device=device)
+    code = L.fe_csg_run(nodes, counts.ctypes.data, pays, lengths.ctypes.data, n, shape.ctypes.data, enc.ctypes.data,
+                        len(enc), bad.ctypes.data, _lib.ptr(out), _lib.ptr(ws), ws.numel(), _lib.stream_handle(None))
     if code == 4:
         raise ValueError('Operation exceeds finite float32 range')
+    if code == 2:
+        raise RuntimeError('Native CSG failed: ' + L.fe_last_error().decode())
     if code:
         raise ValueError(f'Native CSG validation failed: {code}')
     return fields[0], out

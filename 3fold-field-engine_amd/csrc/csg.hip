@@ -23,6 +23,8 @@
 #include <climits>
 #include <cmath>
 #include <cstring>
+#include <functional>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -155,65 +157,76 @@ __device__ __forceinline__ int ref_tile(int x, int y, int z, int nty, int ntz)
     return ((x >> 4) * nty + (y >> 5)) * ntz + (z >> 4);
 }
 
+// Persistent tiles: workgroup b runs output tiles b, b + G, ...  Slots below
+// lds_slots live in LDS; the others (programs with more simultaneously live
+// values than LDS holds) in this workgroup's spill_slots * TS floats of
+// `arena`.  err_nf receives the first reference tile with a non-finite result.
 __global__ __launch_bounds__(256) void k_csg_tiles(const FieldRef* __restrict__ fields, const MOp* __restrict__ prog,
-                                                   int nops, int final_slot, int nx, int ny, int nz, int gty, int gtz,
+                                                   int nops, int final_slot, int lds_slots, int spill_slots,
+                                                   float* __restrict__ arena, int nx, int ny, int nz, int gty, int gtz,
                                                    int ntiles, float* __restrict__ out, int* __restrict__ err_nf)
 {
     extern __shared__ float slots[];
     __shared__ int cur[FRONT], nxt[FRONT], leaves[FRONT];
     __shared__ int counters[4];
     __shared__ int bad_tile;
-    const int t = blockIdx.x;
-    if (t >= ntiles) return;
-    const int tz = t % gtz, ty = (t / gtz) % gty, tx = t / (gtz * gty);
-    const int tlo[3] = {tx * TX, ty * TY, tz * TZ};
-    const int thi[3] = {min(tlo[0] + TX, nx), min(tlo[1] + TY, ny), min(tlo[2] + TZ, nz)};
-    const int nty = (ny + 31) >> 5, ntz = (nz + 15) >> 4;
     const int tid = threadIdx.x;
+    float* spill = arena + (size_t)blockIdx.x * spill_slots * TS;
+    auto slot = [&](int s) -> float* { return s < lds_slots ? slots + s * TS : spill + (size_t)(s - lds_slots) * TS; };
+    const int nty = (ny + 31) >> 5, ntz = (nz + 15) >> 4;
     if (tid == 0) bad_tile = INT_MAX;
     __syncthreads();
-    for (int o = 0; o < nops; ++o) {
-        const MOp m = prog[o];
-        float* dst = slots + m.dst * TS;
-        if (m.kind == 0) {
-            read_tile(fields[m.a], tlo, thi, dst, cur, nxt, leaves, counters);
-            continue;
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const int tz = t % gtz, ty = (t / gtz) % gty, tx = t / (gtz * gty);
+        const int tlo[3] = {tx * TX, ty * TY, tz * TZ};
+        const int thi[3] = {min(tlo[0] + TX, nx), min(tlo[1] + TY, ny), min(tlo[2] + TZ, nz)};
+        for (int o = 0; o < nops; ++o) {
+            const MOp m = prog[o];
+            float* dst = slot(m.dst);
+            if (m.kind == 0) {
+                read_tile(fields[m.a], tlo, thi, dst, cur, nxt, leaves, counters);
+                continue;
+            }
+            // elementwise: dst may be one of the operands' slots
+            const float* A = slot(m.a);
+            const float* Bv = slot(m.b);
+            int bad = INT_MAX;
+            for (int j = tid; j < TS; j += 256) {
+                const int z = j % TZ, y = (j / TZ) % TY, x = j / (TZ * TY);
+                const int gx = tlo[0] + x, gy = tlo[1] + y, gz = tlo[2] + z;
+                if (gx >= thi[0] || gy >= thi[1] || gz >= thi[2]) continue;
+                const float a = A[j];
+                float r;
+                if (m.kind == 3) r = (float)((double)a - m.off);
+                else {
+                    const float b = m.kind == 1 ? Bv[j] : -Bv[j];
+                    r = m.kind == 1 ? (a < b ? a : b) : (a > b ? a : b);
+                }
+                if (!isfinite(r)) bad = min(bad, ref_tile(gx, gy, gz, nty, ntz));
+                dst[j] = r;
+            }
+            if (bad != INT_MAX) atomicMin(&bad_tile, bad);
+            __syncthreads();
         }
-        const float* A = slots + m.a * TS;
-        const float* Bv = slots + m.b * TS;
-        int bad = INT_MAX;
+        const float* F = slot(final_slot);
         for (int j = tid; j < TS; j += 256) {
             const int z = j % TZ, y = (j / TZ) % TY, x = j / (TZ * TY);
             const int gx = tlo[0] + x, gy = tlo[1] + y, gz = tlo[2] + z;
-            if (gx >= thi[0] || gy >= thi[1] || gz >= thi[2]) continue;
-            const float a = A[j];
-            float r;
-            if (m.kind == 3) r = (float)((double)a - m.off);
-            else {
-                const float b = m.kind == 1 ? Bv[j] : -Bv[j];
-                r = m.kind == 1 ? (a < b ? a : b) : (a > b ? a : b);
-            }
-            if (!isfinite(r)) bad = min(bad, ref_tile(gx, gy, gz, nty, ntz));
-            dst[j] = r;
+            if (gx < thi[0] && gy < thi[1] && gz < thi[2]) out[((int64_t)gx * ny + gy) * nz + gz] = F[j];
         }
-        if (bad != INT_MAX) atomicMin(&bad_tile, bad);
-        __syncthreads();
-    }
-    const float* F = slots + final_slot * TS;
-    for (int j = tid; j < TS; j += 256) {
-        const int z = j % TZ, y = (j / TZ) % TY, x = j / (TZ * TY);
-        const int gx = tlo[0] + x, gy = tlo[1] + y, gz = tlo[2] + z;
-        if (gx < thi[0] && gy < thi[1] && gz < thi[2]) out[((int64_t)gx * ny + gy) * nz + gz] = F[j];
+        __syncthreads();  // the next tile overwrites the slots
     }
     if (tid == 0 && bad_tile != INT_MAX) atomicMin(err_nf, bad_tile);
 }
 
-// Field checks with the reference's gather codes: malformed leaf records
-// (FE/csg_tile_native_v1/gather.cpp:34-41) raise `malformed`; internal records
-// must point forward to records inside the table (pre-order tables do), else
-// `malformed`; every leaf adds one to each domain sample it covers.
-__global__ __launch_bounds__(256) void k_field_check(FieldRef F, int nx, int ny, int nz, uint32_t* __restrict__ cover,
-                                                     int* __restrict__ malformed)
+// Operand check with the reference gather's codes
+// (FE/csg_tile_native_v1/gather.cpp:28-47), one workgroup per record:
+// malformed leaf records, and internal records that do not point forward to
+// records inside the table (the tile reader walks the tree), set `malformed`;
+// every leaf marks the grid samples it covers, 32 per word: a bit of `once`
+// on its first cover, of `twice` on any later one.
+__global__ __launch_bounds__(256) void k_field_check(FieldRef F, int nx, int ny, int nz, uint32_t* __restrict__ once,
+                                                     uint32_t* __restrict__ twice, int* __restrict__ malformed)
 {
     const int64_t i = blockIdx.x;
     if (i >= F.count) return;
@@ -240,29 +253,42 @@ __global__ __launch_bounds__(256) void k_field_check(FieldRef F, int nx, int ny,
     }
     const int hx = min(nd.shape[0], nx - nd.lo[0]), hy = min(nd.shape[1], ny - nd.lo[1]), hz = min(nd.shape[2], nz - nd.lo[2]);
     if (hx <= 0 || hy <= 0 || hz <= 0) return;
-    const int64_t cnt = (int64_t)hx * hy * hz;
-    for (int64_t j = threadIdx.x; j < cnt; j += 256) {
-        const int z = (int)(j % hz);
-        const int64_t r = j / hz;
-        const int y = (int)(r % hy), x = (int)(r / hy);
-        const int64_t s = ((int64_t)(nd.lo[0] + x) * ny + nd.lo[1] + y) * nz + nd.lo[2] + z;
-        if (cover[s] < 2) atomicAdd(&cover[s], 1u);  // 1 = exactly once, >= 2 = more than once
+    // rows (x, y) of the clipped box, each a run of hz consecutive samples:
+    // one atomic per 32-sample word the run touches
+    const int wpr = ((hz + 31) >> 5) + 1;
+    const int64_t items = (int64_t)hx * hy * wpr;
+    for (int64_t j = threadIdx.x; j < items; j += 256) {
+        const int k = (int)(j % wpr);
+        const int64_t row = j / wpr;
+        const int y = (int)(row % hy), x = (int)(row / hy);
+        const int64_t s0 = ((int64_t)(nd.lo[0] + x) * ny + nd.lo[1] + y) * nz + nd.lo[2];
+        const int64_t s1 = s0 + hz;  // exclusive
+        const int64_t w = (s0 >> 5) + k;
+        if (w > ((s1 - 1) >> 5)) continue;
+        const int lo = (int)(max(s0, w << 5) - (w << 5)), hi = (int)(min(s1, (w << 5) + 32) - (w << 5));
+        const uint32_t mask = (hi == 32 ? 0xffffffffu : ((1u << hi) - 1u)) & ~((1u << lo) - 1u);
+        const uint32_t dup = atomicOr(&once[w], mask) & mask;
+        if (dup) atomicOr(&twice[w], dup);
     }
 }
 
-// First reference tile containing a sample covered other than exactly once.
-__global__ __launch_bounds__(256) void k_cover_scan(const uint32_t* __restrict__ cover, int nx, int ny, int nz,
-                                                    int* __restrict__ err_gather)
+// First reference tile holding a sample covered other than exactly once.
+__global__ __launch_bounds__(256) void k_cover_scan(const uint32_t* __restrict__ once, const uint32_t* __restrict__ twice,
+                                                    int nx, int ny, int nz, int* __restrict__ err_gather)
 {
     const int nty = (ny + 31) >> 5, ntz = (nz + 15) >> 4;
-    const int64_t n = (int64_t)nx * ny * nz;
+    const int64_t n = (int64_t)nx * ny * nz, words = (n + 31) >> 5;
     int bad = INT_MAX;
-    for (int64_t s = (int64_t)blockIdx.x * 256 + threadIdx.x; s < n; s += (int64_t)gridDim.x * 256) {
-        if (cover[s] != 1) {
+    for (int64_t w = (int64_t)blockIdx.x * 256 + threadIdx.x; w < words; w += (int64_t)gridDim.x * 256) {
+        const int valid = (int)min((int64_t)32, n - (w << 5));
+        uint32_t b = ~once[w] | twice[w];
+        if (valid < 32) b &= (1u << valid) - 1u;
+        while (b) {
+            const int64_t s = (w << 5) + __builtin_ctz(b);
+            b &= b - 1;
             const int z = (int)(s % nz);
             const int64_t r = s / nz;
-            const int y = (int)(r % ny), x = (int)(r / ny);
-            bad = min(bad, ref_tile(x, y, z, nty, ntz));
+            bad = min(bad, ref_tile((int)(r / ny), (int)(r % ny), z, nty, ntz));
         }
     }
     for (int d = 32; d >= 1; d >>= 1) bad = min(bad, __shfl_xor(bad, d));
@@ -275,9 +301,10 @@ struct Instr24 {
 };
 static_assert(sizeof(Instr24) == 24, "instruction layout");
 
-// Host validation of FE/csg_batch_native_v1/execute.cpp:14-24; returns 0, 1 or 2
-// and the last-use table.
-int validate_program(int32_t fields, const int32_t* shape, const Instr24* program, int32_t n, std::vector<int>& last)
+// Program checks of FE/csg_batch_native_v1/execute.cpp:14-24; returns 0, 1 or 2
+// and the last-use table.  capped: the reference's 2^24-sample limit.
+int validate_program(int32_t fields, const int32_t* shape, const Instr24* program, int32_t n, bool capped,
+                     std::vector<int>& last)
 {
     if (!shape || !program || fields < 1 || fields > 256 || n < 1 || n > 1024) return 1;
     int64_t total = 1;
@@ -285,7 +312,7 @@ int validate_program(int32_t fields, const int32_t* shape, const Instr24* progra
         if (shape[a] < 1 || shape[a] > 4096) return 1;
         total *= shape[a];
     }
-    if (total > 16777216) return 1;
+    if (capped && total > 16777216) return 1;
     last.assign(fields + n, -1);
     for (int i = 0; i < n; ++i) {
         const Instr24& p = program[i];
@@ -302,127 +329,375 @@ int validate_program(int32_t fields, const int32_t* shape, const Instr24* progra
     return 0;
 }
 
-// Linear-scan slot allocation: inputs are read right before their first use.
-int compile_program(int32_t fields, const Instr24* program, int32_t n, const std::vector<int>& last, std::vector<MOp>& ops,
-                    int& final_slot)
+// Evaluation order of the instructions.  Results are elementwise and every
+// instruction still runs on every tile, so any order that puts operands
+// first gives the same values and the same per-tile error flags.  This one
+// keeps few values live at once: depth first from the results nobody uses
+// (in program order, the final result last), and of two operands the one
+// needing more slots (its Ershov number, counting shared values as if
+// repeated) first.
+std::vector<int> eval_order(int32_t fields, const Instr24* program, int32_t n, const std::vector<int>& last)
 {
-    std::vector<int> slot(fields + n, -1), free_slots;
+    std::vector<int> need(fields + n, 1);
+    for (int i = 0; i < n; ++i) {
+        const Instr24& p = program[i];
+        const int na = need[p.left];
+        if (p.op == 2) need[fields + i] = na;
+        else {
+            const int nb = need[p.right];
+            need[fields + i] = na == nb ? na + 1 : std::max(na, nb);
+        }
+    }
+    std::vector<char> state(n, 0);  // 0 new, 1 operands pending, 2 emitted
+    std::vector<int> order, stack;
+    order.reserve(n);
+    auto visit = [&](int root) {
+        stack.push_back(root);
+        while (!stack.empty()) {
+            const int i = stack.back();
+            if (state[i] == 2) { stack.pop_back(); continue; }
+            if (state[i] == 1) { state[i] = 2; order.push_back(i); stack.pop_back(); continue; }
+            state[i] = 1;
+            const Instr24& p = program[i];
+            int first = p.left, second = p.op == 2 ? -1 : p.right;
+            if (second >= 0 && need[second] > need[first]) std::swap(first, second);
+            // pushed in reverse: `first` is evaluated first
+            if (second >= fields && state[second - fields] == 0) stack.push_back(second - fields);
+            if (first >= fields && state[first - fields] == 0) stack.push_back(first - fields);
+        }
+    };
+    for (int i = 0; i + 1 < n; ++i)
+        if (last[fields + i] < 0) visit(i);
+    visit(n - 1);
+    return order;
+}
+
+// Slot allocation over the evaluation order: an input is read into a slot
+// right before its first use; a value's slot is free after its last use (the
+// lowest free slot is taken first, so short-lived values stay in the low,
+// LDS-resident slots); an instruction may write into the slot of an operand
+// it is the last user of.  Returns the slot count.
+int compile_program(int32_t fields, const Instr24* program, int32_t n, const std::vector<int>& order,
+                    std::vector<MOp>& ops, int& final_slot)
+{
+    std::vector<int> lastpos(fields + n, -1);
+    for (int k = 0; k < n; ++k) {
+        const Instr24& p = program[order[k]];
+        lastpos[p.left] = k;
+        if (p.op != 2) lastpos[p.right] = k;
+    }
+    lastpos[fields + n - 1] = n;
+    std::vector<int> slot(fields + n, -1);
+    std::vector<int> free_slots;  // kept sorted descending: back() is the lowest
     int nslots = 0;
     auto take = [&]() {
         if (!free_slots.empty()) {
-            int s = free_slots.back();
+            const int s = free_slots.back();
             free_slots.pop_back();
             return s;
         }
         return nslots++;
     };
-    for (int i = 0; i < n; ++i) {
+    auto release = [&](int s) {
+        free_slots.insert(std::upper_bound(free_slots.begin(), free_slots.end(), s, std::greater<int>()), s);
+    };
+    for (int k = 0; k < n; ++k) {
+        const int i = order[k];
         const Instr24& p = program[i];
         const int refs[2] = {p.left, p.op == 2 ? -1 : p.right};
-        for (int r : refs) {
+        for (int r : refs)
             if (r >= 0 && r < fields && slot[r] < 0) {
                 slot[r] = take();
                 ops.push_back({0, slot[r], r, 0, 0.0});
             }
-        }
-        // operands whose last use is this instruction free their slots after it
+        const int sa = slot[p.left], sb = p.op == 2 ? 0 : slot[p.right];
+        if (lastpos[p.left] == k) release(sa);
+        if (p.op != 2 && p.right != p.left && lastpos[p.right] == k) release(sb);
         const int d = take();
-        ops.push_back({p.op == 0 ? 1 : (p.op == 1 ? 2 : 3), d, slot[p.left], p.op == 2 ? 0 : slot[p.right],
-                       p.op == 2 ? p.offset : 0.0});
+        ops.push_back({p.op == 0 ? 1 : (p.op == 1 ? 2 : 3), d, sa, sb, p.op == 2 ? p.offset : 0.0});
         slot[fields + i] = d;
-        if (last[p.left] == i) free_slots.push_back(slot[p.left]);
-        if (p.op != 2 && p.right != p.left && last[p.right] == i) free_slots.push_back(slot[p.right]);
-        if (last[fields + i] < 0) free_slots.push_back(d);
+        if (lastpos[fields + i] < 0) release(d);
     }
     final_slot = slot[fields + n - 1];
     return nslots;
 }
 
+constexpr int LDS_SLOTS_MAX = 16;  // 128 KiB of slots (+ 12 KiB of tree lists) per workgroup
+constexpr int LDS_SLOTS_SPILL = 8;  // LDS slots of a program that spills to the arena
+
+size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
+
+// Launch geometry of a compiled program on the current device.
+struct Plan {
+    std::vector<MOp> ops;
+    int nslots = 0, final_slot = 0, lds_slots = 0, spill = 0, grid = 0, ntiles = 0, gty = 0, gtz = 0;
+    size_t lds = 0, ws_refs = 0, ws_ops = 0, ws_flags = 0, ws_arena = 0, ws_total = 0;
+};
+
+hipError_t make_plan(int32_t fields, const Instr24* program, int32_t n, const std::vector<int>& last,
+                     const int32_t* shape, Plan& P)
+{
+    const std::vector<int> order = eval_order(fields, program, n, last);
+    P.nslots = compile_program(fields, program, n, order, P.ops, P.final_slot);
+    P.lds_slots = P.nslots <= LDS_SLOTS_MAX ? P.nslots : LDS_SLOTS_SPILL;
+    P.spill = P.nslots - P.lds_slots;
+    P.lds = (size_t)P.lds_slots * TS * 4;
+    const int gtx = (shape[0] + TX - 1) / TX;
+    P.gty = (shape[1] + TY - 1) / TY;
+    P.gtz = (shape[2] + TZ - 1) / TZ;
+    const int64_t nt = (int64_t)gtx * P.gty * P.gtz;
+    if (nt > INT_MAX) return hipErrorInvalidValue;
+    P.ntiles = (int)nt;
+    int dev = 0, cus = 0, per_cu = 0;
+    hipError_t e;
+    if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
+    if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess) return e;
+    if (P.lds > 65536 &&
+        (e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_csg_tiles), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)P.lds)) != hipSuccess)
+        return e;
+    if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_csg_tiles, 256, P.lds)) != hipSuccess) return e;
+    P.grid = (int)std::min<int64_t>(nt, (int64_t)std::max(cus, 1) * std::max(per_cu, 1));
+    P.ws_refs = align256(sizeof(FieldRef) * fields);
+    P.ws_ops = align256(sizeof(MOp) * P.ops.size());
+    P.ws_flags = 256;
+    P.ws_arena = (size_t)P.grid * P.spill * TS * 4;
+    P.ws_total = P.ws_refs + P.ws_ops + P.ws_flags + P.ws_arena;
+    return hipSuccess;
+}
+
+// Runs a planned program; first_bad: the first failing reference tile of the
+// used operands' reads (INT_MAX: none).  Returns the status.
+int run_plan(const Plan& P, const std::vector<FieldRef>& refs, const int32_t* shape, int first_bad, float* output,
+             char* ws, hipStream_t st)
+{
+    FieldRef* dref = reinterpret_cast<FieldRef*>(ws);
+    MOp* dops = reinterpret_cast<MOp*>(ws + P.ws_refs);
+    int* flag = reinterpret_cast<int*>(ws + P.ws_refs + P.ws_ops);
+    float* arena = reinterpret_cast<float*>(ws + P.ws_refs + P.ws_ops + P.ws_flags);
+    const int none = INT_MAX;
+    int nf = INT_MAX;
+    FE_TRY(hipMemcpyAsync(dref, refs.data(), sizeof(FieldRef) * refs.size(), hipMemcpyHostToDevice, st));
+    FE_TRY(hipMemcpyAsync(dops, P.ops.data(), sizeof(MOp) * P.ops.size(), hipMemcpyHostToDevice, st));
+    FE_TRY(hipMemcpyAsync(flag, &none, sizeof none, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_csg_tiles, dim3((unsigned)P.grid), dim3(256), P.lds, st, dref, dops, (int)P.ops.size(),
+                       P.final_slot, P.lds_slots, P.spill, arena, shape[0], shape[1], shape[2], P.gty, P.gtz, P.ntiles,
+                       output, flag);
+    FE_TRY(hipGetLastError());
+    FE_TRY(hipMemcpyAsync(&nf, flag, sizeof nf, hipMemcpyDeviceToHost, st));
+    FE_TRY(hipStreamSynchronize(st));
+    if (first_bad == INT_MAX && nf == INT_MAX) return 0;
+    return first_bad <= nf ? 3 : 4;
+}
+
+// Checks the used operands on `stream` (cover bitmaps in `bits`, 2 * words
+// uint32; results in res[f]); first failing reference tile per operand, 0
+// for a malformed table, INT_MAX when every sample is covered exactly once.
+hipError_t check_operands(const std::vector<FieldRef>& refs, const std::vector<char>& used, const int32_t* shape,
+                          uint32_t* bits, int* dres, std::vector<int>& res, hipStream_t st)
+{
+    const int nx = shape[0], ny = shape[1], nz = shape[2];
+    const int64_t words = ((int64_t)nx * ny * nz + 31) >> 5;
+    const int nf = (int)refs.size();
+    std::vector<int> init(2 * nf, 0);
+    for (int f = 0; f < nf; ++f) init[2 * f + 1] = INT_MAX;
+    hipError_t e;
+    if ((e = hipMemcpyAsync(dres, init.data(), sizeof(int) * 2 * nf, hipMemcpyHostToDevice, st)) != hipSuccess) return e;
+    for (int f = 0; f < nf; ++f) {
+        if (!used[f]) continue;
+        if ((e = hipMemsetAsync(bits, 0, (size_t)words * 8, st)) != hipSuccess) return e;
+        hipLaunchKernelGGL(k_field_check, dim3((unsigned)refs[f].count), dim3(256), 0, st, refs[f], nx, ny, nz, bits,
+                           bits + words, dres + 2 * f);
+        hipLaunchKernelGGL(k_cover_scan, dim3((unsigned)std::min<int64_t>((words + 255) / 256, 4096)), dim3(256), 0,
+                           st, bits, bits + words, nx, ny, nz, dres + 2 * f + 1);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    std::vector<int> got(2 * nf, 0);
+    if ((e = hipMemcpyAsync(got.data(), dres, sizeof(int) * 2 * nf, hipMemcpyDeviceToHost, st)) != hipSuccess) return e;
+    if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
+    res.assign(nf, INT_MAX);
+    for (int f = 0; f < nf; ++f)
+        if (used[f]) res[f] = got[2 * f] ? 0 : got[2 * f + 1];
+    return hipSuccess;
+}
+
+// Device buffers kept between calls of the per-call entry points (grown as
+// needed, one set per device; the calls are serialised).
+struct Cache {
+    char* p = nullptr;
+    size_t n = 0;
+};
+std::mutex cache_lock;
+Cache cache_ws[64], cache_bits[64], cache_res[64], cache_host[64];
+std::mutex host_lock;  // csg_execute: its operand uploads
+
+hipError_t cached(Cache* table, size_t need, char** out)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= 64) return hipErrorInvalidDevice;
+    Cache& c = table[dev];
+    if (c.n < need) {
+        if (c.p && (e = hipFree(c.p)) != hipSuccess) return e;
+        c.p = nullptr;
+        c.n = 0;
+        if ((e = hipMalloc(&c.p, need)) != hipSuccess) return e;
+        c.n = need;
+    }
+    *out = c.p;
+    return hipSuccess;
+}
+
+bool operand_ok(const void* nodes, int64_t count, const void* payload, int64_t samples)
+{
+    return nodes && payload && count >= 1 && count <= INT32_MAX && samples >= 1;
+}
+
+std::vector<FieldRef> field_refs(const void* const* nodes, const int64_t* counts, const void* const* payloads,
+                                 const int64_t* lengths, int32_t fields)
+{
+    std::vector<FieldRef> refs(fields);
+    for (int f = 0; f < fields; ++f)
+        refs[f] = {static_cast<const Node52*>(nodes[f]), static_cast<const uint32_t*>(payloads[f]), counts[f],
+                   lengths[f]};
+    return refs;
+}
+
+// fe_csg_device with or without the reference's size caps.
+int csg_device(const void* const* nodes, const int64_t* counts, const void* const* payloads, const int64_t* lengths,
+               int32_t fields, const int32_t* shape, const void* program, int32_t n, float* output, bool capped,
+               hipStream_t st)
+{
+    if (!nodes || !counts || !payloads || !lengths || !shape || !program || !output) return 1;
+    std::vector<int> last;
+    const Instr24* prog = static_cast<const Instr24*>(program);
+    const int v = validate_program(fields, shape, prog, n, capped, last);
+    if (v) return v;
+    std::vector<char> used(fields, 0);
+    for (int f = 0; f < fields; ++f) {
+        used[f] = last[f] >= 0;
+        if (used[f] && (!operand_ok(nodes[f], counts[f], payloads[f], lengths[f]) ||
+                        (capped && (counts[f] > 33554432 || lengths[f] > 16777216))))
+            return 3;  // the reference gather's code 1 fails the first tile
+    }
+    const std::vector<FieldRef> refs = field_refs(nodes, counts, payloads, lengths, fields);
+    std::lock_guard<std::mutex> hold(cache_lock);
+    const int64_t words = ((int64_t)shape[0] * shape[1] * shape[2] + 31) >> 5;
+    char *bits = nullptr, *res = nullptr, *ws = nullptr;
+    FE_TRY(cached(cache_bits, (size_t)words * 8, &bits));
+    FE_TRY(cached(cache_res, sizeof(int) * 2 * fields, &res));
+    std::vector<int> bad;
+    FE_TRY(check_operands(refs, used, shape, reinterpret_cast<uint32_t*>(bits), reinterpret_cast<int*>(res), bad, st));
+    int first_bad = INT_MAX;
+    for (int f = 0; f < fields; ++f) first_bad = std::min(first_bad, bad[f]);
+    if (first_bad == 0) return 3;  // tile 0's reads already fail (or a table is malformed)
+    Plan P;
+    FE_TRY(make_plan(fields, prog, n, last, shape, P));
+    FE_TRY(cached(cache_ws, P.ws_total, &ws));
+    return run_plan(P, refs, shape, first_bad, output, ws, st);
+}
+
 }  // namespace
 
-// Device CSG: fields[f] = {device nodes, count, device payload, samples};
-// output: device float32 nx*ny*nz.  Status as FE/csg_batch_native_v1/execute.cpp.
-extern "C" int fe_csg_device(const void* const* nodes, const int64_t* counts, const void* const* payloads,
-                             const int64_t* lengths, int32_t fields, const int32_t* shape, const void* program,
-                             int32_t n, float* output, void* stream) noexcept
+// Validates one packed operand for programs over a grid of `shape` (once per
+// operand; fe_csg_run then reads it without checking): *first_bad = the first
+// reference tile, x-major over (16,32,16) tiles, where the reference's
+// gather of this operand fails (0 for a malformed table), INT32_MAX when
+// every grid sample is covered by exactly one leaf.  Allocates its bitmaps.
+extern "C" int fe_csg_check(const void* nodes, int64_t count, const void* payload, int64_t samples,
+                            const int32_t* shape, int32_t* first_bad, void* stream) noexcept
 {
     try {
-        if (!nodes || !counts || !payloads || !lengths || !shape || !program || !output) return 1;
-        std::vector<int> last;
-        const Instr24* prog = static_cast<const Instr24*>(program);
-        int v = validate_program(fields, shape, prog, n, last);
-        if (v) return v;
+        if (!shape || !first_bad || !operand_ok(nodes, count, payload, samples)) return 1;
+        for (int a = 0; a < 3; ++a)
+            if (shape[a] < 1 || shape[a] > 4096) return 1;
         hipStream_t st = static_cast<hipStream_t>(stream);
-        const int nx = shape[0], ny = shape[1], nz = shape[2];
-        const int64_t N = (int64_t)nx * ny * nz;
-        std::vector<FieldRef> refs(fields);
-        for (int f = 0; f < fields; ++f) {
-            refs[f] = {static_cast<const Node52*>(nodes[f]), static_cast<const uint32_t*>(payloads[f]), counts[f],
-                       lengths[f]};
-            if (last[f] >= 0 && (!nodes[f] || !payloads[f] || counts[f] < 1 || counts[f] > 33554432 || lengths[f] < 1 ||
-                                 lengths[f] > 16777216))
-                return 3;
-        }
-        std::vector<MOp> ops;
-        int final_slot = 0;
-        const int nslots = compile_program(fields, prog, n, last, ops, final_slot);
-        const size_t lds = (size_t)nslots * TS * 4;
-        if (lds > 128 * 1024) return 5;  // more simultaneously live values than the tile slots hold
-        FieldRef* dref = nullptr;
-        MOp* dops = nullptr;
-        int* flags = nullptr;  // [0] malformed, [1] first gather-error tile, [2] first non-finite tile
-        uint32_t* cover = nullptr;
-        int status = 5;
-        int hflags[3] = {0, INT_MAX, INT_MAX};
-        do {
-            if (hipMalloc(&dref, sizeof(FieldRef) * fields) != hipSuccess) break;
-            if (hipMalloc(&dops, sizeof(MOp) * ops.size()) != hipSuccess) break;
-            if (hipMalloc(&flags, 16) != hipSuccess) break;
-            if (hipMalloc(&cover, (size_t)N * 4) != hipSuccess) break;
-            if (hipMemcpyAsync(dref, refs.data(), sizeof(FieldRef) * fields, hipMemcpyHostToDevice, st) != hipSuccess) break;
-            if (hipMemcpyAsync(dops, ops.data(), sizeof(MOp) * ops.size(), hipMemcpyHostToDevice, st) != hipSuccess) break;
-            if (hipMemcpyAsync(flags, hflags, 12, hipMemcpyHostToDevice, st) != hipSuccess) break;
-            bool fail = false;
-            for (int f = 0; f < fields && !fail; ++f) {
-                if (last[f] < 0) continue;
-                if (hipMemsetAsync(cover, 0, (size_t)N * 4, st) != hipSuccess) { fail = true; break; }
-                hipLaunchKernelGGL(k_field_check, dim3((unsigned)refs[f].count), dim3(256), 0, st, refs[f], nx, ny, nz,
-                                   cover, flags);
-                hipLaunchKernelGGL(k_cover_scan, dim3((unsigned)std::min<int64_t>((N + 255) / 256, 8192)), dim3(256), 0,
-                                   st, cover, nx, ny, nz, flags + 1);
-            }
-            if (fail) break;
-            int pre[2];
-            if (hipMemcpyAsync(pre, flags, 8, hipMemcpyDeviceToHost, st) != hipSuccess) break;
-            if (hipStreamSynchronize(st) != hipSuccess) break;
-            if (pre[0]) { status = 3; break; }  // tile 0's reads already fail
-            const int gtx = (nx + TX - 1) / TX, gty = (ny + TY - 1) / TY, gtz = (nz + TZ - 1) / TZ;
-            const int ntiles = gtx * gty * gtz;
-            if (lds > 65536 &&
-                hipFuncSetAttribute(reinterpret_cast<const void*>(&k_csg_tiles), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    (int)lds) != hipSuccess)
-                break;
-            hipLaunchKernelGGL(k_csg_tiles, dim3((unsigned)ntiles), dim3(256), lds, st, dref, dops, (int)ops.size(),
-                               final_slot, nx, ny, nz, gty, gtz, ntiles, output, flags + 2);
-            if (hipGetLastError() != hipSuccess) break;
-            if (hipMemcpyAsync(hflags, flags, 12, hipMemcpyDeviceToHost, st) != hipSuccess) break;
-            if (hipStreamSynchronize(st) != hipSuccess) break;
-            const int g = hflags[1], nf = hflags[2];
-            if (g == INT_MAX && nf == INT_MAX) status = 0;
-            else status = (g <= nf) ? 3 : 4;
-        } while (0);
-        if (dref) (void)hipFree(dref);
-        if (dops) (void)hipFree(dops);
-        if (flags) (void)hipFree(flags);
-        if (cover) (void)hipFree(cover);
-        return status;
+        const int64_t words = ((int64_t)shape[0] * shape[1] * shape[2] + 31) >> 5;
+        const std::vector<FieldRef> refs = {{static_cast<const Node52*>(nodes), static_cast<const uint32_t*>(payload),
+                                             count, samples}};
+        std::lock_guard<std::mutex> hold(cache_lock);
+        char *bits = nullptr, *res = nullptr;
+        FE_TRY(cached(cache_bits, (size_t)words * 8, &bits));
+        FE_TRY(cached(cache_res, sizeof(int) * 2, &res));
+        std::vector<int> bad;
+        FE_TRY(check_operands(refs, std::vector<char>(1, 1), shape, reinterpret_cast<uint32_t*>(bits),
+                              reinterpret_cast<int*>(res), bad, st));
+        *first_bad = bad[0];
+        return 0;
     } catch (...) {
         return 5;
     }
 }
 
-// FE/csg_batch_native_v1/execute.cpp:10 (csg_execute) over host arrays:
-// upload the used operands, run fe_csg_device, download the dense output.
+// Device bytes of the workspace fe_csg_run needs for a program over `shape`
+// on the current device (-1: an invalid program, as fe_csg_run would refuse).
+extern "C" int64_t fe_csg_workspace_bytes(int32_t fields, const int32_t* shape, const void* program,
+                                          int32_t n) noexcept
+{
+    try {
+        std::vector<int> last;
+        const Instr24* prog = static_cast<const Instr24*>(program);
+        if (validate_program(fields, shape, prog, n, false, last)) return -1;
+        Plan P;
+        if (make_plan(fields, prog, n, last, shape, P) != hipSuccess) return -1;
+        return (int64_t)P.ws_total;
+    } catch (...) {
+        return -1;
+    }
+}
+
+// The program over operands checked by fe_csg_check (first_bad: their
+// results, host int32 per field; unused fields are ignored), with a caller
+// workspace of fe_csg_workspace_bytes: no allocation, no operand scan, one
+// host synchronisation for the status.  Axes <= 4096, no sample cap.
+extern "C" int fe_csg_run(const void* const* nodes, const int64_t* counts, const void* const* payloads,
+                          const int64_t* lengths, int32_t fields, const int32_t* shape, const void* program,
+                          int32_t n, const int32_t* first_bad, float* output, void* workspace, int64_t workspace_bytes,
+                          void* stream) noexcept
+{
+    try {
+        if (!nodes || !counts || !payloads || !lengths || !shape || !program || !first_bad || !output || !workspace)
+            return 1;
+        std::vector<int> last;
+        const Instr24* prog = static_cast<const Instr24*>(program);
+        const int v = validate_program(fields, shape, prog, n, false, last);
+        if (v) return v;
+        int bad = INT_MAX;
+        for (int f = 0; f < fields; ++f) {
+            if (last[f] < 0) continue;
+            if (!operand_ok(nodes[f], counts[f], payloads[f], lengths[f])) return 3;
+            bad = std::min(bad, first_bad[f] < 0 ? 0 : first_bad[f]);
+        }
+        if (bad == 0) return 3;
+        Plan P;
+        FE_TRY(make_plan(fields, prog, n, last, shape, P));
+        if ((int64_t)P.ws_total > workspace_bytes) return 1;
+        return run_plan(P, field_refs(nodes, counts, payloads, lengths, fields), shape, bad, output,
+                        static_cast<char*>(workspace), static_cast<hipStream_t>(stream));
+    } catch (...) {
+        return 5;
+    }
+}
+
+// Device CSG with a check of every used operand per call: fields[f] =
+// {device nodes, count, device payload, samples}; output: device float32
+// nx*ny*nz.  Status as FE/csg_batch_native_v1/execute.cpp:10-54; axes <= 4096,
+// no sample cap.  Buffers are kept between calls.
+extern "C" int fe_csg_device(const void* const* nodes, const int64_t* counts, const void* const* payloads,
+                             const int64_t* lengths, int32_t fields, const int32_t* shape, const void* program,
+                             int32_t n, float* output, void* stream) noexcept
+{
+    try {
+        return csg_device(nodes, counts, payloads, lengths, fields, shape, program, n, output, false,
+                          static_cast<hipStream_t>(stream));
+    } catch (...) {
+        return 5;
+    }
+}
+
+// FE/csg_batch_native_v1/execute.cpp:10 (csg_execute) over host arrays, with
+// the reference's caps: upload the used operands, run, download the output.
 extern "C" int csg_execute(const void* const* nodes, const int64_t* counts, const void* const* payload,
                            const int64_t* lengths, int32_t fields, const int32_t* shape, const void* program,
                            int32_t instructions, float* output) noexcept
@@ -430,42 +705,39 @@ extern "C" int csg_execute(const void* const* nodes, const int64_t* counts, cons
     try {
         if (!nodes || !counts || !payload || !lengths || !shape || !program || !output) return 1;
         std::vector<int> last;
-        int v = validate_program(fields, shape, static_cast<const Instr24*>(program), instructions, last);
+        int v = validate_program(fields, shape, static_cast<const Instr24*>(program), instructions, true, last);
         if (v) return v;
         const int64_t N = (int64_t)shape[0] * shape[1] * shape[2];
-        std::vector<void*> dn(fields, nullptr), dp(fields, nullptr);
-        float* dout = nullptr;
-        int status = 5;
-        do {
-            bool fail = false;
-            for (int f = 0; f < fields; ++f) {
-                if (last[f] < 0) continue;
-                if (!nodes[f] || !payload[f] || counts[f] < 1 || counts[f] > 33554432 || lengths[f] < 1 ||
-                    lengths[f] > 16777216) {
-                    status = 3;
-                    fail = true;
-                    break;
-                }
-                if (hipMalloc(&dn[f], (size_t)counts[f] * 52) != hipSuccess ||
-                    hipMalloc(&dp[f], (size_t)lengths[f] * 4) != hipSuccess ||
-                    hipMemcpy(dn[f], nodes[f], (size_t)counts[f] * 52, hipMemcpyHostToDevice) != hipSuccess ||
-                    hipMemcpy(dp[f], payload[f], (size_t)lengths[f] * 4, hipMemcpyHostToDevice) != hipSuccess) {
-                    fail = true;
-                    break;
-                }
-            }
-            if (fail) break;
-            if (hipMalloc(&dout, (size_t)N * 4) != hipSuccess) break;
-            status = fe_csg_device(const_cast<const void* const*>(dn.data()), counts,
-                                   const_cast<const void* const*>(dp.data()), lengths, fields, shape, program,
-                                   instructions, dout, nullptr);
-            if (status == 0 && hipMemcpy(output, dout, (size_t)N * 4, hipMemcpyDeviceToHost) != hipSuccess) status = 5;
-        } while (0);
+        std::vector<size_t> noff(fields, 0), poff(fields, 0);
+        size_t total = align256((size_t)N * 4);
         for (int f = 0; f < fields; ++f) {
-            if (dn[f]) (void)hipFree(dn[f]);
-            if (dp[f]) (void)hipFree(dp[f]);
+            if (last[f] < 0) continue;
+            if (!operand_ok(nodes[f], counts[f], payload[f], lengths[f]) || counts[f] > 33554432 ||
+                lengths[f] > 16777216)
+                return 3;
+            noff[f] = total;
+            total += align256((size_t)counts[f] * 52);
+            poff[f] = total;
+            total += align256((size_t)lengths[f] * 4);
         }
-        if (dout) (void)hipFree(dout);
+        std::lock_guard<std::mutex> hold_host(host_lock);
+        char* base = nullptr;
+        {
+            std::lock_guard<std::mutex> hold(cache_lock);
+            FE_TRY(cached(cache_host, total, &base));
+        }
+        std::vector<const void*> dn(fields, nullptr), dp(fields, nullptr);
+        for (int f = 0; f < fields; ++f) {
+            if (last[f] < 0) continue;
+            dn[f] = base + noff[f];
+            dp[f] = base + poff[f];
+            FE_TRY(hipMemcpy(base + noff[f], nodes[f], (size_t)counts[f] * 52, hipMemcpyHostToDevice));
+            FE_TRY(hipMemcpy(base + poff[f], payload[f], (size_t)lengths[f] * 4, hipMemcpyHostToDevice));
+        }
+        float* dout = reinterpret_cast<float*>(base);
+        const int status = csg_device(dn.data(), counts, dp.data(), lengths, fields, shape, program, instructions,
+                                      dout, true, nullptr);
+        if (status == 0) FE_TRY(hipMemcpy(output, dout, (size_t)N * 4, hipMemcpyDeviceToHost));
         return status;
     } catch (...) {
         return 5;

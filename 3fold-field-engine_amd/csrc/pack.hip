@@ -23,6 +23,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -30,39 +31,42 @@ namespace {
 
 struct PNode {
     int32_t lo[3], shape[3];
-    int32_t inv;    // bit a set: axis a invariant
-    int32_t child;  // left child index in the next level (right = child + 1), -1: none
+    int32_t inv;  // bit a set: axis a invariant
     int32_t stored[3];
     int32_t split;  // kept split
+    int64_t child;  // arena index of the left child (right = child + 1), -1: none
     int64_t reduced, direct, cost, nn, pay, idx, off;
     int32_t reach, pad;
 };
 
 __device__ __forceinline__ int64_t node_count(const PNode& p) { return (int64_t)p.shape[0] * p.shape[1] * p.shape[2]; }
 
-// AND-reduction of the three invariance flags over one part of a node.
-__global__ __launch_bounds__(256) void k_pack_inv(PNode* __restrict__ nodes, int n, int S, const uint32_t* __restrict__ d,
-                                                  int64_t sx, int64_t sy)
+// AND-reduction of the three invariance flags over the nodes of one level
+// ([lv.x, lv.x + lv.y) of the arena), S parts per node, grid-stride.
+__global__ __launch_bounds__(256) void k_pack_inv(PNode* __restrict__ arena, const int64_t* __restrict__ lv, int S,
+                                                  const uint32_t* __restrict__ d, int64_t sx, int64_t sy)
 {
-    const int node = blockIdx.x / S, part = blockIdx.x % S;
-    if (node >= n) return;
-    PNode& P = nodes[node];
-    const int64_t cnt = node_count(P);
-    const int64_t b = cnt * part / S, e = cnt * (part + 1) / S;
-    const int hy = P.shape[1], hz = P.shape[2];
-    const int64_t o = (int64_t)P.lo[0] * sx + (int64_t)P.lo[1] * sy + P.lo[2];
-    bool ex = true, ey = true, ez = true;
-    for (int64_t l = b + threadIdx.x; l < e; l += 256) {
-        const int z = (int)(l % hz);
-        const int64_t r = l / hz;
-        const int y = (int)(r % hy), x = (int)(r / hy);
-        const uint32_t v = d[o + x * sx + y * sy + z];
-        ex = ex && v == d[o + y * sy + z];
-        ey = ey && v == d[o + x * sx + z];
-        ez = ez && v == d[o + x * sx + y * sy];
+    const int64_t start = lv[0], n = lv[1];
+    for (int64_t it = blockIdx.x; it < n * S; it += gridDim.x) {
+        PNode& P = arena[start + it / S];
+        const int part = (int)(it % S);
+        const int64_t cnt = node_count(P);
+        const int64_t b = cnt * part / S, e = cnt * (part + 1) / S;
+        const int hy = P.shape[1], hz = P.shape[2];
+        const int64_t o = (int64_t)P.lo[0] * sx + (int64_t)P.lo[1] * sy + P.lo[2];
+        bool ex = true, ey = true, ez = true;
+        for (int64_t l = b + threadIdx.x; l < e; l += 256) {
+            const int z = (int)(l % hz);
+            const int64_t r = l / hz;
+            const int y = (int)(r % hy), x = (int)(r / hy);
+            const uint32_t v = d[o + x * sx + y * sy + z];
+            ex = ex && v == d[o + y * sy + z];
+            ey = ey && v == d[o + x * sx + z];
+            ez = ez && v == d[o + x * sx + y * sy];
+        }
+        const int m = (__ballot(!ex) ? 0 : 1) | (__ballot(!ey) ? 0 : 2) | (__ballot(!ez) ? 0 : 4);
+        if ((threadIdx.x & 63) == 0 && m != 7) atomicAnd(&P.inv, m);
     }
-    const int m = (__ballot(!ex) ? 0 : 1) | (__ballot(!ey) ? 0 : 2) | (__ballot(!ez) ? 0 : 4);
-    if ((threadIdx.x & 63) == 0 && m != 7) atomicAnd(&P.inv, m);
 }
 
 __device__ void init_node(PNode& c, const int* lo, const int* shape)
@@ -79,121 +83,141 @@ __device__ void init_node(PNode& c, const int* lo, const int* shape)
     c.reduced = c.direct = c.cost = c.nn = c.pay = c.idx = c.off = 0;
 }
 
-__global__ __launch_bounds__(256) void k_pack_decide(PNode* __restrict__ nodes, int n, int leaf, PNode* __restrict__ next,
-                                                     int* __restrict__ next_count)
+// Leaf-or-candidate decision of every node of level l; a candidate's two
+// children are appended at the arena cursor (ctl[0]); ctl[1] is set when the
+// arena is full (the caller then packs again with a larger one).
+__global__ __launch_bounds__(256) void k_pack_decide(PNode* __restrict__ arena, const int64_t* __restrict__ lv, int leaf,
+                                                     int64_t cap, unsigned long long* __restrict__ ctl)
 {
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
-    PNode& P = nodes[i];
-    int64_t reduced = 1;
-    for (int a = 0; a < 3; ++a) {
-        P.stored[a] = ((P.inv >> a) & 1) ? 1 : P.shape[a];
-        reduced *= P.stored[a];
-    }
-    P.reduced = reduced;
-    P.direct = 52 + 4 * reduced;
-    P.child = -1;
-    if (reduced <= 64 || node_count(P) <= leaf) return;
-    int axis = -1;
-    for (int a = 0; a < 3; ++a)
-        if (P.stored[a] > 1 && (axis < 0 || P.shape[a] > P.shape[axis])) axis = a;
-    if (axis < 0) return;
-    const int c = atomicAdd(next_count, 2);
-    int ls[3], rs[3], rl[3];
-    for (int a = 0; a < 3; ++a) {
-        ls[a] = rs[a] = P.shape[a];
-        rl[a] = P.lo[a];
-    }
-    ls[axis] = P.shape[axis] / 2;
-    rs[axis] -= ls[axis];
-    rl[axis] += ls[axis];
-    init_node(next[c], P.lo, ls);
-    init_node(next[c + 1], rl, rs);
-    P.child = c;
-}
-
-__global__ __launch_bounds__(256) void k_pack_bottom(PNode* __restrict__ nodes, int n, const PNode* __restrict__ next)
-{
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
-    PNode& P = nodes[i];
-    P.split = 0;
-    P.cost = P.direct;
-    P.nn = 1;
-    P.pay = P.reduced;
-    if (P.child < 0) return;
-    const PNode& L = next[P.child];
-    const PNode& R = next[P.child + 1];
-    const int64_t s = 52 + L.cost + R.cost;
-    if (s < P.direct) {
-        P.split = 1;
-        P.cost = s;
-        P.nn = 1 + L.nn + R.nn;
-        P.pay = L.pay + R.pay;
+    const int64_t start = lv[0], n = lv[1];
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+        PNode& P = arena[start + i];
+        int64_t reduced = 1;
+        for (int a = 0; a < 3; ++a) {
+            P.stored[a] = ((P.inv >> a) & 1) ? 1 : P.shape[a];
+            reduced *= P.stored[a];
+        }
+        P.reduced = reduced;
+        P.direct = 52 + 4 * reduced;
+        P.child = -1;
+        if (reduced <= 64 || node_count(P) <= leaf) continue;
+        int axis = -1;
+        for (int a = 0; a < 3; ++a)
+            if (P.stored[a] > 1 && (axis < 0 || P.shape[a] > P.shape[axis])) axis = a;
+        if (axis < 0) continue;
+        const unsigned long long c = atomicAdd(&ctl[0], 2ull);
+        if ((int64_t)c + 2 > cap) {
+            atomicOr(&ctl[1], 1ull);
+            continue;
+        }
+        int ls[3], rs[3], rl[3];
+        for (int a = 0; a < 3; ++a) {
+            ls[a] = rs[a] = P.shape[a];
+            rl[a] = P.lo[a];
+        }
+        ls[axis] = P.shape[axis] / 2;
+        rs[axis] -= ls[axis];
+        rl[axis] += ls[axis];
+        init_node(arena[c], P.lo, ls);
+        init_node(arena[c + 1], rl, rs);
+        P.child = (int64_t)c;
     }
 }
 
-__global__ __launch_bounds__(256) void k_pack_top(const PNode* __restrict__ nodes, int n, PNode* __restrict__ next)
+// Range of level l + 1: the children appended after level l.
+__global__ void k_pack_level(int64_t* __restrict__ lv, const unsigned long long* __restrict__ ctl, int64_t cap)
 {
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
-    const PNode& P = nodes[i];
-    if (!P.reach || !P.split) return;
-    PNode& L = next[P.child];
-    PNode& R = next[P.child + 1];
-    L.reach = R.reach = 1;
-    L.idx = P.idx + 1;
-    L.off = P.off;
-    R.idx = P.idx + 1 + L.nn;
-    R.off = P.off + L.pay;
+    const int64_t s = lv[0] + lv[1];
+    lv[2] = s;
+    lv[3] = max((int64_t)0, min((int64_t)ctl[0], cap) - s);
 }
 
-// Writes the 52-byte record (as 13 words) of every node in the final tree and
-// copies each final leaf's stored samples into the payload.
-__global__ __launch_bounds__(256) void k_pack_emit(const PNode* __restrict__ nodes, int n, const PNode* __restrict__ next,
+__global__ __launch_bounds__(256) void k_pack_bottom(PNode* __restrict__ arena, const int64_t* __restrict__ lv)
+{
+    const int64_t start = lv[0], n = lv[1];
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+        PNode& P = arena[start + i];
+        P.split = 0;
+        P.cost = P.direct;
+        P.nn = 1;
+        P.pay = P.reduced;
+        if (P.child < 0) continue;
+        const PNode& L = arena[P.child];
+        const PNode& R = arena[P.child + 1];
+        const int64_t s = 52 + L.cost + R.cost;
+        if (s < P.direct) {
+            P.split = 1;
+            P.cost = s;
+            P.nn = 1 + L.nn + R.nn;
+            P.pay = L.pay + R.pay;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_pack_top(PNode* __restrict__ arena, const int64_t* __restrict__ lv)
+{
+    const int64_t start = lv[0], n = lv[1];
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+        const PNode& P = arena[start + i];
+        if (!P.reach || !P.split) continue;
+        PNode& L = arena[P.child];
+        PNode& R = arena[P.child + 1];
+        L.reach = R.reach = 1;
+        L.idx = P.idx + 1;
+        L.off = P.off;
+        R.idx = P.idx + 1 + L.nn;
+        R.off = P.off + L.pay;
+    }
+}
+
+// Writes the 52-byte record (as 13 words) of every node of the final tree in
+// one level and copies each final leaf's stored samples into the payload.
+__global__ __launch_bounds__(256) void k_pack_emit(const PNode* __restrict__ arena, const int64_t* __restrict__ lv,
                                                    uint32_t* __restrict__ rec, uint32_t* __restrict__ payload,
                                                    const uint32_t* __restrict__ d, int64_t sx, int64_t sy)
 {
-    const int i = blockIdx.x;
-    if (i >= n) return;
-    const PNode& P = nodes[i];
-    if (!P.reach) return;
-    if (threadIdx.x == 0) {
-        uint32_t* r = rec + P.idx * 13;
-        for (int a = 0; a < 3; ++a) {
-            r[a] = (uint32_t)P.lo[a];
-            r[3 + a] = (uint32_t)P.shape[a];
-            r[6 + a] = P.split ? 0u : (uint32_t)P.stored[a];
+    const int64_t start = lv[0], n = lv[1];
+    for (int64_t i = blockIdx.x; i < n; i += gridDim.x) {
+        const PNode& P = arena[start + i];
+        if (!P.reach) continue;
+        if (threadIdx.x == 0) {
+            uint32_t* r = rec + P.idx * 13;
+            for (int a = 0; a < 3; ++a) {
+                r[a] = (uint32_t)P.lo[a];
+                r[3 + a] = (uint32_t)P.shape[a];
+                r[6 + a] = P.split ? 0u : (uint32_t)P.stored[a];
+            }
+            int64_t off = -1;
+            if (P.split) {
+                const PNode& L = arena[P.child];
+                r[9] = (uint32_t)(P.idx + 1);
+                r[10] = (uint32_t)(P.idx + 1 + L.nn);
+            } else {
+                r[9] = r[10] = 0xffffffffu;
+                off = P.off;
+            }
+            r[11] = (uint32_t)((uint64_t)off & 0xffffffffu);
+            r[12] = (uint32_t)((uint64_t)off >> 32);
         }
-        int64_t off = -1;
-        if (P.split) {
-            const PNode& L = next[P.child];
-            r[9] = (uint32_t)(P.idx + 1);
-            r[10] = (uint32_t)(P.idx + 1 + L.nn);
-        } else {
-            r[9] = r[10] = 0xffffffffu;
-            off = P.off;
+        if (P.split) continue;
+        const int s1 = P.stored[1], s2 = P.stored[2];
+        const int64_t o = (int64_t)P.lo[0] * sx + (int64_t)P.lo[1] * sy + P.lo[2];
+        for (int64_t j = threadIdx.x; j < P.reduced; j += 256) {
+            const int z = (int)(j % s2);
+            const int64_t r = j / s2;
+            const int y = (int)(r % s1), x = (int)(r / s1);
+            payload[P.off + j] = d[o + x * sx + y * sy + z];
         }
-        r[11] = (uint32_t)((uint64_t)off & 0xffffffffu);
-        r[12] = (uint32_t)((uint64_t)off >> 32);
-    }
-    if (P.split) return;
-    const int s1 = P.stored[1], s2 = P.stored[2];
-    const int64_t o = (int64_t)P.lo[0] * sx + (int64_t)P.lo[1] * sy + P.lo[2];
-    for (int64_t j = threadIdx.x; j < P.reduced; j += 256) {
-        const int z = (int)(j % s2);
-        const int64_t r = j / s2;
-        const int y = (int)(r % s1), x = (int)(r / s1);
-        payload[P.off + j] = d[o + x * sx + y * sy + z];
     }
 }
 
-__global__ __launch_bounds__(256) void k_nonfinite(const uint32_t* __restrict__ d, int64_t n, int* __restrict__ flag)
+__global__ __launch_bounds__(256) void k_nonfinite(const uint32_t* __restrict__ d, int64_t n,
+                                                   unsigned long long* __restrict__ flag)
 {
     bool bad = false;
     for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
         bad = bad || (d[i] & 0x7f800000u) == 0x7f800000u;
-    if (__ballot(bad) && (threadIdx.x & 63) == 0) atomicOr(flag, 1);
+    if (__ballot(bad) && (threadIdx.x & 63) == 0) atomicOr(flag, 1ull);
 }
 
 struct Packed {
@@ -207,39 +231,68 @@ struct Packed {
     }
 };
 
-// Device pack of `data` (device, C order, nx*ny*nz fp32).  Returns nullptr on
-// a non-finite sample or a runtime failure (fe_last_error tells which).
+// Work buffers of pack_device kept between calls (per device, grown as
+// needed): the candidate-node arena, the level ranges and three counters.
+struct PackArena {
+    char* p = nullptr;
+    size_t n = 0;
+};
+std::mutex arena_lock;
+PackArena arenas[64];
+
+hipError_t pack_arena(size_t need, char** out)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= 64) return hipErrorInvalidDevice;
+    PackArena& a = arenas[dev];
+    if (a.n < need) {
+        if (a.p && (e = hipFree(a.p)) != hipSuccess) return e;
+        a.p = nullptr;
+        a.n = 0;
+        if ((e = hipMalloc(&a.p, need)) != hipSuccess) return e;
+        a.n = need;
+    }
+    *out = a.p;
+    return hipSuccess;
+}
+
+int ceil_log2(int n)
+{
+    int k = 0;
+    while ((1 << k) < n) ++k;
+    return k;
+}
+
+// Device pack of `data` (device, C order, nx*ny*nz fp32).  The candidate
+// split tree is built level by level in one node arena with the level ranges
+// kept on the device, so the levels run back to back: the host reads the
+// tree's sizes (and the non-finite flag) once before allocating the output,
+// then waits once for the records and payload.  A tree larger than the arena
+// (rare: the first arena holds 4 nodes per leaf-sized piece of the grid) is
+// built again in one large enough for any tree.
 Packed* pack_device(const float* data, int nx, int ny, int nz, int leaf, hipStream_t st, int* status)
 {
     *status = FE_RUNTIME;
     const int64_t N = (int64_t)nx * ny * nz;
     const uint32_t* d = reinterpret_cast<const uint32_t*>(data);
     const int64_t sx = (int64_t)ny * nz, sy = nz;
-    std::vector<PNode*> lv;
-    std::vector<int> cnt;
-    int* counter = nullptr;
-    int* hcounter = nullptr;
-    Packed* out = nullptr;
-    bool ok = false;
-    auto cleanup = [&]() {
-        for (PNode* p : lv) (void)hipFree(p);
-        if (counter) (void)hipFree(counter);
-        if (hcounter) (void)hipHostFree(hcounter);
-    };
-    do {
-        if (hipMalloc(&counter, 16) != hipSuccess) break;
-        if (hipHostMalloc(&hcounter, 16, hipHostMallocDefault) != hipSuccess) break;
-        if (hipMemsetAsync(counter, 0, 16, st) != hipSuccess) break;
-        {
-            const unsigned g = (unsigned)std::min<int64_t>((N + 255) / 256, 4096);
-            hipLaunchKernelGGL(k_nonfinite, dim3(g), dim3(256), 0, st, d, N, counter);
-            if (hipMemcpyAsync(hcounter, counter, 4, hipMemcpyDeviceToHost, st) != hipSuccess) break;
-            if (hipStreamSynchronize(st) != hipSuccess) break;
-            if (hcounter[0]) {
-                *status = FE_INVALID;
-                break;
-            }
+    // every split halves one axis, so no chain of splits is longer than this
+    const int depth = ceil_log2(nx) + ceil_log2(ny) + ceil_log2(nz) + 1;
+    const int64_t pieces = (N + leaf) / (leaf + 1) + 1;
+    std::lock_guard<std::mutex> hold(arena_lock);
+    for (int attempt = 0; attempt < 2; ++attempt) {
+        const int64_t cap = attempt == 0 ? 4 * pieces + 1024 : 2 * (int64_t)depth * pieces + 1024;
+        const size_t lv_bytes = (size_t)(depth + 2) * 16, ctl_bytes = 32;
+        char* base = nullptr;
+        if (hipError_t e = pack_arena(ctl_bytes + lv_bytes + (size_t)cap * sizeof(PNode), &base)) {
+            fe_note_error(e, "pack arena");
+            return nullptr;
         }
+        unsigned long long* ctl = reinterpret_cast<unsigned long long*>(base);  // [0] cursor [1] overflow [2] non-finite
+        int64_t* lv = reinterpret_cast<int64_t*>(base + ctl_bytes);            // per level {start, count}
+        PNode* arena = reinterpret_cast<PNode*>(base + ctl_bytes + lv_bytes);
         PNode root;
         std::memset(&root, 0, sizeof root);
         root.shape[0] = nx;
@@ -248,65 +301,69 @@ Packed* pack_device(const float* data, int nx, int ny, int nz, int leaf, hipStre
         root.inv = 7;
         root.child = -1;
         root.reach = 1;
-        PNode* r0 = nullptr;
-        if (hipMalloc(&r0, sizeof(PNode)) != hipSuccess) break;
-        lv.push_back(r0);
-        cnt.push_back(1);
-        if (hipMemcpyAsync(r0, &root, sizeof root, hipMemcpyHostToDevice, st) != hipSuccess) break;
-        bool fail = false;
-        while (cnt.back() > 0) {
-            const int n = cnt.back();
-            const int64_t per = std::max<int64_t>(1, N / n);
-            const int S = (int)std::max<int64_t>(1, std::min<int64_t>((per + 16383) / 16384, (1 << 30) / n));
-            hipLaunchKernelGGL(k_pack_inv, dim3((unsigned)((int64_t)n * S)), dim3(256), 0, st, lv.back(), n, S, d, sx,
-                               sy);
-            PNode* next = nullptr;
-            if (hipMalloc(&next, sizeof(PNode) * (size_t)2 * n) != hipSuccess) { fail = true; break; }
-            lv.push_back(next);
-            if (hipMemsetAsync(counter, 0, 4, st) != hipSuccess) { fail = true; break; }
-            hipLaunchKernelGGL(k_pack_decide, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, lv[lv.size() - 2], n,
-                               leaf, next, counter);
-            if (hipMemcpyAsync(hcounter, counter, 4, hipMemcpyDeviceToHost, st) != hipSuccess) { fail = true; break; }
-            if (hipStreamSynchronize(st) != hipSuccess) { fail = true; break; }
-            cnt.push_back(hcounter[0]);
+        const unsigned long long ctl0[4] = {1, 0, 0, 0};
+        std::vector<int64_t> lv0((size_t)(depth + 2) * 2, 0);
+        lv0[1] = 1;
+        hipError_t e = hipSuccess;
+        auto ok = [&](hipError_t r, const char* where) {
+            if (r != hipSuccess && e == hipSuccess) {
+                e = r;
+                fe_note_error(r, where);
+            }
+            return e == hipSuccess;
+        };
+        if (!ok(hipMemcpyAsync(ctl, ctl0, sizeof ctl0, hipMemcpyHostToDevice, st), "pack init") ||
+            !ok(hipMemcpyAsync(lv, lv0.data(), lv0.size() * 8, hipMemcpyHostToDevice, st), "pack init") ||
+            !ok(hipMemcpyAsync(arena, &root, sizeof root, hipMemcpyHostToDevice, st), "pack init"))
+            return nullptr;
+        hipLaunchKernelGGL(k_nonfinite, dim3((unsigned)std::min<int64_t>((N + 255) / 256, 4096)), dim3(256), 0, st, d, N,
+                           ctl + 2);
+        const unsigned grid_nodes = 2048;
+        for (int l = 0; l < depth; ++l) {
+            // parts per node: about 16 K samples each at this level's node size
+            const int64_t per = std::max<int64_t>(1, N >> std::min(l, 62));
+            const int S = (int)std::min<int64_t>(4096, (per + 16383) / 16384);
+            hipLaunchKernelGGL(k_pack_inv, dim3(grid_nodes * 4), dim3(256), 0, st, arena, lv + 2 * l, S, d, sx, sy);
+            hipLaunchKernelGGL(k_pack_decide, dim3(grid_nodes), dim3(256), 0, st, arena, lv + 2 * l, leaf, cap, ctl);
+            hipLaunchKernelGGL(k_pack_level, dim3(1), dim3(1), 0, st, lv + 2 * l, ctl, cap);
         }
-        if (fail) break;
-        const int L = (int)cnt.size();
-        for (int l = L - 1; l >= 0; --l) {
-            if (cnt[l] == 0) continue;
-            hipLaunchKernelGGL(k_pack_bottom, dim3((unsigned)((cnt[l] + 255) / 256)), dim3(256), 0, st, lv[l], cnt[l],
-                               l + 1 < L ? lv[l + 1] : nullptr);
-        }
+        for (int l = depth - 1; l >= 0; --l)
+            hipLaunchKernelGGL(k_pack_bottom, dim3(grid_nodes), dim3(256), 0, st, arena, lv + 2 * l);
         int64_t sizes[2];
-        if (hipMemcpyAsync(sizes, &lv[0]->nn, 16, hipMemcpyDeviceToHost, st) != hipSuccess) break;
-        if (hipStreamSynchronize(st) != hipSuccess) break;
-        out = new (std::nothrow) Packed();
-        if (!out) break;
+        unsigned long long flags[4];
+        if (!ok(hipGetLastError(), "pack levels") ||
+            !ok(hipMemcpyAsync(sizes, &arena[0].nn, 16, hipMemcpyDeviceToHost, st), "pack sizes") ||
+            !ok(hipMemcpyAsync(flags, ctl, sizeof flags, hipMemcpyDeviceToHost, st), "pack sizes") ||
+            !ok(hipStreamSynchronize(st), "pack sizes"))
+            return nullptr;
+        if (flags[2]) {
+            *status = FE_INVALID;
+            return nullptr;
+        }
+        if (flags[1]) continue;  // the arena was too small: again with the bound
+        Packed* out = new (std::nothrow) Packed();
+        if (!out) return nullptr;
         out->n_nodes = sizes[0];
         out->n_samples = sizes[1];
-        if (hipMalloc(&out->nodes, (size_t)out->n_nodes * 52) != hipSuccess) break;
-        if (hipMalloc(&out->payload, (size_t)std::max<int64_t>(1, out->n_samples) * 4) != hipSuccess) break;
-        for (int l = 0; l + 1 < L; ++l) {
-            if (cnt[l] == 0) continue;
-            hipLaunchKernelGGL(k_pack_top, dim3((unsigned)((cnt[l] + 255) / 256)), dim3(256), 0, st, lv[l], cnt[l],
-                               lv[l + 1]);
+        if (!ok(hipMalloc(&out->nodes, (size_t)out->n_nodes * 52), "pack output") ||
+            !ok(hipMalloc(&out->payload, (size_t)std::max<int64_t>(1, out->n_samples) * 4), "pack output")) {
+            delete out;
+            return nullptr;
         }
-        for (int l = 0; l < L; ++l) {
-            if (cnt[l] == 0) continue;
-            hipLaunchKernelGGL(k_pack_emit, dim3((unsigned)cnt[l]), dim3(256), 0, st, lv[l], cnt[l],
-                               l + 1 < L ? lv[l + 1] : nullptr, out->nodes, out->payload, d, sx, sy);
+        for (int l = 0; l + 1 < depth; ++l)
+            hipLaunchKernelGGL(k_pack_top, dim3(grid_nodes), dim3(256), 0, st, arena, lv + 2 * l);
+        for (int l = 0; l < depth; ++l)
+            hipLaunchKernelGGL(k_pack_emit, dim3(grid_nodes * 4), dim3(256), 0, st, arena, lv + 2 * l, out->nodes,
+                               out->payload, d, sx, sy);
+        if (!ok(hipGetLastError(), "pack emit") || !ok(hipStreamSynchronize(st), "pack emit")) {
+            delete out;
+            return nullptr;
         }
-        if (hipGetLastError() != hipSuccess) break;
-        if (hipStreamSynchronize(st) != hipSuccess) break;
-        ok = true;
         *status = FE_OK;
-    } while (0);
-    cleanup();
-    if (!ok) {
-        delete out;
-        out = nullptr;
+        return out;
     }
-    return out;
+    fe_note_error(hipErrorOutOfMemory, "pack arena bound");
+    return nullptr;
 }
 
 // ---------------------------------------------------------------- lookups
@@ -426,7 +483,8 @@ extern "C" void* fe_pack_device(const float* data, int nx, int ny, int nz, int l
     int st_local = FE_INVALID;
     int* s = status ? status : &st_local;
     *s = FE_INVALID;
-    if (!data || nx < 1 || ny < 1 || nz < 1 || leaf < 8 || leaf > 4096) return nullptr;
+    if (!data || nx < 1 || ny < 1 || nz < 1 || nx > 4096 || ny > 4096 || nz > 4096 || leaf < 8 || leaf > 4096)
+        return nullptr;
     try {
         return pack_device(data, nx, ny, nz, leaf, static_cast<hipStream_t>(stream), s);
     } catch (...) {

@@ -213,10 +213,30 @@ int csg_gather(const void *nodes, int64_t count, const void *payload, int64_t sa
 int csg_execute(const void *const *nodes, const int64_t *counts, const void *const *payloads, const int64_t *lengths,
                 int32_t fields, const int32_t *shape, const void *program, int32_t instructions, float *output);
 
-/* The same program over device-resident operands into a device dense output. */
+/* The same program over device-resident operands into a device dense output,
+ * every used operand checked on each call; axes <= 4096 with no sample cap
+ * (the reference caps stay on csg_execute).  Device buffers are kept between
+ * calls (one set per device; concurrent calls are serialised). */
 int fe_csg_device(const void *const *nodes, const int64_t *counts, const void *const *payloads,
                   const int64_t *lengths, int32_t fields, const int32_t *shape, const void *program,
                   int32_t instructions, float *output, void *stream);
+
+/* Check-once form of fe_csg_device (no reference counterpart: the reference
+ * re-gathers every operand tile by tile, csg_batch_native_v1/execute.cpp:25-30).
+ * fe_csg_check validates one operand for a grid of `shape` with the
+ * reference gather's rules (gather.cpp:28-47): *first_bad = the first
+ * reference (16,32,16) tile, x-major, whose gather of this operand fails (0
+ * for a malformed table), INT32_MAX when every sample is covered exactly once.
+ * fe_csg_run then executes a program over checked operands (first_bad: their
+ * host int32 results, per field) in a caller workspace of at least
+ * fe_csg_workspace_bytes (-1: invalid program), allocating nothing; status
+ * codes as csg_execute. */
+int fe_csg_check(const void *nodes, int64_t count, const void *payload, int64_t samples, const int32_t *shape,
+                 int32_t *first_bad, void *stream);
+int64_t fe_csg_workspace_bytes(int32_t fields, const int32_t *shape, const void *program, int32_t instructions);
+int fe_csg_run(const void *const *nodes, const int64_t *counts, const void *const *payloads, const int64_t *lengths,
+               int32_t fields, const int32_t *shape, const void *program, int32_t instructions,
+               const int32_t *first_bad, float *output, void *workspace, int64_t workspace_bytes, void *stream);
 
 /* ---------------------------------------------------------------- mesh stage */
 /* Replace the CUDA column scan of cuda_columns_v1/columns.cu:50-113 (status 0

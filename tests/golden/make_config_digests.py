@@ -7,13 +7,15 @@ from oracle/fe_oracle.c (or_classify, or_compact, or_eval_tiles: the
 restatement of FE/faltkarna_v1.py:128-190, 277, pinned by
 tests/test_oracle_golden.py).  cfg1 and cfg2 add the cropped dense field
 (or_reconstruct), its uint8 occupancy (dense < 0) and the int32 EDT of the
-occupancy: the single-plane squares (or_edt_envelope, FE/edt_native_linear_v1/
-edt.cu:8-29) and the paired squares with the signed field at pitch 1
-(or_edt_pair).  GPU tests compare the device results' digests with these:
+occupancy: the single-plane squares (or_edt_envelope,
+FE/edt_native_linear_v1/edt.cu:8-29) and the paired squares with the signed
+field at pitch 1 (or_edt_pair); cfg2 also the leaf-box pack (leaf 512) of its
+dense field (or_pack, the restatement of
+FE/adaptive_pack_native_v1/pack.cpp:18-55, without the 2^24-sample cap).  GPU tests compare the device results' digests with these:
 a bit-exact check at full size.
 
   python tests/golden/make_config_digests.py [part ...]   (parts: cfg1 cfg2
-  cfg3 cfg5 crowded; default all; about 25 minutes on 8 cores)
+  cfg2pack crowded cfg3 cfg5; default all; about 40 minutes on 8 cores)
 """
 import hashlib
 import json
@@ -83,6 +85,17 @@ def main(parts):
         data["cfg2"] = full_record(R.large_field_512(2), 512, 8)
         save(data)
         print("cfg2", round(time.time() - t0, 1), flush=True)
+    if "cfg2pack" in parts:
+        # the leaf-box pack (leaf 512) of cfg2's dense field: 2^27 samples, past
+        # the reference packer's 2^24 cap (the oracle restates it uncapped)
+        import oracle as O
+        kind, ids, tiles = (lambda r: (r[0], r[2], r[3]))(oracle_build(R.large_field_512(2), 512, 8))
+        dense = O.reconstruct(kind, ids, tiles, 512, 512, 512, 8)
+        nodes, payload = O.pack(dense, 512)
+        data["cfg2"].update(pack_nodes=sha(nodes), pack_payload=sha(payload), pack_count=int(len(nodes)),
+                            pack_samples=int(len(payload)))
+        save(data)
+        print("cfg2pack", round(time.time() - t0, 1), flush=True)
     if "crowded" in parts:
         data["crowded"] = {name: build_record(rec, n, B)[0] for name, n, B, rec in C.crowded_cases(R)}
         print("crowded", round(time.time() - t0, 1), flush=True)
@@ -115,4 +128,4 @@ def main(parts):
 
 
 if __name__ == "__main__":
-    raise SystemExit(main(sys.argv[1:] or ["cfg1", "cfg2", "crowded", "cfg3", "cfg5"]))
+    raise SystemExit(main(sys.argv[1:] or ["cfg1", "cfg2", "cfg2pack", "crowded", "cfg3", "cfg5"]))

@@ -112,6 +112,11 @@ def test_cfg2_large_field(fe, gpu, digests):
     plan.launch()
     check(plan_digests(plan, 0), ref, "cfg2 resident")
     dense, _ = full_checks(fe, plan, ref, 512)
+    # the leaf-box pack of the 2^27-sample dense field (past the reference's 2^24 cap)
+    packed = fe.AdaptiveSampleBlocks(dense, [0, 0, 0], 1.0)
+    assert (len(packed.nodes), len(packed.payload)) == (ref["pack_count"], ref["pack_samples"])
+    assert sha(packed.nodes) == ref["pack_nodes"] and sha(packed.payload) == ref["pack_payload"]
+    del packed
     emax, emean, n = fp64_error(fe, rec, dense.cpu().numpy())
     # 1e-6 relative to the coordinate scale (near the surface the value itself
     # cancels towards 0, so its own ulp is no scale for the error)
