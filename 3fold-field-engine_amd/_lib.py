@@ -39,6 +39,8 @@ SIGNATURES = {
     "fe_octree_sample": (ct.c_int, [P, P, P, P, P, I64, F64, P, I64, P, P, P]),
     # field build
     "fe_field_build": (ct.c_int, [P, ct.c_int, ct.c_int, P, ct.c_int, P, ct.c_int, P, ct.c_int, ct.c_int, P]),
+    "fe_field_build_counted": (ct.c_int, [P, ct.c_int, ct.c_int, P, ct.c_int, P, ct.c_int, P, ct.c_int, ct.c_int, P,
+                                          P]),
     "fe_field_classify": (ct.c_int, [P, ct.c_int, ct.c_int, P, ct.c_int, ct.c_int, P]),
     "fe_field_tiles_ids": (ct.c_int, [P, P, I64, P]),
     "fe_field_reconstruct": (ct.c_int, [P, ct.c_int, I64, P, P, P]),

@@ -371,8 +371,12 @@ __device__ __forceinline__ void block_box(const FieldDesc& D, int bi, int bj, in
 // superblock list holds lcap entries (>= every recipe's primitive count), each
 // wave's group list min(lcap, FE_MAX_PRIMS / 2) (a longer superblock list is
 // evaluated unrefined), so small recipes keep the block's LDS small.
+// evals (optional, device uint64[6]): [0] primitive evaluations at classified
+// lattice points, [1] those points, [4] interval evaluations of the culling
+// (in this kernel); [2], [3], [5] the same for the tile pass (k_tiles).
 __global__ __launch_bounds__(256) void k_classify(const FieldDesc* __restrict__ descs, const int32_t* __restrict__ sb_prefix,
-                                                  int R, int total, uint32_t grid, int lcap)
+                                                  int R, int total, uint32_t grid, int lcap,
+                                                  unsigned long long* __restrict__ evals)
 {
     extern __shared__ uint16_t lists_lds[];
     __shared__ Clamp2 wtot[4];
@@ -395,6 +399,7 @@ __global__ __launch_bounds__(256) void k_classify(const FieldDesc* __restrict__ 
     const int n = fits ? cull_box(D.prims, D.nprims, cx, cy, cz, rad, D.slack, list, wtot, wcnt) : D.nprims;
     const int tid = threadIdx.x, lane = tid & 63, wid = __builtin_amdgcn_readfirstlane(tid >> 6);
     if (tid == 0) D.list_len[sb] = (fits && n <= D.cap) ? n : -1;
+    if (evals && tid == 0 && fits) atomicAdd(&evals[4], (unsigned long long)D.nprims);
     if (fits && n <= D.cap)
         for (int j = tid; j < n; j += 256) D.lists[(int64_t)sb * D.cap + j] = list[j];
 
@@ -419,6 +424,7 @@ __global__ __launch_bounds__(256) void k_classify(const FieldDesc* __restrict__ 
             // few entries it would drop from a 45-point lattice's list
             gn = cull_wave(D.prims, list, n, cx, cy, cz, rad, D.slack, wl);
             gl = wl;
+            if (evals && lane == 0) atomicAdd(&evals[4], (unsigned long long)n);
         }
         const int la = lane / 9, lb = (lane / 3) % 3, lc = lane % 3;
         const float lx = D.x0 + (float)((gx + la) * B) * D.pitch;
@@ -430,6 +436,11 @@ __global__ __launch_bounds__(256) void k_classify(const FieldDesc* __restrict__ 
         const bool ok_z = lc == 0 || same((D.z0 + (float)((gz + lc - 1) * B) * D.pitch) + side, lz);
         const uint64_t lat = (1ull << 45) - 1;
         const bool lattice = (__ballot(ok_x && ok_y && ok_z) & lat) == lat;
+        if (evals && lane == 0) {
+            const unsigned long long pts = lattice ? 45ull : 128ull;
+            atomicAdd(&evals[0], pts * (unsigned long long)gn);
+            atomicAdd(&evals[1], pts);
+        }
         float lv = 0.0f;
         if (lattice) {
             float x = lx, y = ly, z = lz;
@@ -542,7 +553,8 @@ __global__ __launch_bounds__(256) void k_compact_write(const FieldDesc* __restri
 // compile-time edge: a lane's KV points of a pass then differ only in x (B = 8,
 // 16) and the compiler shares their y and z terms in every primitive formula.
 template <int B, int KV>
-__device__ __forceinline__ void tile_eval(const FieldDesc& D, int64_t a, int bl, uint16_t* wl)
+__device__ __forceinline__ void tile_eval(const FieldDesc& D, int64_t a, int bl, uint16_t* wl,
+                                          unsigned long long* __restrict__ evals)
 {
     const int lane = threadIdx.x & 63;
     constexpr int b3 = B * B * B;
@@ -560,6 +572,7 @@ __device__ __forceinline__ void tile_eval(const FieldDesc& D, int64_t a, int bl,
             float cx, cy, cz, rad;
             block_box(D, bi, bj, bk, 1, cx, cy, cz, rad);
             n = cull_wave(D.prims, src, nsrc, cx, cy, cz, rad, D.slack, wl);
+            if (evals && lane == 0) atomicAdd(&evals[5], (unsigned long long)(nsrc + (n > 8 ? n : 0)));
             // a second pass over the refined list (in place): its runs join
             // across dropped steps and are no longer cut every 64 entries
             if (n > 8) n = cull_wave(D.prims, wl, n, cx, cy, cz, rad, D.slack, wl);
@@ -568,6 +581,10 @@ __device__ __forceinline__ void tile_eval(const FieldDesc& D, int64_t a, int bl,
             list = src;
             n = nsrc;
         }
+    }
+    if (evals && lane == 0) {
+        atomicAdd(&evals[2], (unsigned long long)n * b3);
+        atomicAdd(&evals[3], (unsigned long long)b3);
     }
     float* out = D.tiles + a * b3;
     for (int v0 = 0; v0 < b3; v0 += 64 * KV) {
@@ -594,7 +611,8 @@ __device__ __forceinline__ void tile_eval(const FieldDesc& D, int64_t a, int bl,
 // wave g mod nwaves), so a recipe with fewer tiles than waves does not land
 // on the same low-numbered waves as every other recipe.
 template <int B, int KV>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_tiles(const FieldDesc* __restrict__ descs, int R)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_tiles(const FieldDesc* __restrict__ descs, int R,
+                                                                                          unsigned long long* __restrict__ evals)
 {
     __shared__ uint16_t wlist[4][FE_MAX_PRIMS / 2];
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
@@ -605,7 +623,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
         const FieldDesc& D = descs[r];
         const int64_t na = min((int64_t)D.counts[0], D.tile_cap);
         const int64_t a0 = wave >= base ? wave - base : wave - base + nwaves;
-        for (int64_t a = a0; a < na; a += nwaves) tile_eval<B, KV>(D, a, D.ids[a], wlist[wv]);
+        for (int64_t a = a0; a < na; a += nwaves) tile_eval<B, KV>(D, a, D.ids[a], wlist[wv], evals);
         base = (base + na) % nwaves;
     }
 }
@@ -619,7 +637,7 @@ __global__ __launch_bounds__(256) void k_tiles_ids(FieldDesc D, const int32_t* _
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
     const int64_t a = (int64_t)blockIdx.x * 4 + wv;
     if (a >= na) return;
-    tile_eval<B, KV>(D, a, ids[a], wlist[wv]);
+    tile_eval<B, KV>(D, a, ids[a], wlist[wv], nullptr);
 }
 
 // ---------------------------------------------------------------- reconstruct / checksum
@@ -781,32 +799,51 @@ __global__ __launch_bounds__(256) void k_tiles_dense(const float* __restrict__ d
 static int classify_lcap(int max_prims) { return (max_prims + 7) & ~7; }
 static size_t classify_lds(int lcap) { return (size_t)2 * (lcap + 4 * std::min(lcap, FE_MAX_PRIMS / 2)); }
 
-extern "C" int fe_field_build(const void* descs, int R, int B, const int32_t* sb_prefix, int total_sb,
-                              const int32_t* ch_prefix, int total_chunks, int32_t* chunk_count, int tile_waves,
-                              int max_prims, void* stream) noexcept
+static int field_build(const void* descs, int R, int B, const int32_t* sb_prefix, int total_sb,
+                       const int32_t* ch_prefix, int total_chunks, int32_t* chunk_count, int tile_waves, int max_prims,
+                       unsigned long long* evals, hipStream_t st)
 {
     if (!descs || R < 1 || (B != 4 && B != 8 && B != 16) || total_sb < 1 || total_chunks < 1 || !chunk_count ||
         max_prims < 1 || max_prims > FE_MAX_PRIMS)
         return FE_INVALID;
-    hipStream_t st = static_cast<hipStream_t>(stream);
     const FieldDesc* d = static_cast<const FieldDesc*>(descs);
     const uint32_t g = round8(total_sb);
     const int lcap = classify_lcap(max_prims);
-    hipLaunchKernelGGL(k_classify, dim3(g), dim3(256), classify_lds(lcap), st, d, sb_prefix, R, total_sb, g, lcap);
+    hipLaunchKernelGGL(k_classify, dim3(g), dim3(256), classify_lds(lcap), st, d, sb_prefix, R, total_sb, g, lcap,
+                       evals);
     FE_TRY(hipGetLastError());
     hipLaunchKernelGGL(k_compact_count, dim3(total_chunks), dim3(256), 0, st, d, ch_prefix, R, total_chunks, chunk_count);
     FE_TRY(hipGetLastError());
     hipLaunchKernelGGL(k_compact_write, dim3(total_chunks), dim3(256), 0, st, d, ch_prefix, R, total_chunks, chunk_count);
     FE_TRY(hipGetLastError());
     const int blocks = tile_waves > 0 ? (tile_waves + 3) / 4 : 2048;
-    if (B == 4) hipLaunchKernelGGL((k_tiles<4, 1>), dim3(blocks), dim3(256), 0, st, d, R);
+    if (B == 4) hipLaunchKernelGGL((k_tiles<4, 1>), dim3(blocks), dim3(256), 0, st, d, R, evals);
     // B = 8: one pass of 8 voxels per lane; with the edge a compile-time
     // constant the 8 points share y and z and fit 64 VGPRs (8 waves/SIMD)
     // without spills
-    else if (B == 8) hipLaunchKernelGGL((k_tiles<8, 8>), dim3(blocks), dim3(256), 0, st, d, R);
-    else hipLaunchKernelGGL((k_tiles<16, 4>), dim3(blocks), dim3(256), 0, st, d, R);
+    else if (B == 8) hipLaunchKernelGGL((k_tiles<8, 8>), dim3(blocks), dim3(256), 0, st, d, R, evals);
+    else hipLaunchKernelGGL((k_tiles<16, 4>), dim3(blocks), dim3(256), 0, st, d, R, evals);
     FE_TRY(hipGetLastError());
     return FE_OK;
+}
+
+extern "C" int fe_field_build(const void* descs, int R, int B, const int32_t* sb_prefix, int total_sb,
+                              const int32_t* ch_prefix, int total_chunks, int32_t* chunk_count, int tile_waves,
+                              int max_prims, void* stream) noexcept
+{
+    return field_build(descs, R, B, sb_prefix, total_sb, ch_prefix, total_chunks, chunk_count, tile_waves, max_prims,
+                       nullptr, static_cast<hipStream_t>(stream));
+}
+
+// fe_field_build that also counts the primitive evaluations (evals: device
+// uint64[6], added to; layout at k_classify).
+extern "C" int fe_field_build_counted(const void* descs, int R, int B, const int32_t* sb_prefix, int total_sb,
+                                      const int32_t* ch_prefix, int total_chunks, int32_t* chunk_count,
+                                      int tile_waves, int max_prims, unsigned long long* evals, void* stream) noexcept
+{
+    if (!evals) return FE_INVALID;
+    return field_build(descs, R, B, sb_prefix, total_sb, ch_prefix, total_chunks, chunk_count, tile_waves, max_prims,
+                       evals, static_cast<hipStream_t>(stream));
 }
 
 // Classification only (the first launch of the plain per-call path).
@@ -820,7 +857,8 @@ extern "C" int fe_field_classify(const void* descs, int R, int B, const int32_t*
     const FieldDesc* d = static_cast<const FieldDesc*>(descs);
     const uint32_t g = round8(total_sb);
     const int lcap = classify_lcap(max_prims);
-    hipLaunchKernelGGL(k_classify, dim3(g), dim3(256), classify_lds(lcap), st, d, sb_prefix, R, total_sb, g, lcap);
+    hipLaunchKernelGGL(k_classify, dim3(g), dim3(256), classify_lds(lcap), st, d, sb_prefix, R, total_sb, g, lcap,
+                       (unsigned long long*)nullptr);
     FE_TRY(hipGetLastError());
     return FE_OK;
 }

@@ -447,10 +447,31 @@ __global__ __launch_bounds__(256) void k_query(const Node52* __restrict__ nodes,
     out[i] = (float)acc;
 }
 
-// Dense reconstruction of a packed field: one workgroup per leaf writes its box.
+// Writes voxels [j0, j1) (z fastest) of a leaf's clipped box (hx, hy, hz),
+// threads striding by `step`.
+__device__ __forceinline__ void unpack_range(const Node52& nd, const uint32_t* __restrict__ payload, int hy, int hz,
+                                             int ny, int nz, int64_t j0, int64_t j1, int64_t step,
+                                             uint32_t* __restrict__ dense)
+{
+    const int64_t off = node_offset(nd);
+    for (int64_t j = j0; j < j1; j += step) {
+        const int z = (int)(j % hz);
+        const int64_t r = j / hz;
+        const int y = (int)(r % hy), x = (int)(r / hy);
+        const int64_t sx = nd.stored[0] == 1 ? 0 : x, sy = nd.stored[1] == 1 ? 0 : y, sz = nd.stored[2] == 1 ? 0 : z;
+        dense[((int64_t)(nd.lo[0] + x) * ny + nd.lo[1] + y) * nz + nd.lo[2] + z] =
+            payload[off + (sx * nd.stored[1] + sy) * nd.stored[2] + sz];
+    }
+}
+
+#define UNPACK_BIG 65536  // voxels: larger leaves are spread over the whole grid
+
+// Dense reconstruction of a packed field: one workgroup per leaf writes its
+// box; leaves of more than UNPACK_BIG voxels are listed in big[1 + i]
+// (big[0] counts them) for k_unpack_big.
 __global__ __launch_bounds__(256) void k_unpack(const Node52* __restrict__ nodes, int64_t count,
                                                 const uint32_t* __restrict__ payload, int nx, int ny, int nz,
-                                                uint32_t* __restrict__ dense)
+                                                uint32_t* __restrict__ dense, unsigned int* __restrict__ big)
 {
     const int64_t id = blockIdx.x;
     if (id >= count) return;
@@ -458,15 +479,27 @@ __global__ __launch_bounds__(256) void k_unpack(const Node52* __restrict__ nodes
     if (nd.children[0] >= 0) return;
     const int hx = min(nd.shape[0], nx - nd.lo[0]), hy = min(nd.shape[1], ny - nd.lo[1]), hz = min(nd.shape[2], nz - nd.lo[2]);
     if (hx <= 0 || hy <= 0 || hz <= 0) return;
-    const int64_t off = node_offset(nd);
     const int64_t cnt = (int64_t)hx * hy * hz;
-    for (int64_t j = threadIdx.x; j < cnt; j += 256) {
-        const int z = (int)(j % hz);
-        const int64_t r = j / hz;
-        const int y = (int)(r % hy), x = (int)(r / hy);
-        const int64_t sx = nd.stored[0] == 1 ? 0 : x, sy = nd.stored[1] == 1 ? 0 : y, sz = nd.stored[2] == 1 ? 0 : z;
-        dense[((int64_t)(nd.lo[0] + x) * ny + nd.lo[1] + y) * nz + nd.lo[2] + z] =
-            payload[off + (sx * nd.stored[1] + sy) * nd.stored[2] + sz];
+    if (cnt > UNPACK_BIG) {
+        if (threadIdx.x == 0) big[1 + atomicAdd(big, 1u)] = (unsigned int)id;
+        return;
+    }
+    unpack_range(nd, payload, hy, hz, ny, nz, threadIdx.x, cnt, 256, dense);
+}
+
+// The listed large leaves, one after another, each spread over every block.
+__global__ __launch_bounds__(256) void k_unpack_big(const Node52* __restrict__ nodes, const uint32_t* __restrict__ payload,
+                                                    int nx, int ny, int nz, uint32_t* __restrict__ dense,
+                                                    const unsigned int* __restrict__ big)
+{
+    const unsigned int n = big[0];
+    for (unsigned int i = 0; i < n; ++i) {
+        const Node52& nd = nodes[big[1 + i]];
+        const int hx = min(nd.shape[0], nx - nd.lo[0]), hy = min(nd.shape[1], ny - nd.lo[1]),
+                  hz = min(nd.shape[2], nz - nd.lo[2]);
+        const int64_t cnt = (int64_t)hx * hy * hz;
+        unpack_range(nd, payload, hy, hz, ny, nz, (int64_t)blockIdx.x * 256 + threadIdx.x, cnt,
+                     (int64_t)gridDim.x * 256, dense);
     }
 }
 
@@ -537,10 +570,23 @@ extern "C" int fe_query_packed(const void* nodes, const uint32_t* payload, int n
 extern "C" int fe_unpack(const void* nodes, int64_t count, const uint32_t* payload, int nx, int ny, int nz, float* dense,
                          void* stream) noexcept
 {
-    if (!nodes || !payload || !dense || count < 1 || nx < 1 || ny < 1 || nz < 1) return FE_INVALID;
-    hipLaunchKernelGGL(k_unpack, dim3((unsigned)count), dim3(256), 0, static_cast<hipStream_t>(stream),
-                       static_cast<const Node52*>(nodes), count, payload, nx, ny, nz, reinterpret_cast<uint32_t*>(dense));
+    if (!nodes || !payload || !dense || count < 1 || count > INT32_MAX || nx < 1 || ny < 1 || nz < 1)
+        return FE_INVALID;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    // the big-leaf list (at most one entry per leaf) lives in the pack arena,
+    // held for the whole call
+    std::lock_guard<std::mutex> hold(arena_lock);
+    char* base = nullptr;
+    FE_TRY(pack_arena((size_t)(count + 1) * 4, &base));
+    unsigned int* big = reinterpret_cast<unsigned int*>(base);
+    FE_TRY(hipMemsetAsync(big, 0, 4, st));
+    hipLaunchKernelGGL(k_unpack, dim3((unsigned)count), dim3(256), 0, st, static_cast<const Node52*>(nodes), count,
+                       payload, nx, ny, nz, reinterpret_cast<uint32_t*>(dense), big);
+    hipLaunchKernelGGL(k_unpack_big, dim3(2048), dim3(256), 0, st, static_cast<const Node52*>(nodes), payload, nx, ny,
+                       nz, reinterpret_cast<uint32_t*>(dense), big);
     FE_TRY(hipGetLastError());
+    // the arena may be reused by the next call once this stream is done
+    FE_TRY(hipStreamSynchronize(st));
     return FE_OK;
 }
 

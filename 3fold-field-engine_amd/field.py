@@ -200,6 +200,16 @@ class FieldPlan:
                                          _lib.stream_handle(stream)),
                "field build")
 
+    def launch_counted(self, evals, stream=None):
+        """launch() that also adds its primitive- and interval-evaluation counts
+        to `evals`, a device int64 tensor of 6 (layout in include/fe_hip.h,
+        fe_field_build_counted)."""
+        _check(_lib.lib().fe_field_build_counted(_lib.ptr(self.desc), len(self.items), self.B,
+                                                 _lib.ptr(self.sb_prefix), self.total_sb, _lib.ptr(self.ch_prefix),
+                                                 self.total_ch, _lib.ptr(self.chunk_count), self.tile_waves,
+                                                 self.max_prims, _lib.ptr(evals), _lib.stream_handle(stream)),
+               "counted field build")
+
     def _slice(self, pool, i, name):
         off, n = self.views[i][name]
         return pool[off:off + n]

@@ -67,38 +67,179 @@ def profiled_traffic(kernel, n):
     return None
 
 
-def cpu_baseline(fe, rec, occ_host, threads, n):
-    """The C oracle (oracle/fe_oracle.c) on host cores over the whole step:
-    classify + compaction + tiles of the full grid (same recipe, same grid),
-    then the paired EDT of the full occupancy."""
+def cpu_env():
+    """The host's CPU as lscpu reports it, and the cores this process may use."""
+    info = {}
+    try:
+        import subprocess
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=30).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            info[k.strip()] = v.strip()
+    except (OSError, ValueError):
+        pass
+    def num(key):
+        try:
+            return int(info.get(key, ""))
+        except ValueError:
+            return None
+    sockets, per_socket, per_core = num("Socket(s)"), num("Core(s) per socket"), num("Thread(s) per core")
+    physical = sockets * per_socket if sockets and per_socket else None
+    usable = len(os.sched_getaffinity(0))
+    threads_per_core = per_core or 1
+    # cores this process can run on: its CPU set, counted in physical cores
+    cores = min(physical or usable, max(1, usable // threads_per_core)) if usable else (physical or 1)
+    return {"model": info.get("Model name"), "sockets": sockets, "cores_per_socket": per_socket,
+            "threads_per_core": per_core, "physical_cores": physical, "affinity_cpus": usable, "cores_used": cores}
+
+
+def omp_threads(n):
+    """Sets the OpenMP team size of the oracle library (libgomp is shared with it)."""
+    import ctypes
+    for name in ("libgomp.so.1", "libgomp.so"):
+        try:
+            ctypes.CDLL(name).omp_set_num_threads(int(n))
+            return True
+        except OSError:
+            continue
+    return False
+
+
+def median_legs(legs, runs=5):
+    """Times each (name, fn) leg: one warm-up, then `runs` rounds in which the
+    legs alternate order (forward, reversed, ...); median seconds per leg."""
+    for _, fn in legs:
+        fn()
+    times = {name: [] for name, _ in legs}
+    for r in range(runs):
+        order = legs if r % 2 == 0 else legs[::-1]
+        for name, fn in order:
+            t0 = time.perf_counter()
+            fn()
+            times[name].append(time.perf_counter() - t0)
+    return {name: float(np.median(v)) for name, v in times.items()}
+
+
+def cpu_baseline(fe, rec, occ_host, n, env):
+    """The C oracle (oracle/fe_oracle.c: the restated field build and the
+    lower-envelope EDT) on this host's cores, on a bounded sample of the same
+    workload: 8-plane x-rows of the 512^3 grid spread along x (the same blocks,
+    recipe and arithmetic as the full build) and the paired EDT of the same
+    planes of its occupancy.  All cores (OpenMP) and one thread, each leg after a warm-up as
+    the median of 5 with the legs' order reversed every other round."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import oracle as O
-    os.environ.setdefault("OMP_NUM_THREADS", str(threads))
-    g = O.grid_record(np.float32(0), np.float32(0), np.float32(0), np.float32(1.0), n, n, n, 8,
-                      np.float32(np.sqrt(3.0) * 8))
     prims = np.ascontiguousarray(rec).view(O.PRIM)
-    t0 = time.perf_counter()
-    O.build_sparse(prims, g)
-    t_field = time.perf_counter() - t0
-    t1 = time.perf_counter()
-    O.edt_pair(occ_host, 1.0)
-    t_edt = time.perf_counter() - t1
-    vox = n ** 3
-    return {"value": vox / (t_field + t_edt) / 1e6, "unit": "Mvox/s", "cores": threads, "kind": "port",
-            "sample": f"the whole {n}^3 step on the oracle: field build {t_field:.2f} s "
-                      f"({vox / t_field / 1e6:.2f} Mvox/s) + paired EDT of the occupancy {t_edt:.2f} s "
-                      f"({vox / t_edt / 1e6:.1f} Mvox/s)",
-            "field_mvox_s": vox / t_field / 1e6, "edt_mvox_s": vox / t_edt / 1e6}
+    out = {"host": env, "kind": "port", "unit": "Mvox/s"}
+    # one block row (8 x-planes) every n/8 planes: the same blocks as the full
+    # grid's and the same share of the recipe's dense centre; one thread: a
+    # row outside it and one through its middle
+    rows_all = [k * n // 8 // 8 * 8 for k in range(8)]
+    rows_one = [rows_all[1], rows_all[4]]
+    for label, threads, rows in (("all_cores", env["cores_used"], rows_all), ("one_thread", 1, rows_one)):
+        omp_threads(threads)
+        grids = [O.grid_record(np.float32(x0), np.float32(0), np.float32(0), np.float32(1.0), 8, n, n, 8,
+                               np.float32(np.sqrt(3.0) * 8)) for x0 in rows]
+        occs = [np.ascontiguousarray(occ_host[x0:x0 + 8]) for x0 in rows]
+
+        def field():
+            for g in grids:
+                O.build_sparse(prims, g)
+
+        def edt():
+            for o in occs:
+                O.edt_pair(o, 1.0)
+        t = median_legs([("field", field), ("edt", edt)])
+        vox = 8 * len(rows) * n * n
+        out[label] = {"threads": threads, "sample_voxels": vox, "x_rows": rows,
+                      "field_s": t["field"], "edt_s": t["edt"],
+                      "field_mvox_s": vox / t["field"] / 1e6, "edt_mvox_s": vox / t["edt"] / 1e6,
+                      "step_mvox_s": vox / (t["field"] + t["edt"]) / 1e6}
+    omp_threads(env["cores_used"])
+    allc = out["all_cores"]
+    out.update(value=allc["step_mvox_s"], cores=env["cores_used"],
+               sample=f"field build of 8-plane x-rows starting at {allc['x_rows']} of the {n}^3 workload (one "
+                      f"eighth; one thread: rows at {out['one_thread']['x_rows']}) + paired EDT of the same planes "
+                      f"of its occupancy; median of 5 after a warm-up, legs alternately reversed")
+    return out
 
 
-def bench_batch(fe, torch, count, plain_sample):
-    """Uneven recipes (seed 3) in one graph-captured launch vs sequential plain calls."""
+def field_counts(fe, torch, plan):
+    """Primitive evaluations of one build (device counters): the culled
+    per-point primitive count P_eff in the classify and tile passes beside the
+    recipe's own P, and the culling's interval evaluations."""
+    ev = torch.zeros(6, dtype=torch.int64, device=plan.device)
+    plan.launch_counted(ev)
+    c = ev.cpu().numpy().astype(np.float64)
+    P = plan.max_prims
+    return {"prims": P, "classify_points": int(c[1]), "classify_evals": int(c[0]),
+            "classify_p_eff": c[0] / max(c[1], 1), "tile_voxels": int(c[3]), "tile_evals": int(c[2]),
+            "tile_p_eff": c[2] / max(c[3], 1), "classify_interval_evals": int(c[4]),
+            "tile_interval_evals": int(c[5]), "unculled_evals": int(P * (c[1] + c[3]))}
+
+
+def bench_cfg1(fe, torch, cpu=True, env=None, reps=5):
+    """The 128^3 part (40 spheres + 24 boxes, seed 1, block 8) and the int32
+    EDT of its occupancy: the C oracle on the host (all cores and one thread),
+    the plain per-call GPU path (classify, host compaction, tiles, host
+    occupancy, host-copy EDT call: the reference's structure) and the resident
+    device path."""
+    dev = torch.device("cuda:0")
+    rec = fe.recipes.plain_part_128(1)
+    meta = fe.GridMeta(0.0, 0.0, 0.0, 1.0, 128, 128, 128, block=8)
+    N = 128 ** 3
+    edt = fe.NativeEDTVolume()
+
+    def plain():
+        sf = fe.klassificera_och_evaluera_recept(meta, rec)
+        occ = fe.reconstruct_solid_from_sparse(sf, 128, 128, 128, 8)
+        return edt.squared_distance(occ)
+    t = median_legs([("plain", plain)], runs=reps)
+    plan = fe.FieldPlan([(meta, rec)], device=dev)
+    dense = torch.empty((128,) * 3, dtype=torch.float32, device=dev)
+    occ = torch.empty((128,) * 3, dtype=torch.uint8, device=dev)
+    sq = torch.empty((128,) * 3, dtype=torch.int32, device=dev)
+    work = torch.empty_like(sq)
+
+    def resident():
+        plan.launch()
+        plan.reconstruct(0, dense, occ)
+        fe.edt_device(occ, 1, sq, work)
+    res_ms = event_ms(torch, resident, reps)
+    out = {"workload": "128^3 part at 1 mm, 64 primitives, block 8, then the int32 EDT of its occupancy",
+           "occupancy_fraction": float(occ.float().mean().item()), "n_active": plan.n_active(0),
+           "plain_per_call_ms": t["plain"] * 1e3, "plain_mvox_s": N / t["plain"] / 1e6,
+           "resident_ms": res_ms, "resident_mvox_s": N / (res_ms * 1e-3) / 1e6}
+    if cpu:
+        sys.path.insert(0, str(ROOT / "oracle"))
+        import oracle as O
+        prims = np.ascontiguousarray(rec).view(O.PRIM)
+        g = O.grid_record(np.float32(0), np.float32(0), np.float32(0), np.float32(1.0), 128, 128, 128, 8,
+                          np.float32(np.sqrt(3.0) * 8))
+
+        def oracle_part():
+            kind, cma, ids, tiles = O.build_sparse(prims, g)
+            o = (O.reconstruct(kind, ids, tiles, 128, 128, 128, 8) < 0).astype(np.uint8)
+            O.edt_envelope(o)
+        for label, threads in (("cpu_all_cores", env["cores_used"]), ("cpu_one_thread", 1)):
+            omp_threads(threads)
+            c = median_legs([("oracle", oracle_part)], runs=reps)
+            out[label] = {"threads": threads, "ms": c["oracle"] * 1e3, "mvox_s": N / c["oracle"] / 1e6}
+        omp_threads(env["cores_used"])
+        out["speedup_plain_vs_cpu_all_cores"] = out["cpu_all_cores"]["ms"] / out["plain_per_call_ms"]
+        out["speedup_resident_vs_cpu_all_cores"] = out["cpu_all_cores"]["ms"] / res_ms
+    del plan, dense, occ, sq, work
+    torch.cuda.empty_cache()
+    return out
+
+
+def bench_batch(fe, torch, count):
+    """Uneven recipes (seed 3) in one graph-captured launch vs sequential plain
+    calls of every recipe (each at the block edge the batch chose for it)."""
     items = fe.batch.batch_workload(seed=3, count=count)
-    idx = list(range(0, count, max(1, count // plain_sample)))[:plain_sample]
-    res, b = fe.batch.time_batch_vs_plain(items, reps=3, plain_items=idx)
+    res, b = fe.batch.time_batch_vs_plain(items, reps=3)
     vox = sum(s[4] * s[5] * s[6] for s, _ in items)
-    res.update(recipes=count, voxels=vox, batch_gvox_s=vox / (res["batch_ms"] * 1e-3) / 1e9,
-               plain_sample=len(idx), note="plain rate extrapolated from an evenly spaced sample of the batch")
+    res.update(recipes=count, voxels=vox, batch_gvox_s=vox / (res["batch_ms"] * 1e-3) / 1e9, plain_sample=count)
     del b
     torch.cuda.empty_cache()
     return res
@@ -195,14 +336,23 @@ def kernel_launches(fe, torch, n, reps=3):
         L.fe_edt_axis_pass(P(pair), P(work), n, n, n, 2, 0, st)
     torch.cuda.synchronize()
     na = plan.n_active(0)
-    del plan, dense, pair, work
+    # resident point queries (k_query_points) on the built field
+    field = fe.server.resident_fields(plan, torch.cuda.current_stream())[0]
+    pts = torch.rand((1 << 20, 3), device=dev) * (n - 1)
+    for _ in range(reps):
+        field.query_device(pts)
+    del plan, pair, work, field
+    torch.cuda.empty_cache()
+    pc = bench_pack_csg(fe, torch, dense, reps=reps)
+    del dense
     torch.cuda.empty_cache()
     src = torch.ones(1 << 28, dtype=torch.int32, device=dev)
     dst = torch.empty_like(src)
     for _ in range(reps):
         dst.copy_(src)
     torch.cuda.synchronize()
-    return {"n": n, "n_active": na, "n_blocks": meta.n_blocks, "copy_bytes_each_way": 1 << 30, "reps": reps}
+    return {"n": n, "n_active": na, "n_blocks": meta.n_blocks, "copy_bytes_each_way": 1 << 30, "reps": reps,
+            "pack_csg": pc}
 
 
 def bench_octree(fe, torch, dense, reps=3, queries=1 << 20):
@@ -374,15 +524,16 @@ def bench_ikarus(fe, torch, reps=5, n_points=1 << 24, cpu_points=1 << 18, cpu=Tr
     return res
 
 
-def bench_serve(fe, torch, rates, count):
-    """Poisson arrivals of 512^3 / 1024-primitive recipes into the field server.
-    Each rate first runs the same requests once untimed on a server of its own
-    (the device allocator's first-use cost of workspaces and resident fields is
-    a one-time start-up cost of a long-running server), then the timed pass on
-    a fresh server."""
-    rng = np.random.default_rng(5)
-    metas = [(fe.GridMeta(0.0, 0.0, 0.0, 1.0, 512, 512, 512, block=8), fe.recipes.mixed_recipe(rng, 1024, 512))
-             for _ in range(count)]
+def bench_serve(fe, torch, rates, count, plain_count=64):
+    """Poisson arrivals of recipes of the large-field distribution (1024
+    primitives, 512^3; seed 5) into the field server under a 200 GB budget,
+    `count` requests per rate, each field then answering 10^5 uniform points
+    of its box.  Each rate first runs the same requests once untimed on a
+    server of its own (first-use device allocation is a start-up cost of a
+    long-running server), then the timed pass on a fresh server.  The plain
+    per-call rate is timed on the first `plain_count` requests."""
+    recs = fe.recipes.served_requests(5, count)
+    metas = [(fe.GridMeta(0.0, 0.0, 0.0, 1.0, 512, 512, 512, block=8), r) for r in recs]
     out = {}
     for rate in rates:
         for timed in (False, True):
@@ -394,14 +545,120 @@ def bench_serve(fe, torch, rates, count):
         out[f"rate_{int(rate)}"] = dict(fields_s=fps, **{k: st[k] for k in ("build_p50_ms", "build_p99_ms",
                                                                                "query_p50_ms", "query_p99_ms")})
         torch.cuda.empty_cache()
-    t0 = time.perf_counter()
-    for m, r in metas[:8]:
+    for m, r in metas[:2]:
         fe.klassificera_och_evaluera_recept(m, r)
     torch.cuda.synchronize()
-    out["plain_fields_s"] = 8 / (time.perf_counter() - t0)
+    t0 = time.perf_counter()
+    for m, r in metas[:plain_count]:
+        fe.klassificera_och_evaluera_recept(m, r)
+    torch.cuda.synchronize()
+    out["plain_fields_s"] = plain_count / (time.perf_counter() - t0)
+    for k in list(out):
+        if k.startswith("rate_"):
+            out[k]["speedup_vs_plain"] = out[k]["fields_s"] / out["plain_fields_s"]
     out["requests_per_rate"] = count
+    out["plain_sample"] = plain_count
     out["warmup"] = "one untimed pass of the same requests at the same rate before each timed pass"
     return out
+
+
+def bench_pack_csg(fe, torch, dense, reps=3):
+    """The packed-field kernels on the 512^3 dense field (2^27 samples, past
+    the reference's 2^24 cap): device pack (leaf 512), unpack, exact lookups
+    and trilinear queries of 2^20 points, and a three-operand CSG program
+    (union, difference, offset) over the field and two further builds of the
+    same distribution.  Algorithmic bytes: pack 4N + 4 payload + 52 nodes,
+    unpack the same, CSG 4N(F+1) + 52 sum(nodes), queries 48 B per point,
+    lookups 28 B per point."""
+    dev = dense.device
+    L = fe._lib.lib()
+    P = fe._lib.ptr
+    st = fe._lib.stream_handle(None)
+    N = dense.numel()
+    shape = tuple(dense.shape)
+    nodes, payload = fe.blocks.pack_device(dense)
+    pack_ms = event_ms(torch, lambda: fe.blocks.pack_device(dense), reps)
+    n_nodes, n_pay = int(nodes.shape[0]), int(payload.numel())
+    pack_bytes = 4 * N + 4 * n_pay + 52 * n_nodes
+    out_dense = torch.empty_like(dense)
+    unpack_ms = event_ms(torch, lambda: L.fe_unpack(P(nodes), n_nodes, P(payload), *shape, P(out_dense), st), reps)
+    assert torch.equal(out_dense, dense)
+    rng = np.random.default_rng(8)
+    q = 1 << 20
+    idx = torch.from_numpy(np.stack([rng.integers(0, s, q) for s in shape], 1).astype(np.int64)).to(dev)
+    got = torch.empty(q, dtype=torch.int32, device=dev)
+    look_ms = event_ms(torch, lambda: L.fe_samples_at(P(nodes), P(payload), P(idx), q, P(got), st), reps)
+    pts = torch.from_numpy(rng.uniform(0, 511, (q, 3))).to(dev)
+    qv = torch.empty(q, dtype=torch.float32, device=dev)
+    org = np.zeros(3, np.float64)
+    query_ms = event_ms(torch, lambda: L.fe_query_packed(P(nodes), P(payload), *shape,
+                                                         org.ctypes.data_as(ct.POINTER(ct.c_double)), 1.0, P(pts), q,
+                                                         P(qv), st), reps)
+    out = {"samples": N, "nodes": n_nodes, "payload": n_pay,
+           "pack_ms": pack_ms, "pack_gb_s": pack_bytes / (pack_ms * 1e-3) / 1e9,
+           "unpack_ms": unpack_ms, "unpack_gb_s": pack_bytes / (unpack_ms * 1e-3) / 1e9,
+           "lookups": q, "lookup_ms": look_ms, "lookup_gb_s": 28 * q / (look_ms * 1e-3) / 1e9,
+           "queries": q, "query_ms": query_ms, "query_gb_s": 48 * q / (query_ms * 1e-3) / 1e9,
+           "pack_note": "pack_ms includes the driver's two host synchronisations and the output allocation"}
+    # CSG over the field and two more builds of the distribution
+    metas = [(fe.GridMeta(0.0, 0.0, 0.0, 1.0, *shape, block=8), r) for r in fe.recipes.served_requests(5, 2)]
+    plan = fe.FieldPlan(metas, device=dev)
+    plan.launch()
+    others = [plan.reconstruct(j) for j in range(2)]
+    del plan
+    fields = [fe.AdaptiveSampleBlocks(t, [0, 0, 0], 1.0) for t in (dense, *others)]
+    prog = [("union", 0, 1), ("difference", 3, 2), ("offset", 4, 0.5)]
+    fe.csg.evaluate_dense(fields, prog)
+    csg_ms = event_ms(torch, lambda: fe.csg.evaluate_dense(fields, prog), reps)
+    nodes_all = sum(len(f.nodes) for f in fields)
+    csg_bytes = 4 * N * (3 + 1) + 52 * nodes_all
+    read_bytes = 4 * N + sum(4 * len(f.payload) for f in fields) + 52 * nodes_all
+    out.update(csg_operands=3, csg_instructions=len(prog), csg_ms=csg_ms,
+               csg_gb_s=csg_bytes / (csg_ms * 1e-3) / 1e9,
+               csg_stored_gb_s=read_bytes / (csg_ms * 1e-3) / 1e9,
+               csg_note="csg_ms is the whole fe_csg_run call (table uploads, kernel, status read-back); csg_gb_s "
+                        "counts every operand sample read once (4N(F+1) + 52 sum(nodes)), csg_stored_gb_s the "
+                        "stored payloads and tables once plus the output")
+    del fields, others, out_dense
+    torch.cuda.empty_cache()
+    return out
+
+
+def fp64_check(rec, dense, sample=1 << 18, seed=0):
+    """Error of the stored fp32 field against the FP64 closed forms of its
+    primitives (oracle/fp64.py), on a seeded sample of the non-background
+    voxels: max and mean absolute error, and their bound 1e-6 of the
+    coordinate scale."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import fp64
+    import oracle as O
+    vals = dense.reshape(-1)
+    stored = np.flatnonzero(np.abs(vals) < 1e5)
+    pick = np.random.default_rng(seed).choice(stored, min(sample, len(stored)), replace=False)
+    pts = np.stack(np.unravel_index(pick, dense.shape), -1).astype(np.float64)
+    err = np.abs(vals[pick].astype(np.float64) - fp64.eval_fp64(np.ascontiguousarray(rec).view(O.PRIM), pts))
+    return {"sample_voxels": int(len(pick)), "max_abs_error": float(err.max()), "mean_abs_error": float(err.mean()),
+            "bound": 1e-6 * max(float(dense.shape[0]), float(np.max(np.abs([rec["cx"], rec["cy"], rec["cz"]])) +
+                                                                  np.max(np.abs([rec["a"], rec["b"], rec["c"]]))))}
+
+
+def profiled_valu(kernel):
+    """VALU figures of `kernel` from the rocprofv3 counter summary
+    profiles/field512_valu_pmc.txt (None when absent)."""
+    path = ROOT / "profiles" / "field512_valu_pmc.txt"
+    if not path.is_file():
+        return None
+    for line in path.read_text().splitlines():
+        parts = line.split()
+        if parts and parts[0] == kernel and "valu_busy_pct" in parts:
+            vals = dict(zip(parts[1::2], parts[2::2]))
+            return {k: float(v) for k, v in vals.items()}
+    return None
+
+
+def roofline_row(ms, nbytes):
+    gbs = nbytes / (ms * 1e-3) / 1e9
+    return {"ms": ms, "algorithmic_bytes": int(nbytes), "gb_s": gbs, "hbm_frac": gbs / HBM_PEAK_GBS}
 
 
 def main():
@@ -413,12 +670,14 @@ def main():
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--quick", action="store_true", help="single-field step only")
     ap.add_argument("--batch-count", type=int, default=256)
-    ap.add_argument("--serve-count", type=int, default=48)
-    ap.add_argument("--only", choices=["edt-heavy", "kernels", "mesh", "ikarus", "batch"], default=None,
-                    help="edt-heavy: the thin-feature EDT section only; kernels: each hot kernel of the "
-                         "512^3 step three times plus a 1 GiB copy (for rocprofv3 counter runs); "
-                         "mesh: the mesh-stage section only; ikarus: the IKARUS-tree section only; "
-                         "batch: the batched uneven recipes only")
+    ap.add_argument("--serve-count", type=int, default=1000)
+    ap.add_argument("--serve-rates", default="200,500,1000,2000")
+    ap.add_argument("--only", choices=["edt-heavy", "kernels", "mesh", "ikarus", "batch", "serve", "cfg1",
+                                       "pack-csg", "cpu"], default=None,
+                    help="one section only: edt-heavy (thin-feature EDT), kernels (each hot kernel of the 512^3 "
+                         "step three times plus a 1 GiB copy, for rocprofv3 counter runs), mesh, ikarus, batch "
+                         "(the 256 uneven recipes), serve (the served builds), cfg1 (the 128^3 part), pack-csg "
+                         "(packed-field kernels at 512^3), cpu (the CPU baseline)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU: the block-sparse build and the EDT run on one "
@@ -427,20 +686,18 @@ def main():
     import torch
     fe = load_package()
     dev = torch.device("cuda:0")
-    if args.only == "edt-heavy":
-        print(json.dumps({"edt_heavy": bench_edt_heavy(fe, torch)}))
-        return 0
-    if args.only == "kernels":
-        print(json.dumps(kernel_launches(fe, torch, args.n)))
-        return 0
-    if args.only == "mesh":
-        print(json.dumps({"mesh": bench_mesh(fe, torch, cpu=not args.no_cpu)}))
-        return 0
-    if args.only == "ikarus":
-        print(json.dumps({"ikarus": bench_ikarus(fe, torch, cpu=not args.no_cpu)}))
-        return 0
-    if args.only == "batch":
-        print(json.dumps({"batched": bench_batch(fe, torch, args.batch_count, 32)}))
+    env = cpu_env()
+    omp_threads(env["cores_used"])
+    rates = tuple(float(r) for r in args.serve_rates.split(","))
+    single = {"edt-heavy": lambda: {"edt_heavy": bench_edt_heavy(fe, torch)},
+              "kernels": lambda: kernel_launches(fe, torch, args.n),
+              "mesh": lambda: {"mesh": bench_mesh(fe, torch, cpu=not args.no_cpu)},
+              "ikarus": lambda: {"ikarus": bench_ikarus(fe, torch, cpu=not args.no_cpu)},
+              "batch": lambda: {"batched": bench_batch(fe, torch, args.batch_count)},
+              "serve": lambda: {"served": bench_serve(fe, torch, rates, args.serve_count)},
+              "cfg1": lambda: {"cfg1": bench_cfg1(fe, torch, cpu=not args.no_cpu, env=env)}}
+    if args.only in single:
+        print(json.dumps(single[args.only]()))
         return 0
     n = args.n
     rec = fe.recipes.large_field_512(seed=2) if n == 512 else fe.recipes.mixed_recipe(np.random.default_rng(2), 1024, n)
@@ -462,6 +719,15 @@ def main():
         field_part()
         edt_part()
 
+    if args.only == "pack-csg":
+        field_part()
+        print(json.dumps({"pack_csg": bench_pack_csg(fe, torch, dense)}))
+        return 0
+    if args.only == "cpu":
+        field_part()
+        torch.cuda.synchronize()
+        print(json.dumps({"cpu_baseline": cpu_baseline(fe, rec, occ.cpu().numpy(), n, env)}))
+        return 0
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
@@ -500,6 +766,8 @@ def main():
     dom = max(kernels, key=lambda k: kernels[k][0])
     dms, dbytes = kernels[dom]
     achieved = dbytes / (dms * 1e-3) / 1e9
+    counts = field_counts(fe, torch, plan)
+    valu = {k: profiled_valu(k) for k in ("classify", "field_tiles")}
     out = {
         "metric": "field Mvox/s + EDT Mvox/s; batched/served fields/s, p99 latency vs plain",
         "value": N / (ms * 1e-3) / 1e6,
@@ -514,25 +782,33 @@ def main():
         "field_mvox_s": N / (field_ms * 1e-3) / 1e6,
         "edt_mvox_s": N / (edt_ms * 1e-3) / 1e6,
         "kernel_ms": {k: v[0] for k, v in kernels.items()},
+        "kernel_roofline": {k: roofline_row(*v) for k, v in kernels.items()},
+        "field_build_valu": {"counts": counts, "profiled": valu,
+                             "note": "the analytic build is VALU work: P_eff primitive evaluations per point "
+                                     "after culling against the recipe's P"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": profiled_traffic(dom, n),
                      "algorithmic_bytes": dbytes},
     }
     if not args.quick:
         out["octree"] = bench_octree(fe, torch, dense)
+        out["pack_csg"] = bench_pack_csg(fe, torch, dense)
     occ_host = occ.cpu().numpy()
     out["config"]["occupancy_fraction"] = float(occ_host.mean())
+    dense_host = dense.cpu().numpy() if not args.no_cpu else None
     del plan, dense, pair, work
     torch.cuda.empty_cache()
     if not args.quick:
-        out["batched"] = bench_batch(fe, torch, args.batch_count, plain_sample=32)
-        out["served"] = bench_serve(fe, torch, (200.0, 2000.0), args.serve_count)
+        out["cfg1"] = bench_cfg1(fe, torch, cpu=not args.no_cpu, env=env)
+        out["batched"] = bench_batch(fe, torch, args.batch_count)
+        out["served"] = bench_serve(fe, torch, rates, args.serve_count)
         out["edt_heavy"] = bench_edt_heavy(fe, torch)
         out["mesh"] = bench_mesh(fe, torch, cpu=not args.no_cpu)
         out["ikarus"] = bench_ikarus(fe, torch, cpu=not args.no_cpu)
     if not args.no_cpu:
-        threads = int(os.environ.get("OMP_NUM_THREADS", "16"))
-        out["cpu_baseline"] = cpu_baseline(fe, rec, occ_host, threads, n)
+        cb = cpu_baseline(fe, rec, occ_host, n, env)
+        cb["fp64_check"] = fp64_check(rec, dense_host)
+        out["cpu_baseline"] = cb
     print(json.dumps(out))
     return 0
 

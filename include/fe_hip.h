@@ -137,6 +137,15 @@ int fe_octree_sample(const uint8_t *packed, const int32_t *branch_prefix, const 
 int fe_field_build(const void *descs, int R, int B, const int32_t *sb_prefix, int total_sb, const int32_t *ch_prefix,
                    int total_chunks, int32_t *chunk_count, int tile_waves, int max_prims, void *stream);
 
+/* fe_field_build that also counts the work of the culled evaluation (no
+ * reference counterpart; for measurement): evals is a device uint64[6] added
+ * to: [0] primitive evaluations at classified lattice points and [1] those
+ * points, [2] primitive evaluations at tile voxels and [3] those voxels,
+ * [4] / [5] interval evaluations of the culling in the classify / tile pass. */
+int fe_field_build_counted(const void *descs, int R, int B, const int32_t *sb_prefix, int total_sb,
+                           const int32_t *ch_prefix, int total_chunks, int32_t *chunk_count, int tile_waves,
+                           int max_prims, unsigned long long *evals, void *stream);
+
 /* k_classify_testfalt launch of faltkarna_v1.py:269-273 (kinds, corner_min_abs,
  * culled primitive lists) without compaction or tiles.  max_prims (both
  * calls): the longest recipe's primitive count (<= 4096), which sizes the
