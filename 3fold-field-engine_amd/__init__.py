@@ -11,7 +11,7 @@ _lib.lib()  # fail loudly at import when the HIP library is missing
 from .edt import (EDTBatch, NativeEDT, NativeEDTLarge, NativeEDTVolume, PersistentPairEDT,  # noqa: E402
                   edt_device, signed_distance, signed_from_pair_device, squared_distance)
 
-from . import recipes, field, batch, server, blocks, csg, octree, mesh, mesh_service, ikarus, ikarus_queries  # noqa: E402
+from . import recipes, field, batch, server, blocks, csg, octree, mesh, mesh_service, ikarus, ikarus_queries, multires  # noqa: E402
 from .mesh import PreparedMeshField, surface_raster_and_flood_cuda  # noqa: E402
 from .blocks import NODE, AdaptiveSampleBlocks, NativeAdaptiveBlocks  # noqa: E402
 from .csg import evaluate, compose, gather_tile, NativeTileReader  # noqa: E402
@@ -21,7 +21,7 @@ from .field import (GridMeta, SparseField, FieldPlan, klassificera_och_evaluera_
                     reconstruct_solid_from_sparse, volym_via_gpu_reduktion, determinism_cert, konservering_check,
                     klassificera_och_evaluera_fran_tatt_falt)
 
-__all__ = ["recipes", "field", "batch", "server", "blocks", "csg", "octree", "mesh", "mesh_service", "ikarus", "ikarus_queries",
+__all__ = ["recipes", "field", "batch", "server", "blocks", "csg", "octree", "mesh", "mesh_service", "ikarus", "ikarus_queries", "multires",
            "PreparedMeshField",
            "surface_raster_and_flood_cuda", "NODE", "AdaptiveSampleBlocks",
            "NativeAdaptiveBlocks", "evaluate", "compose", "gather_tile", "NativeTileReader", "GridMeta", "SparseField", "FieldPlan", "klassificera_och_evaluera_testfalt",

@@ -47,6 +47,9 @@ SIGNATURES = {
     "fe_field_checksum": (ct.c_int, [P, ct.c_int, F64, P, P]),
     "fe_dense_classify": (ct.c_int, [P, ct.c_int, ct.c_int, ct.c_int, ct.c_int, ct.c_float, P, P]),
     "fe_dense_tiles": (ct.c_int, [P, ct.c_int, ct.c_int, ct.c_int, ct.c_int, P, I64, P, P]),
+    "fe_multires_tiles": (ct.c_int, [P, P, ct.c_int, I64, P, P]),
+    "fe_multires_band": (ct.c_int, [P, P, P, P, ct.c_int, P, P]),
+    "fe_multires_query": (ct.c_int, [P, P, P, P, P, P, I64, P, P, P]),
     "fe_query_points": (ct.c_int, [P, P, I64, P, P]),
     # leaf-box pack, lookups, CSG, tile reader
     "adaptive_pack": (P, [P, P, ct.c_int]),

@@ -10,6 +10,8 @@ disabled by target feature.
 import os
 import subprocess
 import sys
+import tempfile
+from concurrent.futures import ThreadPoolExecutor
 from pathlib import Path
 
 HERE = Path(__file__).resolve().parent
@@ -30,12 +32,27 @@ def sources():
     return sorted(str(p) for p in CSRC.glob("*.hip")) + sorted(str(p) for p in CSRC.glob("*.cpp"))
 
 
-def build(verbose=False):
+def build(verbose=False, jobs=None):
+    """Compiles each source to an object in parallel (separate translation
+    units, as one hipcc call would make them), then links the library."""
+    jobs = jobs or min(8, os.cpu_count() or 1)
     tmp = LIB.with_suffix(".so.tmp")
-    cmd = [HIPCC, *FLAGS, *sources(), "-o", str(tmp)]
-    if verbose:
-        print(" ".join(cmd), file=sys.stderr)
-    subprocess.run(cmd, check=True)
+    with tempfile.TemporaryDirectory(prefix="fe_hip_build_") as objdir:
+        objs, cmds = [], []
+        for src in sources():
+            obj = str(Path(objdir) / (Path(src).stem + ".o"))
+            objs.append(obj)
+            cmds.append([HIPCC, *[f for f in FLAGS if f != "-shared"], "-c", src, "-o", obj])
+
+        def run(cmd):
+            if verbose:
+                print(" ".join(cmd), file=sys.stderr)
+            subprocess.run(cmd, check=True)
+
+        with ThreadPoolExecutor(max_workers=jobs) as pool:
+            for f in [pool.submit(run, c) for c in cmds]:
+                f.result()
+        run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", str(tmp)])
     os.replace(tmp, LIB)
     return LIB
 

@@ -176,6 +176,29 @@ int fe_dense_tiles(const float *dense, int nx, int ny, int nz, int B, const int3
  * built field. */
 int fe_query_points(const void *desc, const float *points, int64_t n, float *out, void *stream);
 
+/* ---------------------------------------------------------------- per-block pitch */
+/* The two-level field of faltkarna_v1_multires.py:84-106 (build :254-363,
+ * band :211-252, queries :405-441), from a recipe instead of a mesh.  bricks:
+ * device table of 32-byte records {int32 bi, bj, bk, level; int64 tile
+ * (offset of its (B*2^level + 1)^3 fp32 samples), chunk0 (its first 64-sample
+ * chunk, ascending from 0)}; grid: host 72-byte record {double lo[3], h,
+ * margin; int32 B, nb[3], n[3], pad} (f64 origin and pitch, brick lattice,
+ * coarse sample counts); slot / kind: device int32 per lattice brick (table
+ * index or -1; -1 interior, 0 active, +1 exterior).  tiles: every active
+ * brick's samples from one classified descriptor (its culled lists), level 0
+ * on the coarse lattice with indices clamped to the grid, level l at
+ * x0 + f32(i) * (h / 2^l).  band: the face layers of the `fine` bricks (all
+ * at one level) next to coarser active bricks, as
+ * map_coordinates(order=1, mode="nearest") of those bricks.  query: signed
+ * distance at device f64 points (n x 3) into device f64 out; brick_out
+ * (optional, device int32 n x 4): brick and level per point. */
+int fe_multires_tiles(const void *desc, const void *bricks, int nbricks, int64_t nchunks, float *tiles, void *stream);
+int fe_multires_band(const void *grid, const void *bricks, const int32_t *slot, const int32_t *fine, int nfine,
+                     float *tiles, void *stream);
+int fe_multires_query(const void *grid, const void *bricks, const int32_t *slot, const int32_t *kind,
+                      const float *tiles, const double *points, int64_t n, double *out, int32_t *brick_out,
+                      void *stream);
+
 /* ---------------------------------------------------------------- leaf-box packing */
 /* Node records are 52 bytes {int32 lo[3], shape[3], stored[3], children[2];
  * int64 offset} in pre-order, as adaptive_sample_blocks_v1.py:14-16. */
