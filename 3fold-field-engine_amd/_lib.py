@@ -93,6 +93,8 @@ SIGNATURES = {
     "fe_ikarus_eval": (ct.c_int, [P, ct.c_int, P, P, P, P, I64, P, P]),
     "fe_ikarus_classify": (ct.c_int, [P, ct.c_int, P, P, P, P, P, ct.c_int, P, P, P]),
     "fe_ikarus_tiles": (ct.c_int, [P, ct.c_int, P, P, P, P, P, ct.c_int, P, I64, P, P]),
+    "fe_ikarus_check": (ct.c_int, [P, ct.c_int, I64, ct.c_int]),
+    "fe_ikarus_fault": (ct.c_int, [P]),
     "fe_ikarus_grid": (ct.c_int, [P, ct.c_int, P, P, P, P, F64, P, ct.c_float, ct.c_float, P, P, P, P]),
     "fe_ikarus_refine": (ct.c_int, [P, ct.c_int, P, P, P, P, F64, P, P, I64, ct.c_int, ct.c_float, P, P]),
     "fe_ikarus_pairs_eval": (ct.c_int, [P, ct.c_int, P, P, P, P, P, ct.c_int, I64, P, P]),

@@ -32,17 +32,21 @@ def sources():
     return sorted(str(p) for p in CSRC.glob("*.hip")) + sorted(str(p) for p in CSRC.glob("*.cpp"))
 
 
-def build(verbose=False, jobs=None):
+def build(verbose=False, jobs=None, extra_flags=(), extra_sources=(), out=None, shared=True):
     """Compiles each source to an object in parallel (separate translation
-    units, as one hipcc call would make them), then links the library."""
+    units, as one hipcc call would make them), then links the library.
+
+    extra_flags / extra_sources / out / shared: a variant build (the host
+    sanitizer run of the tests links a driver program against all sources)."""
     jobs = jobs or min(8, os.cpu_count() or 1)
-    tmp = LIB.with_suffix(".so.tmp")
+    out = Path(out) if out else LIB
+    tmp = out.with_name(out.name + ".tmp")
     with tempfile.TemporaryDirectory(prefix="fe_hip_build_") as objdir:
         objs, cmds = [], []
-        for src in sources():
+        for src in [*sources(), *map(str, extra_sources)]:
             obj = str(Path(objdir) / (Path(src).stem + ".o"))
             objs.append(obj)
-            cmds.append([HIPCC, *[f for f in FLAGS if f != "-shared"], "-c", src, "-o", obj])
+            cmds.append([HIPCC, *[f for f in FLAGS if f != "-shared"], *extra_flags, "-c", src, "-o", obj])
 
         def run(cmd):
             if verbose:
@@ -52,9 +56,10 @@ def build(verbose=False, jobs=None):
         with ThreadPoolExecutor(max_workers=jobs) as pool:
             for f in [pool.submit(run, c) for c in cmds]:
                 f.result()
-        run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", str(tmp)])
-    os.replace(tmp, LIB)
-    return LIB
+        run([HIPCC, f"--offload-arch={ARCH}", *(["-shared"] if shared else []), "-fPIC", *extra_flags, *objs,
+             "-o", str(tmp)])
+    os.replace(tmp, out)
+    return out
 
 
 if __name__ == "__main__":

@@ -42,6 +42,8 @@ void fe_note_error(hipError_t e, const char* where)
     snprintf(fe_err_buf, sizeof fe_err_buf, "%s: %s (%d)", where, hipGetErrorString(e), (int)e);
 }
 
+void fe_note_text(const char* msg) { snprintf(fe_err_buf, sizeof fe_err_buf, "%s", msg); }
+
 // Text of the last runtime failure on this host thread ("" if none).
 extern "C" const char* fe_last_error() noexcept { return fe_err_buf; }
 

@@ -12,6 +12,8 @@ enum { FE_OK = 0, FE_INVALID = 1, FE_RUNTIME = 2 };
 // Records the last runtime failure (file:line and the HIP error string) for
 // fe_last_error(); defined in edt.hip.
 void fe_note_error(hipError_t e, const char* where);
+// Records a message for fe_last_error() (invalid input explained).
+void fe_note_text(const char* msg);
 
 #define FE_STR2(x) #x
 #define FE_STR(x) FE_STR2(x)

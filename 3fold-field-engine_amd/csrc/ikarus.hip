@@ -19,6 +19,8 @@
 // Warp's min/max (a<b?a:b, a>b?a:b), clamp (min(max(lo, x), hi)) and lerp
 // (a*(1-t) + b*t).
 #include "fe_common.hpp"
+#include <cstdio>
+#include <mutex>
 
 namespace {
 
@@ -103,8 +105,22 @@ __device__ __forceinline__ float round_box2(float qx, float qy)
     return sqrtf(ox * ox + oy * oy) + wmin(wmax(qx, qy), 0.0f);
 }
 
+// Set (never cleared on the device) when a program underflows or overflows
+// the value stack or holds an unknown opcode; fe_ikarus_fault reads and
+// clears it.  One word per GPU, shared by every call on that GPU.
+__device__ unsigned int ik_fault;
+
+__device__ __noinline__ float ik_bad()
+{
+    atomicOr(&ik_fault, 1u);
+    return __builtin_nanf("");
+}
+
 // Evaluates the program at world point (wx, wy, wz); stk is this thread's
-// column of the block's LDS stack (stride IK_T).
+// column of the block's LDS stack (stride IK_T).  A malformed program (a
+// binary operation without two values, a push past IK_STACK slots, an
+// unknown opcode) stops at that instruction: the value is NaN and the fault
+// word is set, and no LDS outside this thread's column is touched.
 __device__ float ik_eval(const IkIns* __restrict__ prog, int n, const float* __restrict__ par,
                          const IkGrid* __restrict__ grids, const float* __restrict__ gdata, float wx, float wy,
                          float wz, float* stk)
@@ -185,12 +201,17 @@ __device__ float ik_eval(const IkIns* __restrict__ prog, int n, const float* __r
                 v = ik_grid(grids[in.a], gdata, px, py, pz);
                 break;
             case IK_OFFSET:
+                if (!have) return ik_bad();
                 top = top - q[0];
                 continue;
             case IK_SCALE_OUT:
+                if (!have) return ik_bad();
                 top = top * q[0];
                 continue;
-            default: {  // binary: the second-evaluated operand is the top, the first is in LDS
+            case IK_UNION:
+            case IK_INTERSECT:
+            case IK_SUBTRACT: {  // the second-evaluated operand is the top, the first is in LDS
+                if (sp <= 0 || !have) return ik_bad();
                 const float below = stk[(--sp) * IK_T];
                 const float da = in.a ? top : below, db = in.a ? below : top;
                 const float k = q[0];
@@ -218,9 +239,14 @@ __device__ float ik_eval(const IkIns* __restrict__ prog, int n, const float* __r
                 }
                 continue;
             }
+            default:
+                return ik_bad();
         }
         // a leaf value: push the old top, keep the new one in the register
-        if (have) stk[(sp++) * IK_T] = top;
+        if (have) {
+            if (sp >= IK_STACK) return ik_bad();
+            stk[(sp++) * IK_T] = top;
+        }
         top = v;
         have = true;
     }
@@ -444,7 +470,108 @@ bool valid_program(const void* ins, int n_ins, const void* par)
     return ins && par && n_ins > 0 && n_ins <= (1 << 20);
 }
 
+// fp32 parameters an instruction reads (-1: unknown opcode).
+int ik_param_count(int op)
+{
+    switch (op) {
+        case IK_WORLD: case IK_GRID: return 0;
+        case IK_XFORM: return 12;
+        case IK_PROJ: case IK_SCALE_IN: case IK_OFFSET: case IK_SCALE_OUT: return 1;
+        case IK_UNION: case IK_INTERSECT: case IK_SUBTRACT: return 1;
+        case IK_HALFSPACE: case IK_SPHERE: return 4;
+        case IK_BOX: return 6;
+        case IK_CYL: return 5;
+        case IK_CONE: return 7;
+        case IK_FRUSTUM: return 10;
+        default: return -1;
+    }
+}
+
+std::mutex fault_lock;
+unsigned int* fault_ptr[64];  // per device, from hipGetSymbolAddress with that device current
+
+hipError_t fault_word(unsigned int** out)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= 64) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> hold(fault_lock);
+    if (!fault_ptr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(ik_fault));
+        if (e != hipSuccess) return e;
+        fault_ptr[dev] = static_cast<unsigned int*>(p);
+    }
+    *out = fault_ptr[dev];
+    return hipSuccess;
+}
+
 }  // namespace
+
+// Checks a compiled program on the host before upload: 1 <= n_ins <= 2^20,
+// known opcodes, every instruction's parameters inside params[0, n_params),
+// grid leaves below n_grids, axis arguments 0..2, the binary order flag 0/1,
+// and the value stack: a unary or binary operation finds its operands, no
+// push goes past 16 LDS slots, and the program ends with one value.
+// FE_INVALID names the first bad instruction in fe_last_error().
+extern "C" int fe_ikarus_check(const void* ins_host, int n_ins, int64_t n_params, int n_grids) noexcept
+{
+    char msg[160];
+    if (!ins_host || n_ins < 1 || n_ins > (1 << 20) || n_params < 0 || n_grids < 0) {
+        fe_note_text("fe_ikarus_check: a program of 1..2^20 instructions and non-negative sizes required");
+        return FE_INVALID;
+    }
+    const IkIns* in = static_cast<const IkIns*>(ins_host);
+    int sp = 0;
+    bool have = false;
+    for (int i = 0; i < n_ins; ++i) {
+        const int op = in[i].op, a = in[i].a;
+        const int np = ik_param_count(op);
+        const char* bad = nullptr;
+        if (np < 0) bad = "unknown opcode";
+        else if (np > 0 && (in[i].pofs < 0 || (int64_t)in[i].pofs + np > n_params)) bad = "parameters out of range";
+        else if (op == IK_GRID && (a < 0 || a >= n_grids)) bad = "grid leaf out of range";
+        else if ((op == IK_PROJ || op == IK_CYL || op == IK_CONE) && (a < 0 || a > 2)) bad = "axis out of range";
+        else if ((op == IK_UNION || op == IK_INTERSECT || op == IK_SUBTRACT) && (a < 0 || a > 1)) bad = "bad order flag";
+        else if ((op == IK_OFFSET || op == IK_SCALE_OUT) && !have) bad = "unary operation without a value";
+        else if (op == IK_UNION || op == IK_INTERSECT || op == IK_SUBTRACT) {
+            if (!have || sp < 1) bad = "binary operation without two values";
+            else --sp;
+        } else if (op >= IK_HALFSPACE && op <= IK_GRID) {
+            if (have && sp >= IK_STACK) bad = "value stack deeper than 17";
+            else if (have) ++sp;
+            have = true;
+        }
+        if (bad) {
+            snprintf(msg, sizeof msg, "fe_ikarus_check: instruction %d (opcode %d): %s", i, op, bad);
+            fe_note_text(msg);
+            return FE_INVALID;
+        }
+    }
+    if (!have || sp != 0) {
+        fe_note_text("fe_ikarus_check: the program does not end with exactly one value");
+        return FE_INVALID;
+    }
+    return FE_OK;
+}
+
+// Waits for `stream`, then reads and clears this GPU's fault word:
+// FE_INVALID if a program evaluated since the last call was malformed (its
+// values are NaN), else FE_OK.
+extern "C" int fe_ikarus_fault(void* stream) noexcept
+{
+    unsigned int* word = nullptr;
+    FE_TRY(fault_word(&word));
+    FE_TRY(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
+    unsigned int v = 0;
+    FE_TRY(hipMemcpy(&v, word, sizeof v, hipMemcpyDeviceToHost));
+    if (!v) return FE_OK;
+    const unsigned int zero = 0;
+    FE_TRY(hipMemcpy(word, &zero, sizeof zero, hipMemcpyHostToDevice));
+    fe_note_text("fe_ikarus_fault: a program under- or overflowed the value stack or held an unknown opcode");
+    return FE_INVALID;
+}
 
 // Values of a compiled tree at device points (n x 3 fp32) into out (n fp32).
 // ins: n_ins 16-byte instructions {op, a, param offset, 0}; params: fp32;

@@ -102,6 +102,34 @@ def _validate_meta(meta):
         raise ValueError("Grid too large")
 
 
+POOL_ITEM_BYTES = {"i32": 4, "f32": 4, "u16": 2}
+
+
+def _layout(items, caps=None):
+    """Sizes of a plan over items [(GridMeta, recipe)]: superblock counts,
+    list capacities, the padded cma lengths and the element count of each
+    pooled allocation (int32: kind, ids, slots, list lengths, counters;
+    float32: cma then tiles, the tiles 256-byte aligned as the reconstruction
+    reads them as float4; uint16: culled lists)."""
+    B = items[0][0].block
+    SB = SUPER[B]
+    caps = caps if caps is not None else [m.n_blocks for m, _ in items]
+    sbs = [(-(-m.n_bx // SB), -(-m.n_by // SB), -(-m.n_bz // SB)) for m, _ in items]
+    nsb = [a * b * c for a, b, c in sbs]
+    lcap = [min(len(r), LIST_CAP) for _, r in items]
+    cma_len = [-(-m.n_blocks // 64) * 64 for m, _ in items]
+    per = dict(i32=[3 * m.n_blocks + nsb[i] + 4 for i, (m, _) in enumerate(items)],
+               f32=[cma_len[i] + caps[i] * B ** 3 for i in range(len(items))],
+               u16=[nsb[i] * lcap[i] for i in range(len(items))])
+    elems = dict(i32=sum(per["i32"]), f32=sum(per["f32"]), u16=max(1, sum(per["u16"])))
+    return dict(sbs=sbs, nsb=nsb, lcap=lcap, cma_len=cma_len, per_item=per, pool_elems=elems)
+
+
+def plan_pool_bytes(items):
+    """{pool name: bytes} of a plan over items at full tile capacity."""
+    return {k: v * POOL_ITEM_BYTES[k] for k, v in _layout(items)["pool_elems"].items()}
+
+
 class FieldPlan:
     """Device buffers and descriptor table for R builds with one block edge.
 
@@ -109,7 +137,9 @@ class FieldPlan:
     the default is every block, which can never overflow).  pools: optional
     {"i32", "f32", "u16"} device tensors of an earlier plan whose work is
     finished; a pool at least as large as this plan needs is reused (sliced)
-    instead of allocating.  self.pools holds the tensors this plan uses."""
+    instead of allocating, a smaller one is removed from the dict (released)
+    before its replacement is allocated.  self.pools holds the tensors this
+    plan uses."""
 
     def __init__(self, items, tile_caps=None, device=None, corrupt=False, pools=None):
         torch = _torch()
@@ -127,29 +157,25 @@ class FieldPlan:
         n = len(items)
         caps = tile_caps if tile_caps is not None else [m.n_blocks for m, _ in items]
         b3 = B ** 3
-        sbs = [(-(-m.n_bx // self.SB), -(-m.n_by // self.SB), -(-m.n_bz // self.SB)) for m, _ in items]
-        nsb = [a * b * c for a, b, c in sbs]
+        L = _layout(items, caps)
+        sbs, nsb, lcap, cma_len = L["sbs"], L["nsb"], L["lcap"], L["cma_len"]
         nch = [-(-m.n_blocks // 1024) for m, _ in items]
-        lcap = [min(len(r), LIST_CAP) for _, r in items]
-        # pooled allocations: int32, float32, uint16, prims
-        i32_sizes = [3 * m.n_blocks + nsb[i] + 4 for i, (m, _) in enumerate(items)]
-        # tiles start 256-byte aligned (the reconstruction reads them as float4)
-        cma_len = [-(-m.n_blocks // 64) * 64 for m, _ in items]
-        f32_sizes = [cma_len[i] + caps[i] * b3 for i in range(n)]
-        u16_sizes = [nsb[i] * lcap[i] for i in range(n)]
         prim_counts = [len(r) for _, r in items]
         self.pools = {}
 
         def pool(name, size, dtype):
             t = (pools or {}).get(name)
-            if t is None or t.dtype != dtype or t.device != self.device or t.numel() < size:
+            if t is not None and (t.dtype != dtype or t.device != self.device or t.numel() < size):
+                del pools[name]
+                t = None
+            if t is None:
                 t = torch.empty(size, dtype=dtype, device=self.device)
             self.pools[name] = t
             return t[:size]
 
-        self.i32 = pool("i32", sum(i32_sizes), torch.int32)
-        self.f32 = pool("f32", sum(f32_sizes), torch.float32)
-        self.u16 = pool("u16", max(1, sum(u16_sizes)), torch.int16)
+        self.i32 = pool("i32", L["pool_elems"]["i32"], torch.int32)
+        self.f32 = pool("f32", L["pool_elems"]["f32"], torch.float32)
+        self.u16 = pool("u16", L["pool_elems"]["u16"], torch.int16)
         host_prims = np.concatenate([r for _, r in items])
         self.prims = torch.from_numpy(host_prims.view(np.uint8).copy()).to(self.device)
         desc = np.zeros(n, FIELD_DESC)
@@ -177,9 +203,9 @@ class FieldPlan:
             d["lists"] = bu16 + 2 * v["lists"][0]
             d["tile_cap"] = caps[i]
             self.views.append(v)
-            oi += i32_sizes[i]
-            of += f32_sizes[i]
-            ou += u16_sizes[i]
+            oi += L["per_item"]["i32"][i]
+            of += L["per_item"]["f32"][i]
+            ou += L["per_item"]["u16"][i]
             op += prim_counts[i]
         self.desc_host = desc
         self.desc = torch.from_numpy(desc.view(np.uint8).copy()).to(self.device)

@@ -372,7 +372,17 @@ def compile_tree(node):
     par = np.array(params, np.float32) if params else np.zeros(1, np.float32)
     grid_a = np.array(grids, GRID_REC) if grids else np.zeros(0, GRID_REC)
     gdata = np.concatenate(chunks).astype(np.float32) if chunks else np.zeros(0, np.float32)
+    check_program(ins_a, par, grid_a)
     return Program(ins_a, par, grid_a, gdata, depth)
+
+
+def check_program(ins, params, grids):
+    """The library's host check of a compiled program (opcodes, parameter
+    and grid ranges, stack depth); raises ValueError with its message."""
+    ins = np.ascontiguousarray(ins, np.int32).reshape(-1, 4)
+    L = _lib.lib()
+    if L.fe_ikarus_check(ins.ctypes.data, len(ins), len(params), len(grids)):
+        raise ValueError(L.fe_last_error().decode())
 
 
 # ------------------------------------------------------------------ device entry points

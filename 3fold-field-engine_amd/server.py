@@ -125,10 +125,12 @@ def resident_estimate(meta, recipe):
 
 
 class _Request:
-    __slots__ = ("rid", "meta", "recipe", "t_arrival", "t_admit", "t_done", "field", "error", "event", "need")
+    __slots__ = ("rid", "meta", "recipe", "t_arrival", "t_admit", "t_done", "field", "error", "event", "ws", "res",
+                 "need")
 
-    def __init__(self, rid, meta, recipe, need):
-        self.rid, self.meta, self.recipe, self.need = rid, meta, recipe, need
+    def __init__(self, rid, meta, recipe, ws, res):
+        self.rid, self.meta, self.recipe = rid, meta, recipe
+        self.ws, self.res, self.need = ws, res, ws + res
         self.t_arrival = time.perf_counter()
         self.t_admit = self.t_done = None
         self.field = self.error = None
@@ -138,11 +140,15 @@ class _Request:
 class FieldServer:
     """Continuous-batching field builds with resident results.
 
-    hbm_budget: bytes of device memory the server may hold (workspaces of
-    in-flight batches plus resident fields).  slots: batches in flight at once
-    (one stream each); each slot also keeps the largest workspace it has used
-    between its batches (reused, so steady serving allocates nothing per
-    batch).  max_batch: requests per batch."""
+    hbm_budget: bytes of device memory the server may hold: workspaces of
+    in-flight batches, the workspaces slots keep between batches, and
+    resident fields.  slots: batches in flight at once (one stream each);
+    each slot keeps its last workspace pools between batches (reused when
+    large enough, so steady serving allocates nothing per batch; counted in
+    `reserved` while kept).  A batch is charged its resident estimates, its
+    recipe upload and the growth of the slot's pools; a kept pool too small
+    for the batch is released before its replacement is allocated, so a slot
+    never holds both.  max_batch: requests per batch."""
 
     def __init__(self, hbm_budget=200e9, slots=2, max_batch=8, device=None, start=True):
         torch = _torch()
@@ -156,8 +162,9 @@ class FieldServer:
         self.inflight = []
         self.fields = {}
         self.requests = {}
-        self.reserved = 0.0  # workspaces in flight + resident fields
+        self.reserved = 0.0  # workspaces in flight and kept + resident fields
         self.slot_pools = {}  # id(stream) -> the FieldPlan pools last used on that stream
+        self.retained = {}  # id(stream) -> {pool name: bytes} of those pools (part of reserved)
         self.ids = itertools.count()
         self.lock = threading.Condition()
         self.running = True
@@ -172,14 +179,15 @@ class FieldServer:
         """Queues a build; returns its request id.  Raises ValueError when one
         request alone exceeds the HBM budget (admission refusal)."""
         recipe = R.validate(recipe)
-        need = workspace_bytes(meta, recipe) + resident_estimate(meta, recipe)
+        ws, res = workspace_bytes(meta, recipe), resident_estimate(meta, recipe)
+        need = ws + res
         if need > self.budget:
             raise ValueError(f"request needs {need / 1e9:.2f} GB, above the {self.budget / 1e9:.2f} GB HBM budget")
         with self.lock:
             if not self.running:
                 raise RuntimeError("server closed")
             rid = next(self.ids)
-            req = _Request(rid, meta, recipe, need)
+            req = _Request(rid, meta, recipe, ws, res)
             self.requests[rid] = req
             self.pending.append(req)
             self.lock.notify_all()
@@ -217,7 +225,10 @@ class FieldServer:
             self.lock.notify_all()
         if self.worker.is_alive():
             self.worker.join()
-        self.slot_pools.clear()
+        with self.lock:
+            self.slot_pools.clear()
+            self.reserved -= sum(sum(k.values()) for k in self.retained.values())
+            self.retained.clear()
 
     def stats(self):
         b = np.array([(r.t_done - r.t_arrival) * 1e3 for r in self.completed]) if self.completed else np.zeros(1)
@@ -227,18 +238,47 @@ class FieldServer:
                     query_p99_ms=float(np.percentile(q, 99)), resident_bytes=self.reserved)
 
     # ------------------------------------------------------------- worker
-    def _admit(self):
-        """Pops the longest FIFO prefix that fits one batch and the budget."""
-        batch, need = [], 0.0
+    @staticmethod
+    def _charge(reqs, kept):
+        """Bytes a batch of requests adds to `reserved` on a slot keeping the
+        pools `kept` ({name: bytes}): each pool's growth (a kept pool at least
+        as large is reused, a smaller one is released before its replacement
+        is allocated), the recipe upload and the resident estimates."""
+        need = F.plan_pool_bytes([(r.meta, r.recipe) for r in reqs])
+        names = set(need) | set(kept)
+        grow = sum(max(need.get(k, 0), kept.get(k, 0)) - kept.get(k, 0) for k in names)
+        return grow + 32 * sum(len(r.recipe) for r in reqs) + sum(r.res for r in reqs)
+
+    def _admit(self, kept=None):
+        """Pops the longest FIFO prefix that fits one batch and the budget on
+        a slot keeping the pools `kept`; (batch, charge)."""
+        kept = kept or {}
+        batch, charge = [], 0.0
         while self.pending and len(batch) < self.max_batch:
             req = self.pending[0]
             if batch and req.meta.block != batch[0].meta.block:
                 break
-            if self.reserved + need + req.need > self.budget:
+            c = self._charge(batch + [req], kept)
+            if self.reserved + c > self.budget:
                 break
             batch.append(self.pending.popleft())
-            need += req.need
-        return batch, need
+            charge = c
+        return batch, charge
+
+    def _start(self, key, charge):
+        """A batch admitted on slot `key`: `reserved` grows by its charge and
+        the slot's kept pools move inside the batch's in-flight charge, which
+        this returns."""
+        kept = self.retained.pop(key, {})
+        self.reserved += charge
+        return charge + sum(kept.values())
+
+    def _settle(self, key, held):
+        """A batch on slot `key` ended: its in-flight charge `held` leaves
+        `reserved` and the pools the slot now keeps enter it."""
+        kept = {k: t.numel() * t.element_size() for k, t in self.slot_pools.get(key, {}).items()}
+        self.reserved += sum(kept.values()) - held
+        self.retained[key] = kept
 
     def _loop(self):
         torch = _torch()
@@ -252,12 +292,13 @@ class FieldServer:
                     self.lock.wait(0.05)
                 if not self.running and not self.pending and not self.inflight:
                     return
-                batch, need = ([], 0.0)
+                batch, held = [], 0.0
                 if self.free_streams and self.pending:
-                    batch, need = self._admit()
+                    stream = self.free_streams[-1]
+                    batch, charge = self._admit(self.retained.get(id(stream)))
                     if batch:
-                        self.reserved += need
-                        stream = self.free_streams.pop()
+                        self.free_streams.pop()
+                        held = self._start(id(stream), charge)
             if batch:
                 now = time.perf_counter()
                 for r in batch:
@@ -268,18 +309,19 @@ class FieldServer:
                         # harvested before the slot was freed) is reused when large enough
                         plan = F.FieldPlan([(r.meta, r.recipe) for r in batch], device=self.device,
                                            pools=self.slot_pools.get(id(stream)))
-                        self.slot_pools[id(stream)] = plan.pools
+                        with self.lock:
+                            self.slot_pools[id(stream)] = plan.pools
                         plan.launch(stream)
                         ev = torch.cuda.Event()
                         ev.record(stream)
-                    self.inflight.append((plan, batch, need, stream, ev))
+                    self.inflight.append((plan, batch, held, stream, ev))
                 except Exception as e:  # admission-time failure completes the requests with an error
-                    self._fail(batch, need, stream, e)
+                    self._fail(batch, held, stream, e)
             self._harvest(block=not batch)
 
-    def _fail(self, batch, need, stream, err):
+    def _fail(self, batch, held, stream, err):
         with self.lock:
-            self.reserved -= need
+            self._settle(id(stream), held)
             self.free_streams.append(stream)
         for r in batch:
             r.error = repr(err)
@@ -298,7 +340,7 @@ class FieldServer:
             done = [self.inflight[0]]
         for item in done:
             self.inflight.remove(item)
-            plan, batch, need, stream, ev = item
+            plan, batch, held, stream, ev = item
             fields = []
             try:
                 fields = resident_fields(plan, stream)
@@ -308,7 +350,7 @@ class FieldServer:
                 err = e
             del plan
             with self.lock:
-                self.reserved -= need
+                self._settle(id(stream), held)
                 self.free_streams.append(stream)
                 now = time.perf_counter()
                 for r, fld in itertools.zip_longest(batch, fields):

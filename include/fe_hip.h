@@ -333,7 +333,12 @@ int fe_gwn_device(const double *corners, int64_t triangles, const double *points
  * device pointers except origin_pitch_margin (5 floats) and blocks (3 int32),
  * which are host arrays.  eval: values at points (n x 3 fp32).  classify:
  * faltkarna_v1.py:128-164 rule per block (kind int32, corner_min_abs fp32);
- * tiles: faltkarna_v1.py:166-190 for the ascending active ids. */
+ * tiles: faltkarna_v1.py:166-190 for the ascending active ids.
+ * The entry points check pointers and n_ins only; the interpreter stops a
+ * program that underflows or overflows its 17-value stack or holds an unknown
+ * opcode (NaN values, no memory outside its own stack touched) and latches a
+ * per-GPU fault word that fe_ikarus_fault reads.  Parameter offsets and grid
+ * indices are trusted: run fe_ikarus_check on the host program first. */
 int fe_ikarus_eval(const void *ins, int n_ins, const float *params, const void *grids, const float *grid_data,
                    const float *points, int64_t n, float *out, void *stream);
 int fe_ikarus_classify(const void *ins, int n_ins, const float *params, const void *grids, const float *grid_data,
@@ -342,6 +347,15 @@ int fe_ikarus_classify(const void *ins, int n_ins, const float *params, const vo
 int fe_ikarus_tiles(const void *ins, int n_ins, const float *params, const void *grids, const float *grid_data,
                     const float *origin_pitch_margin, const int32_t *blocks, int B, const int32_t *ids,
                     int64_t n_active, float *tiles, void *stream);
+/* No counterpart in the reference (its programs are generated Python): checks
+ * a host copy of a program before upload -- known opcodes, parameters inside
+ * [0, n_params), grid leaves below n_grids, axis and order arguments, and a
+ * stack that never underflows, stays within 17 values and ends with one.
+ * FE_INVALID names the first bad instruction in fe_last_error(). */
+int fe_ikarus_check(const void *ins_host, int n_ins, int64_t n_params, int n_grids);
+/* Waits for stream, then reads and clears this GPU's interpreter fault word:
+ * FE_INVALID when a malformed program ran since the last call. */
+int fe_ikarus_fault(void *stream);
 
 /* The passes of the hierarchical volume queries of ikarus_v1/queries.py:138-277
  * on an implicit grid (lo: 3 host doubles, shape: 3 host int32, samples

@@ -147,6 +147,47 @@ def test_compiler_program_shape(E):
         E.compile_tree({"op": "torus"})
 
 
+def raw(*ins):
+    return np.array([(op, a, pofs, 0) for op, a, pofs in ins], np.int32).reshape(-1, 4)
+
+
+def test_host_program_check(E, fe):
+    """fe_ikarus_check accepts every compiled program and rejects each kind of
+    malformed one, naming the instruction; the stack bound is the
+    interpreter's 16 LDS slots plus the register top."""
+    L = fe._lib.lib()
+
+    def check(ins, n_params=64, n_grids=1):
+        status = L.fe_ikarus_check(ins.ctypes.data, len(ins), n_params, n_grids)
+        return status, L.fe_last_error().decode()
+
+    for tree in (selftest_v1(E)[0], balanced(E, 16), chain(E, 200, True), random_tree(E, np.random.default_rng(3), 6)):
+        p = E.compile_tree(tree)
+        assert check(p.ins, len(p.params), len(p.grids))[0] == 0
+    S, U = E.SPHERE, E.UNION
+    two = raw((S, 0, 0), (S, 0, 0), (U, 0, 4))
+    assert check(two)[0] == 0
+    # deepest legal stack: 17 leaves, then 16 unions
+    deep = raw(*([(S, 0, 0)] * 17 + [(U, 0, 4)] * 16))
+    assert check(deep)[0] == 0
+    cases = {
+        "value stack deeper than 17": raw(*([(S, 0, 0)] * 18 + [(U, 0, 4)] * 17)),
+        "binary operation without two values": raw((S, 0, 0), (U, 0, 4)),
+        "unknown opcode": raw((S, 0, 0), (99, 0, 0)),
+        "parameters out of range": raw((S, 0, 62)),
+        "grid leaf out of range": raw((E.GRID, 1, 0)),
+        "axis out of range": raw((E.CYL, 3, 0)),
+        "bad order flag": raw((S, 0, 0), (S, 0, 0), (U, 2, 4)),
+        "unary operation without a value": raw((E.OFFSET, 0, 0), (S, 0, 0)),
+        "does not end with exactly one value": raw((S, 0, 0), (S, 0, 0)),
+    }
+    for what, ins in cases.items():
+        status, msg = check(ins)
+        assert status == 1 and what in msg, (what, msg)
+    assert check(raw((S, 0, -1)))[0] == 1
+    assert L.fe_ikarus_check(None, 1, 4, 0) == 1 and check(np.zeros((0, 4), np.int32))[0] == 1
+
+
 def test_json_round_trip(E, tmp_path):
     rng = np.random.default_rng(9)
     tree = random_tree(E, rng, 5)

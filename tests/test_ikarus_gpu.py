@@ -179,3 +179,38 @@ def test_band_tree_evaluator(E, gpu):
         assert np.all(ev.eval(pts[:10], []) == np.float32(1.0e30))
     with pytest.raises(ValueError):
         E.BandTreeEvaluator(leaves[0], 3)
+
+
+def test_malformed_programs_stop_and_latch(E, fe, gpu):
+    """Raw programs past the host check: stack underflow, a push past the 16
+    LDS slots and an unknown opcode evaluate to NaN and latch the fault word,
+    which fe_ikarus_fault reports once; a well-formed program after them is
+    unaffected and leaves no fault."""
+    import torch
+    L, P = fe._lib.lib(), fe._lib.ptr
+    S, U = E.SPHERE, E.UNION
+    params = torch.tensor([1.0, 2.0, 3.0, 4.0, 0.0, 0.0, 0.0, 0.0], device=gpu)
+    pts = torch.rand((1000, 3), device=gpu) * 10
+    out = torch.empty(1000, device=gpu)
+    st = fe._lib.stream_handle(None)
+    assert L.fe_ikarus_fault(st) == 0
+    bad = {
+        "underflow": [(S, 0, 0), (U, 0, 4)],
+        "overflow": [(S, 0, 0)] * 18 + [(U, 0, 4)] * 17,
+        "opcode": [(S, 0, 0), (99, 0, 0)],
+    }
+    for name, ins in bad.items():
+        prog = np.array([(op, a, pofs, 0) for op, a, pofs in ins], np.int32)
+        assert L.fe_ikarus_check(prog.ctypes.data, len(prog), 8, 0) == 1, name
+        d = torch.from_numpy(prog.reshape(-1)).to(gpu)
+        print("malformed program:", name, flush=True)
+        assert L.fe_ikarus_eval(P(d), len(prog), P(params), None, None, P(pts), 1000, P(out), st) == 0
+        assert torch.isnan(out).all().item(), name
+        assert L.fe_ikarus_fault(st) == 1 and "value stack" in L.fe_last_error().decode(), name
+        assert L.fe_ikarus_fault(st) == 0, name
+    good = np.array([(S, 0, 0, 0), (S, 0, 0, 0), (U, 0, 4, 0)], np.int32)
+    d = torch.from_numpy(good.reshape(-1)).to(gpu)
+    assert L.fe_ikarus_eval(P(d), 3, P(params), None, None, P(pts), 1000, P(out), st) == 0
+    ref = torch.linalg.norm(pts - params[:3], dim=1) - 4.0
+    assert torch.isfinite(out).all().item() and torch.allclose(out, ref, atol=1e-5)
+    assert L.fe_ikarus_fault(st) == 0
