@@ -57,7 +57,7 @@ struct EdtVol {
     int64_t plane;      // elements from one plane to the next (>= nx*ny*nz)
     int32_t nx, ny, nz, sentinel;
     uint32_t first[3];  // first tile of the volume in the z, y and x pass
-    uint32_t pad;
+    int32_t planes;     // polarities (1 or 2) of the launch
 };
 static_assert(sizeof(EdtVol) == 56, "EdtVol layout");
 
@@ -391,11 +391,15 @@ __global__ __launch_bounds__(W* T) void k_edt_lines(const int32_t* __restrict__ 
     const int L = axis == 1 ? V.ny : V.nx;
     const int64_t axis_stride = axis == 1 ? (int64_t)nz : yz;
     const int64_t outer_stride = axis == 1 ? yz : (int64_t)nz;
-    const int outer_count = axis == 1 ? V.nx : V.ny;
+    // tiles run z tile fastest, then plane, then the outer axis: the XCDs,
+    // which take contiguous tile ranges, each get both planes' lines in equal
+    // shares (plane 1 of a pair holds mostly constant lines, plane 0 the long
+    // envelopes), and the z tiles of one row stay together for the L2
     const int ztiles = (nz + W - 1) / W;
     const int zt = local % ztiles;
     const int rest = local / ztiles;
-    const int outer = rest % outer_count, plane = rest / outer_count;
+    const int nplanes = V.planes > 1 ? V.planes : 1;
+    const int outer = rest / nplanes, plane = rest % nplanes;
     const int64_t base = V.data + (int64_t)plane * V.plane + (int64_t)outer * outer_stride + (int64_t)zt * W;
     const int nlines = min(W, nz - zt * W);
 
@@ -911,9 +915,10 @@ hipError_t run_passes(const uint8_t* masks, int32_t* work, int32_t* out, const E
     return launch_lines(work, out, table, nvol, one, 0, Lmax_x, tiles[2], compact, false, st);
 }
 
-EdtVol single_vol(int nx, int ny, int nz, int64_t plane)
+EdtVol single_vol(int nx, int ny, int nz, int64_t plane, int planes)
 {
     EdtVol v{};
+    v.planes = planes;
     v.mask = 0;
     v.data = 0;
     v.plane = plane;
@@ -942,7 +947,7 @@ extern "C" int fe_edt_device(const uint8_t* mask, int nx, int ny, int nz, int pl
     if (!mask || !work || !out || nx < 1 || ny < 1 || nz < 1 || nx > FE_EDT_MAX_AXIS || ny > FE_EDT_MAX_AXIS ||
         nz > FE_EDT_MAX_AXIS || (planes != 1 && planes != 2))
         return FE_INVALID;
-    const EdtVol one = single_vol(nx, ny, nz, (int64_t)nx * ny * nz);
+    const EdtVol one = single_vol(nx, ny, nz, (int64_t)nx * ny * nz, planes);
     uint64_t tiles[3];
     const bool zsum = nz >= ZSUM_MIN_NZ;
     vol_tiles(nx, ny, nz, planes, pick_cfg(ny).W, pick_cfg(nx).W, zsum, tiles);
@@ -1015,7 +1020,7 @@ extern "C" int fe_edt_axis_pass(const void* src, int32_t* dst, int nx, int ny, i
         nz > FE_EDT_MAX_AXIS || (planes != 1 && planes != 2) || axis < 0 || axis > 2)
         return FE_INVALID;
     hipStream_t st = static_cast<hipStream_t>(stream);
-    const EdtVol one = single_vol(nx, ny, nz, (int64_t)nx * ny * nz);
+    const EdtVol one = single_vol(nx, ny, nz, (int64_t)nx * ny * nz, planes);
     uint64_t tiles[3];
     const bool zsum = nz >= ZSUM_MIN_NZ;
     vol_tiles(nx, ny, nz, planes, pick_cfg(ny).W, pick_cfg(nx).W, zsum, tiles);
@@ -1071,7 +1076,7 @@ extern "C" void* fe_edt_batch_create(const int32_t* dims, const int64_t* mask_of
             if (mask_off[v] < 0 || mask_off[v] > mask_bytes - n || data_off[v] < 0 || plane_stride[v] < n ||
                 data_off[v] + (planes - 1) * plane_stride[v] > data_elems - n)
                 return nullptr;
-            host[v] = single_vol(nx, ny, nz, plane_stride[v]);
+            host[v] = single_vol(nx, ny, nz, plane_stride[v], planes);
             host[v].mask = mask_off[v];
             host[v].data = data_off[v];
             ranges[v] = {data_off[v], data_off[v] + (planes - 1) * plane_stride[v] + n};
