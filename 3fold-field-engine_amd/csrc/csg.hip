@@ -748,75 +748,81 @@ extern "C" int csg_execute(const void* const* nodes, const int64_t* counts, cons
 namespace {
 // Per node, in record order: status 3 for a malformed leaf; for a leaf
 // overlapping the tile, every covered target records the first covering node.
+// A workgroup per node record (grid-stride), its threads over the overlap.
 __global__ __launch_bounds__(256) void k_gather_first(const Node52* __restrict__ nodes, int64_t count, int64_t samples,
                                                       const int* __restrict__ lower, const int* __restrict__ upper,
                                                       int* __restrict__ first, int* __restrict__ mal)
 {
-    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (i >= count) return;
-    const Node52 nd = nodes[i];
-    if (nd.children[0] >= 0) return;
-    int lo[3], hi[3];
-    bool ov = true;
-    int64_t size = 1;
-    for (int a = 0; a < 3; ++a) {
-        if (nd.lo[a] < 0 || nd.shape[a] < 1 || nd.shape[a] > 4096 || nd.lo[a] > 4096 - nd.shape[a] ||
-            (nd.stored[a] != 1 && nd.stored[a] != nd.shape[a])) {
-            atomicMin(mal, (int)i);
-            return;
+    for (int64_t i = blockIdx.x; i < count; i += gridDim.x) {
+        const Node52 nd = nodes[i];
+        if (nd.children[0] >= 0) continue;
+        int lo[3], hi[3];
+        bool ov = true, bad = false;
+        int64_t size = 1;
+        for (int a = 0; a < 3; ++a) {
+            if (nd.lo[a] < 0 || nd.shape[a] < 1 || nd.shape[a] > 4096 || nd.lo[a] > 4096 - nd.shape[a] ||
+                (nd.stored[a] != 1 && nd.stored[a] != nd.shape[a]))
+                bad = true;
+            size *= nd.stored[a];
+            lo[a] = max(lower[a], nd.lo[a]);
+            hi[a] = min(upper[a], nd.lo[a] + nd.shape[a]);
+            ov = ov && hi[a] > lo[a];
         }
-        size *= nd.stored[a];
-        lo[a] = max(lower[a], nd.lo[a]);
-        hi[a] = min(upper[a], nd.lo[a] + nd.shape[a]);
-        ov = ov && hi[a] > lo[a];
+        const int64_t off = noff(nd);
+        if (bad || off < 0 || size > samples || off > samples - size) {
+            if (threadIdx.x == 0) atomicMin(mal, (int)i);
+            continue;
+        }
+        if (!ov) continue;
+        const int dy = upper[1] - lower[1], dz = upper[2] - lower[2];
+        const int ey = hi[1] - lo[1], ez = hi[2] - lo[2];
+        const int n = (hi[0] - lo[0]) * ey * ez;
+        for (int t = threadIdx.x; t < n; t += 256) {
+            const int x = lo[0] + t / (ey * ez), y = lo[1] + (t / ez) % ey, z = lo[2] + t % ez;
+            atomicMin(&first[((x - lower[0]) * dy + (y - lower[1])) * dz + (z - lower[2])], (int)i);
+        }
     }
-    const int64_t off = noff(nd);
-    if (off < 0 || size > samples || off > samples - size) {
-        atomicMin(mal, (int)i);
-        return;
-    }
-    if (!ov) return;
-    const int dy = upper[1] - lower[1], dz = upper[2] - lower[2];
-    for (int x = lo[0]; x < hi[0]; ++x)
-        for (int y = lo[1]; y < hi[1]; ++y)
-            for (int z = lo[2]; z < hi[2]; ++z)
-                atomicMin(&first[((x - lower[0]) * dy + (y - lower[1])) * dz + (z - lower[2])], (int)i);
 }
 
 // Second visit: a covered target whose first node is another one marks an
-// overlap at this node; the first node copies its bits.
+// overlap at this node; the first node copies its bits.  A workgroup per node
+// record, as above.
 __global__ __launch_bounds__(256) void k_gather_copy(const Node52* __restrict__ nodes, int64_t count,
                                                      const uint32_t* __restrict__ payload, const int* __restrict__ lower,
                                                      const int* __restrict__ upper, const int* __restrict__ first,
                                                      int* __restrict__ ovl, uint32_t* __restrict__ out,
                                                      const int* __restrict__ mal)
 {
-    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (i >= count || (int64_t)*mal <= i) return;  // records after a malformed one are never read
-    const Node52 nd = nodes[i];
-    if (nd.children[0] >= 0) return;
-    int lo[3], hi[3];
-    for (int a = 0; a < 3; ++a) {
-        lo[a] = max(lower[a], nd.lo[a]);
-        hi[a] = min(upper[a], nd.lo[a] + nd.shape[a]);
-        if (hi[a] <= lo[a]) return;
-    }
-    const int dy = upper[1] - lower[1], dz = upper[2] - lower[2];
-    const int64_t off = noff(nd);
-    for (int x = lo[0]; x < hi[0]; ++x)
-        for (int y = lo[1]; y < hi[1]; ++y)
-            for (int z = lo[2]; z < hi[2]; ++z) {
-                const int t = ((x - lower[0]) * dy + (y - lower[1])) * dz + (z - lower[2]);
-                if (first[t] != (int)i) {
-                    atomicMin(ovl, (int)i);
-                } else {
-                    int64_t s = off;
-                    s += (int64_t)(nd.stored[0] == 1 ? 0 : x - nd.lo[0]) * nd.stored[1] * nd.stored[2];
-                    s += (int64_t)(nd.stored[1] == 1 ? 0 : y - nd.lo[1]) * nd.stored[2];
-                    s += nd.stored[2] == 1 ? 0 : z - nd.lo[2];
-                    out[t] = payload[s];
-                }
+    const int64_t stop = min(count, (int64_t)*mal);  // records after a malformed one are never read
+    for (int64_t i = blockIdx.x; i < stop; i += gridDim.x) {
+        const Node52 nd = nodes[i];
+        if (nd.children[0] >= 0) continue;
+        int lo[3], hi[3];
+        bool ov = true;
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = max(lower[a], nd.lo[a]);
+            hi[a] = min(upper[a], nd.lo[a] + nd.shape[a]);
+            ov = ov && hi[a] > lo[a];
+        }
+        if (!ov) continue;
+        const int dy = upper[1] - lower[1], dz = upper[2] - lower[2];
+        const int ey = hi[1] - lo[1], ez = hi[2] - lo[2];
+        const int n = (hi[0] - lo[0]) * ey * ez;
+        const int64_t off = noff(nd);
+        for (int t = threadIdx.x; t < n; t += 256) {
+            const int x = lo[0] + t / (ey * ez), y = lo[1] + (t / ez) % ey, z = lo[2] + t % ez;
+            const int g = ((x - lower[0]) * dy + (y - lower[1])) * dz + (z - lower[2]);
+            if (first[g] != (int)i) {
+                atomicMin(ovl, (int)i);
+            } else {
+                int64_t s2 = off;
+                s2 += (int64_t)(nd.stored[0] == 1 ? 0 : x - nd.lo[0]) * nd.stored[1] * nd.stored[2];
+                s2 += (int64_t)(nd.stored[1] == 1 ? 0 : y - nd.lo[1]) * nd.stored[2];
+                s2 += nd.stored[2] == 1 ? 0 : z - nd.lo[2];
+                out[g] = payload[s2];
             }
+        }
+    }
 }
 }  // namespace
 
@@ -855,7 +861,7 @@ extern "C" int csg_gather(const void* raw, int64_t count, const void* payload, i
             if (hipMemcpy(dn, raw, (size_t)count * 52, hipMemcpyHostToDevice) != hipSuccess) break;
             if (hipMemcpy(dp, payload, (size_t)samples * 4, hipMemcpyHostToDevice) != hipSuccess) break;
             if (hipMemcpy(di, init.data(), init.size() * 4, hipMemcpyHostToDevice) != hipSuccess) break;
-            const unsigned g = (unsigned)((count + 255) / 256);
+            const unsigned g = (unsigned)std::min<int64_t>(count, 65536);  // workgroups over the node records
             hipLaunchKernelGGL(k_gather_first, dim3(g), dim3(256), 0, nullptr, dn, count, samples, di, di + 3, di + 8, di + 6);
             hipLaunchKernelGGL(k_gather_copy, dim3(g), dim3(256), 0, nullptr, dn, count, dp, di, di + 3, di + 8, di + 7,
                                dout, di + 6);

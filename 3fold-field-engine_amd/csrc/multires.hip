@@ -20,10 +20,11 @@
 // products value*wx*wy*wz summed in f64 with the last axis fastest, the sum
 // rounded to fp32.
 //
-// MI355X design: one launch evaluates every brick's tile (waves over 64-sample
-// chunks of all bricks, the brick found by binary search, primitive records
-// read wave-uniformly through the culled list of the brick's superblock from
-// the coarse classification); one launch per refined level writes the bands
+// MI355X design: one launch evaluates every brick's tile (a wave per slab of
+// about 1024 samples of a brick, the brick found by binary search, the
+// culled list of the brick's superblock from the coarse classification
+// refined exactly for the slab's box, primitive records read wave-uniformly);
+// one launch per refined level writes the bands
 // (a workgroup per fine brick, faces in the reference's order); one launch
 // answers a batch of queries (a thread per point).
 #include "recipe_eval.hpp"
@@ -33,7 +34,7 @@
 struct MrBrick {          // one active brick
     int32_t bi, bj, bk, level;
     int64_t tile;         // element offset of its (B*2^level + 1)^3 samples in the tile arena
-    int64_t chunk0;       // its first 64-sample chunk in the tile launch
+    int64_t item0;        // its first slab in the tile launch (mr_slab_rows rows of samples each)
 };
 static_assert(sizeof(MrBrick) == 32, "brick record");
 
@@ -46,54 +47,85 @@ static_assert(sizeof(MrGrid) == 72, "grid record");
 
 namespace {
 
-__device__ __forceinline__ int find_brick(const MrBrick* __restrict__ bricks, int n, int64_t chunk)
+__device__ __forceinline__ int find_brick(const MrBrick* __restrict__ bricks, int n, int64_t item)
 {
-    int lo = 0, hi = n - 1;  // last brick whose first chunk is <= chunk
+    int lo = 0, hi = n - 1;  // last brick whose first slab is <= item
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
-        if (bricks[mid].chunk0 <= chunk) lo = mid;
+        if (bricks[mid].item0 <= item) lo = mid;
         else hi = mid - 1;
     }
     return lo;
 }
 
-// Every active brick's samples.  The superblock's culled list (when it was
-// stored) holds every primitive that can change the fold anywhere in the
-// superblock's closed box, which holds the brick's closed cube.
+// Rows (along the brick's first axis) of one slab of a brick with m samples
+// per axis: about 1024 samples per slab.  The host counts slabs by the same
+// rule (multires.py, _SLAB_SAMPLES).
+__host__ __device__ __forceinline__ int mr_slab_rows(int m) { return max(1, 1024 / (m * m)); }
+
+// Every active brick's samples, one wave per slab.  The wave refines the
+// culled list of the brick's superblock (the whole recipe when none was
+// stored) for the slab's box with the exact rule of cull_wave, so the dropped
+// primitives change no result bit, then folds the refined list at the slab's
+// samples, two per lane per pass.  Sample coordinates: level 0 at
+// x0 + f32(min(b*B + i, nx - 1)) * h, level l at x0 + f32(b*B*2^l + i) * (h/2^l);
+// both are monotone in i, so the slab's extreme samples bound its box.
 __global__ __launch_bounds__(256) void k_mr_tiles(const FieldDesc* __restrict__ desc, const MrBrick* __restrict__ bricks,
-                                                  int nbricks, int64_t nchunks, float* __restrict__ tiles)
+                                                  int nbricks, int64_t nitems, float* __restrict__ tiles)
 {
+    __shared__ uint16_t wlist[4][FE_MAX_PRIMS / 2];
     const FieldDesc& D = *desc;
     const int lane = threadIdx.x & 63;
-    const int64_t w0 = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * 256 + threadIdx.x) >> 6));
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int64_t nw = (int64_t)gridDim.x * 4;
-    for (int64_t c = w0; c < nchunks; c += nw) {
-        const int b = __builtin_amdgcn_readfirstlane(find_brick(bricks, nbricks, c));
+    uint16_t* wl = wlist[wv];
+    for (int64_t it = (int64_t)blockIdx.x * 4 + wv; it < nitems; it += nw) {
+        const int b = __builtin_amdgcn_readfirstlane(find_brick(bricks, nbricks, it));
         const MrBrick Bk = bricks[b];
-        const int n = D.B << Bk.level, m = n + 1;
-        const int64_t m3 = (int64_t)m * m * m;
-        const int64_t s = (c - Bk.chunk0) * 64 + lane;
-        const int64_t sc = s < m3 ? s : m3 - 1;
-        const int i = (int)(sc / ((int64_t)m * m)), j = (int)((sc / m) % m), k = (int)(sc % m);
-        float x, y, z;
-        if (Bk.level == 0) {
-            const int gi = min(Bk.bi * D.B + i, D.nx - 1), gj = min(Bk.bj * D.B + j, D.ny - 1),
-                      gk = min(Bk.bk * D.B + k, D.nz - 1);
-            x = D.x0 + (float)gi * D.pitch;
-            y = D.y0 + (float)gj * D.pitch;
-            z = D.z0 + (float)gk * D.pitch;
-        } else {
-            const float hl = ldexpf(D.pitch, -Bk.level);  // exact: a power-of-two scaling
-            x = D.x0 + (float)(Bk.bi * n + i) * hl;
-            y = D.y0 + (float)(Bk.bj * n + j) * hl;
-            z = D.z0 + (float)(Bk.bk * n + k) * hl;
-        }
+        const int n = D.B << Bk.level, m = n + 1, mm = m * m;
+        const int i0 = (int)(it - Bk.item0) * mr_slab_rows(m), i1 = min(m, i0 + mr_slab_rows(m));
+        const float hl = ldexpf(D.pitch, -Bk.level);  // exact: a power-of-two scaling
+        auto coord = [&](float o, int bb, int q, int nmax) {
+            return Bk.level == 0 ? o + (float)min(bb * D.B + q, nmax - 1) * D.pitch : o + (float)(bb * n + q) * hl;
+        };
+        const float lx = coord(D.x0, Bk.bi, i0, D.nx), ux = coord(D.x0, Bk.bi, i1 - 1, D.nx);
+        const float ly = coord(D.y0, Bk.bj, 0, D.ny), uy = coord(D.y0, Bk.bj, m - 1, D.ny);
+        const float lz = coord(D.z0, Bk.bk, 0, D.nz), uz = coord(D.z0, Bk.bk, m - 1, D.nz);
+        const float cx = 0.5f * (lx + ux), cy = 0.5f * (ly + uy), cz = 0.5f * (lz + uz);
+        const float ex = ux - lx, ey = uy - ly, ez = uz - lz;
+        const float rad = 0.5f * sqrtf(ex * ex + ey * ey + ez * ez) * 1.0001f;
         const int sb = ((Bk.bi / D.SB) * D.nsby + Bk.bj / D.SB) * D.nsbz + Bk.bk / D.SB;
         const int nl = D.list_len[sb];
-        const uint16_t* list = nl >= 0 ? D.lists + (int64_t)sb * D.cap : nullptr;
-        float v;
-        eval_points<1>(D.prims, list, nl >= 0 ? nl : D.nprims, &x, &y, &z, &v);
-        if (s < m3) tiles[Bk.tile + s] = v;
+        const uint16_t* src = nl >= 0 ? D.lists + (int64_t)sb * D.cap : nullptr;
+        const int nsrc = nl >= 0 ? nl : D.nprims;
+        const uint16_t* list = src;
+        int np = nsrc;
+        if (nsrc <= FE_MAX_PRIMS / 2) {
+            np = cull_wave(D.prims, src, nsrc, cx, cy, cz, rad, D.slack, wl);
+            // a second pass over the refined list (in place): its runs join
+            // across dropped steps and are no longer cut every 64 entries
+            if (np > 8) np = cull_wave(D.prims, wl, np, cx, cy, cz, rad, D.slack, wl);
+            list = wl;
+        }
+        const int S = (i1 - i0) * mm;
+        float* out = tiles + Bk.tile + (int64_t)i0 * mm;
+        for (int v0 = 0; v0 < S; v0 += 128) {
+            float x[2], y[2], z[2], f[2];
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                const int s = min(v0 + k * 64 + lane, S - 1);
+                const int i = i0 + s / mm, j = (s / m) % m, q = s % m;
+                x[k] = coord(D.x0, Bk.bi, i, D.nx);
+                y[k] = coord(D.y0, Bk.bj, j, D.ny);
+                z[k] = coord(D.z0, Bk.bk, q, D.nz);
+            }
+            eval_points<2>(D.prims, list, np, x, y, z, f);
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                const int s = v0 + k * 64 + lane;
+                if (s < S) out[s] = f[k];
+            }
+        }
     }
 }
 
@@ -246,16 +278,17 @@ __global__ __launch_bounds__(256) void k_mr_query(MrGrid G, const MrBrick* __res
 }  // namespace
 
 // ---------------------------------------------------------------- C ABI
-// Tiles of `nbricks` active bricks (device table, chunk0 ascending from 0,
-// nchunks = total 64-sample chunks) from the recipe of one classified field
-// descriptor (device; its culled lists as fe_field_classify left them).
-extern "C" int fe_multires_tiles(const void* desc, const void* bricks, int nbricks, int64_t nchunks, float* tiles,
+// Tiles of `nbricks` active bricks (device table, item0 ascending from 0,
+// nitems = total slabs, mr_slab_rows rows of a brick each) from the recipe of
+// one classified field descriptor (device; its culled lists as
+// fe_field_classify left them).
+extern "C" int fe_multires_tiles(const void* desc, const void* bricks, int nbricks, int64_t nitems, float* tiles,
                                  void* stream) noexcept
 {
-    if (!desc || !bricks || nbricks < 1 || nchunks < 1 || !tiles) return FE_INVALID;
-    const int64_t blocks = std::min<int64_t>((nchunks + 3) / 4, 65536);
+    if (!desc || !bricks || nbricks < 1 || nitems < 1 || !tiles) return FE_INVALID;
+    const int64_t blocks = std::min<int64_t>((nitems + 3) / 4, 65536);
     hipLaunchKernelGGL(k_mr_tiles, dim3((unsigned)blocks), dim3(256), 0, static_cast<hipStream_t>(stream),
-                       static_cast<const FieldDesc*>(desc), static_cast<const MrBrick*>(bricks), nbricks, nchunks,
+                       static_cast<const FieldDesc*>(desc), static_cast<const MrBrick*>(bricks), nbricks, nitems,
                        tiles);
     FE_TRY(hipGetLastError());
     return FE_OK;

@@ -1,6 +1,6 @@
-// Recipe records, field descriptors and the fp32 recipe evaluation shared by
-// the block-sparse build (field.hip) and the per-block-pitch field
-// (multires.hip).
+// Recipe records, field descriptors, the fp32 recipe evaluation and the
+// exact wave-level culling of primitive lists, shared by the block-sparse
+// build (field.hip) and the per-block-pitch field (multires.hip).
 #pragma once
 #include "fe_common.hpp"
 
@@ -131,3 +131,110 @@ __device__ __forceinline__ void eval_points(const Prim* __restrict__ prims, cons
     }
 }
 
+// ---------------------------------------------------------------- culling scan
+// A clamp x -> min(max(x,l),u).  c2(c1(x)) is the clamp (c2(l1), c2(u1)).
+struct Clamp2 {
+    float ll, lu, hl, hu;  // clamps of the lower and upper fold bounds
+};
+__device__ __forceinline__ float capply(float x, float l, float u) { return fminf(fmaxf(x, l), u); }
+__device__ __forceinline__ Clamp2 ccompose(const Clamp2& a, const Clamp2& b)  // a first, then b
+{
+    return {capply(a.ll, b.ll, b.lu), capply(a.lu, b.ll, b.lu), capply(a.hl, b.hl, b.hu), capply(a.hu, b.hl, b.hu)};
+}
+__device__ __forceinline__ Clamp2 cshfl_up(const Clamp2& v, int d)
+{
+    return {__shfl_up(v.ll, d), __shfl_up(v.lu, d), __shfl_up(v.hl, d), __shfl_up(v.hu, d)};
+}
+
+// Interval of one primitive's fp32 values over a box given by centre and
+// half diagonal; `slack` (1e-4 of the largest coordinate or size magnitude)
+// covers fp32 rounding of both evaluations and the few-ulp error of the
+// centre value evaluated with bare hardware square roots.
+__device__ __forceinline__ void prim_interval(const Prim& p, float cx, float cy, float cz, float rad, float slack,
+                                              float& lo, float& hi)
+{
+    float v = prim_eval<true>(p.kind, p.cx, p.cy, p.cz, p.a, p.b, p.c, cx, cy, cz);
+    lo = v - rad - slack;
+    hi = v + rad + slack;
+}
+
+// Culling rule.  Consecutive steps of one op form a run, and the fold over a
+// run is a selection over its operands (min for union, max for subtraction
+// and intersection, plus the incoming value), which returns the extreme value
+// and, among operands equal to it, the last.  An operand that is strictly
+// worse than some other operand of its run everywhere in the box is never
+// that value, so dropping it leaves every result bit unchanged; a run's
+// interval after its last step bounds the best other operand.  Runs are cut at
+// wave boundaries (each cut run is itself a run) and primitive 0, which
+// starts the fold, is its own run and always kept.
+
+// Inclusive scan value at the end of this lane's run within the wave.
+__device__ __forceinline__ Clamp2 run_end(const Clamp2& inc, int op, bool first, bool active, bool has_next)
+{
+    const int lane = threadIdx.x & 63;
+    const int me = first ? 3 : op;
+    const int nx = __shfl_down(me, 1);
+    const bool end = active && (lane == 63 || !has_next || nx != me || me == 3);
+    const uint64_t ends = __ballot(end);
+    const uint64_t after = ends >> lane;
+    const int e = after ? lane + __builtin_ctzll(after) : lane;
+    return {__shfl(inc.ll, e), __shfl(inc.lu, e), __shfl(inc.hl, e), __shfl(inc.hu, e)};
+}
+
+// Keep an operand with value interval [lo, hi] whose run ends with the fold
+// interval [rlo, rhi]: union needs lo <= rhi, subtraction -lo >= rlo,
+// intersection hi >= rlo.
+__device__ __forceinline__ bool keep_step(int op, bool first, float lo, float hi, float rlo, float rhi)
+{
+    if (first) return true;
+    if (op == 0) return !(lo > rhi);
+    if (op == 1) return !(-lo < rlo);
+    return !(hi < rlo);
+}
+
+// Wave-level refinement of a primitive list for a smaller box (a group of
+// blocks or one block).  src: the parent list (its first entry is primitive 0)
+// or nullptr for all primitives; out: per-wave LDS array (may be src: entry j
+// is read before any entry at or past the write position is written).  The
+// exact rule above, scanned 64 candidates at a time (runs of the list,
+// which join across dropped steps of other ops).
+static __device__ int cull_wave(const Prim* __restrict__ prims, const uint16_t* src, int nsrc, float cx, float cy, float cz,
+                         float rad, float slack, uint16_t* out)
+{
+    const int lane = threadIdx.x & 63;
+    float Flo = 0.0f, Fhi = 0.0f;
+    int n = 0;
+    for (int base = 0; base < nsrc; base += 64) {
+        const int j = base + lane;
+        Clamp2 cl = {-INFINITY, INFINITY, -INFINITY, INFINITY};
+        float lo = 0.0f, hi = 0.0f;
+        int op = 0, idx = 0;
+        if (j < nsrc) {
+            idx = src ? (int)src[j] : j;
+            const Prim p = prims[idx];
+            prim_interval(p, cx, cy, cz, rad, slack, lo, hi);
+            op = p.op;
+            if (j == 0) cl = {lo, lo, hi, hi};
+            else if (op == 0) cl = {-INFINITY, lo, -INFINITY, hi};
+            else if (op == 1) cl = {-hi, INFINITY, -lo, INFINITY};
+            else cl = {lo, INFINITY, hi, INFINITY};
+        }
+        Clamp2 inc = cl;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            Clamp2 o = cshfl_up(inc, d);
+            if (lane >= d) inc = ccompose(o, inc);
+        }
+        const Clamp2 run = run_end(inc, op, j == 0, j < nsrc, j + 1 < nsrc);
+        const float rlo = capply(Flo, run.ll, run.lu), rhi = capply(Fhi, run.hl, run.hu);
+        const bool keep = j < nsrc && keep_step(op, j == 0, lo, hi, rlo, rhi);
+        const uint64_t bal = __ballot(keep);
+        if (keep) out[n + __popcll(bal & ((1ull << lane) - 1))] = (uint16_t)idx;
+        n += __popcll(bal);
+        const float all_ll = __shfl(inc.ll, 63), all_lu = __shfl(inc.lu, 63);
+        const float all_hl = __shfl(inc.hl, 63), all_hu = __shfl(inc.hu, 63);
+        Flo = capply(Flo, all_ll, all_lu);
+        Fhi = capply(Fhi, all_hl, all_hu);
+    }
+    return n;
+}

@@ -36,7 +36,14 @@ NIVAER = (0, 1, 2)
 MULTIRES_PITCH_FAKTOR = 10.0
 
 MR_BRICK = np.dtype([("bi", "<i4"), ("bj", "<i4"), ("bk", "<i4"), ("level", "<i4"), ("tile", "<i8"),
-                     ("chunk0", "<i8")])
+                     ("item0", "<i8")])
+_SLAB_SAMPLES = 1024  # a tile launch's work item: rows of a brick, about this many samples
+
+
+def slab_rows(m):
+    """Rows of one slab of a brick with m samples per axis (csrc/multires.hip,
+    mr_slab_rows)."""
+    return np.maximum(1, _SLAB_SAMPLES // (m * m))
 MR_GRID = np.dtype([("lo", "<f8", (3,)), ("h", "<f8"), ("margin", "<f8"), ("B", "<i4"), ("nb", "<i4", (3,)),
                     ("n", "<i4", (3,)), ("pad", "<i4")])
 assert MR_BRICK.itemsize == 32 and MR_GRID.itemsize == 72
@@ -224,13 +231,14 @@ def bygg_multires_recept(meta, recipe, niva=None, overlappsband=True, device="cu
     order = np.argsort(lin)
     act, lin = act[order], lin[order]
     lv = niva[act[:, 0], act[:, 1], act[:, 2]].astype(np.int64)
-    samples = ((B << lv) + 1) ** 3
+    m = (B << lv) + 1
+    samples = m ** 3
     bricks = np.zeros(len(act), MR_BRICK)
     bricks["bi"], bricks["bj"], bricks["bk"] = act[:, 0], act[:, 1], act[:, 2]
     bricks["level"] = lv
     bricks["tile"] = np.concatenate([[0], np.cumsum(samples)[:-1]]) if len(act) else []
-    chunks = (samples + 63) // 64
-    bricks["chunk0"] = np.concatenate([[0], np.cumsum(chunks)[:-1]]) if len(act) else []
+    items = -(-m // slab_rows(m))
+    bricks["item0"] = np.concatenate([[0], np.cumsum(items)[:-1]]) if len(act) else []
     index = np.full(int(np.prod(nb)), -1, np.int32)
     index[lin] = np.arange(len(act), dtype=np.int32)
     tiles = torch.empty(max(int(samples.sum()), 1), dtype=torch.float32, device=dev)
@@ -239,7 +247,7 @@ def bygg_multires_recept(meta, recipe, niva=None, overlappsband=True, device="cu
     d_slot = torch.from_numpy(index).to(dev)
     d_kind = torch.from_numpy(kind.reshape(-1).astype(np.int32)).to(dev)
     if len(act):
-        F_._check(L_.fe_multires_tiles(_lib.ptr(plan.desc), _lib.ptr(d_bricks), len(act), int(chunks.sum()),
+        F_._check(L_.fe_multires_tiles(_lib.ptr(plan.desc), _lib.ptr(d_bricks), len(act), int(items.sum()),
                                        _lib.ptr(tiles), st), "multires tiles")
     F = MultiresField(lo=lo, h_grov=h, block=B, brick_shape=nb, kind=kind, niva=niva,
                       margin_mm=math.sqrt(3.0) * B * h,
@@ -249,9 +257,9 @@ def bygg_multires_recept(meta, recipe, niva=None, overlappsband=True, device="cu
     n_faces = 0
     if overlappsband:
         g = _grid_record(F)
+        n_faces = _coarser_faces(F)
         for level in NIVAER[1:]:
             fine = np.flatnonzero(bricks["level"] == level).astype(np.int32)
-            n_faces += _coarser_faces(F, bricks[fine])
             if len(fine):
                 d_fine = torch.from_numpy(fine).to(dev)
                 F_._check(L_.fe_multires_band(g.ctypes.data, _lib.ptr(d_bricks), _lib.ptr(d_slot), _lib.ptr(d_fine),
@@ -265,19 +273,16 @@ def bygg_multires_recept(meta, recipe, niva=None, overlappsband=True, device="cu
     return F
 
 
-def _coarser_faces(F, fine_bricks):
-    """Faces of the given bricks that touch an active brick of a lower level."""
+def _coarser_faces(F):
+    """Faces between an active brick and an active face neighbour of a lower
+    level (the faces the overlap band rewrites)."""
+    lv = np.where(F.kind == 0, F.niva, -1).astype(np.int16)
     n = 0
-    nb = np.array(F.brick_shape)
-    for b in fine_bricks:
-        p = np.array([b["bi"], b["bj"], b["bk"]])
-        for ax in range(3):
-            for side in (-1, 1):
-                q = p.copy()
-                q[ax] += side
-                if np.any(q < 0) or np.any(q >= nb) or F.kind[tuple(q)] != 0:
-                    continue
-                n += int(F.niva[tuple(q)] < b["level"])
+    for ax in range(3):
+        a = np.moveaxis(lv, ax, 0)
+        lo, hi = a[:-1], a[1:]
+        both = (lo >= 0) & (hi >= 0)
+        n += int(np.count_nonzero(both & (lo != hi)))
     return n
 
 
@@ -336,11 +341,17 @@ def gradient(F, punkter):
     return g
 
 
-def kontinuitetscheck(F, eps_frac=1.0e-3, max_punkter_per_face=None):
+def kontinuitetscheck(F, eps_frac=1.0e-3, max_punkter_per_face=None, bara_inre=False):
     """Largest jump of the delivered field across the faces where the pitch
     changes (:462-520): every such face sampled on the finer side's face
     lattice, the query taken eps_frac of the finer pitch on each side; the
-    bound is half the coarser pitch.  Every face's points go in one query."""
+    bound is half the coarser pitch.  Every face's points go in one query.
+
+    bara_inre (an extension, default off = the reference's check): only the
+    face lattice's interior points, whose two probes stay in the two bricks
+    of the face.  A probe from a point on a face's edge can land in a third,
+    diagonal brick; when that brick is inactive it answers +-margin, which the
+    reference's check then reports as a jump of the field."""
     nb = np.array(F.brick_shape)
     faces, inner, outer = [], [], []
     for b in F.dev["bricks_host"]:
@@ -358,7 +369,7 @@ def kontinuitetscheck(F, eps_frac=1.0e-3, max_punkter_per_face=None):
                 lvl_n = int(F.niva[tuple(q)])
                 if lvl_n >= lvl:
                     continue
-                idx = [np.arange(n_l + 1, dtype=np.float64)] * 3
+                idx = [np.arange(1 if bara_inre else 0, n_l if bara_inre else n_l + 1, dtype=np.float64)] * 3
                 idx[ax] = np.array([0.0 if side < 0 else float(n_l)])
                 gi, gj, gk = np.meshgrid(*idx, indexing="ij")
                 pts = np.stack([gi.ravel(), gj.ravel(), gk.ravel()], axis=1) * h_f + b_origin

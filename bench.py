@@ -346,13 +346,55 @@ def kernel_launches(fe, torch, n, reps=3):
     pc = bench_pack_csg(fe, torch, dense, reps=reps)
     del dense
     torch.cuda.empty_cache()
+    mr = bench_multires(fe, torch, reps=reps)
     src = torch.ones(1 << 28, dtype=torch.int32, device=dev)
     dst = torch.empty_like(src)
     for _ in range(reps):
         dst.copy_(src)
     torch.cuda.synchronize()
     return {"n": n, "n_active": na, "n_blocks": meta.n_blocks, "copy_bytes_each_way": 1 << 30, "reps": reps,
-            "pack_csg": pc}
+            "pack_csg": pc, "multires": mr}
+
+
+def bench_multires(fe, torch, reps=3, queries=1 << 20):
+    """Per-block pitch on the 128^3 part (64 primitives, block 8, levels from
+    the primitives' radii at h_l <= r/10, balanced): the whole build (host
+    level map and balance, the device tiles and overlap bands) by wall time,
+    the device distance queries by HIP events, and the continuity report.
+    Query bytes per point: 24 in, 8 out, 8 tile samples of 4 bytes."""
+    M = fe.multires
+    dev = torch.device("cuda:0")
+    rec = fe.recipes.plain_part_128(seed=1)
+    meta = fe.GridMeta(0.0, 0.0, 0.0, 1.0, 128, 128, 128, block=8)
+    F = M.bygg_multires_recept(meta, rec)
+    times = []
+    for _ in range(reps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        F = M.bygg_multires_recept(meta, rec)
+        times.append((time.perf_counter() - t0) * 1e3)
+    rng = np.random.default_rng(9)
+    pts = torch.from_numpy(rng.uniform(0.0, 128.0, (queries, 3))).to(dev)
+    out = torch.empty(queries, dtype=torch.float64, device=dev)
+    g = M._grid_record(F)
+    L, P, st = fe._lib.lib(), fe._lib.ptr, fe._lib.stream_handle(None)
+    q_ms = event_ms(torch, lambda: L.fe_multires_query(g.ctypes.data, P(F.dev["bricks"]), P(F.dev["slot"]),
+                                                       P(F.dev["kind"]), P(F.dev["tiles"]), P(pts), queries, P(out),
+                                                       None, st), reps)
+    c = M.kontinuitetscheck(F)
+    ci = M.kontinuitetscheck(F, bara_inre=True)
+    return {"bricks": int(np.prod(F.brick_shape)), "active": F.stats["n_aktiva"], "per_level": F.stats["n_per_niva"],
+            "stored_samples": M.lagrade_sampel(F), "active_voxels": M.aktiva_voxlar(F),
+            "build_ms": float(np.median(times)), "queries": queries, "query_ms": q_ms,
+            "query_mpts_s": queries / (q_ms * 1e-3) / 1e6, "query_gb_s": 64 * queries / (q_ms * 1e-3) / 1e9,
+            "continuity": {"faces": c["n_pitchbyten_faces"], "max_jump_mm": c["max_hopp_mm"],
+                           "bound_mm": c["grans_mm"], "green": c["GRON"],
+                           "interior_max_jump_mm": ci["max_hopp_mm"], "interior_green": ci["GRON"],
+                           "note": "the reference's check samples whole face lattices; a probe from a face "
+                                   "edge can land in an inactive diagonal brick (+-margin), which it reports "
+                                   "as a jump; interior: the face points whose probes stay in the two bricks"},
+            "note": "build_ms is the whole bygg_multires_recept call (classification, host level map and "
+                    "balance, tiles, bands, host statistics)"}
 
 
 def bench_octree(fe, torch, dense, reps=3, queries=1 << 20):
@@ -619,6 +661,27 @@ def bench_pack_csg(fe, torch, dense, reps=3):
                csg_note="csg_ms is the whole fe_csg_run call (table uploads, kernel, status read-back); csg_gb_s "
                         "counts every operand sample read once (4N(F+1) + 52 sum(nodes)), csg_stored_gb_s the "
                         "stored payloads and tables once plus the output")
+    # csg_gather (the reference's host-array tile reader): one (16, 32, 16) tile
+    # of the packed 512^3 field per call; each call uploads the node table and
+    # payload (host arrays in, as the interface takes them), so the call time
+    # is PCIe-bound and the kernels' own durations come from the kernel trace
+    hn, hp = nodes.cpu().numpy(), payload.cpu().numpy()
+    lo = np.array([248, 240, 248], np.int32)
+    hi = lo + np.array([16, 32, 16], np.int32)
+    tile = np.empty((16, 32, 16), np.float32)
+    times = []
+    for _ in range(reps + 1):
+        t0 = time.perf_counter()
+        code = L.csg_gather(hn.ctypes.data, n_nodes, hp.ctypes.data, n_pay, lo.ctypes.data, hi.ctypes.data,
+                            tile.ctypes.data)
+        times.append((time.perf_counter() - t0) * 1e3)
+        assert code == 0, code
+    want = dense[248:264, 240:272, 248:264].cpu().numpy()
+    assert tile.tobytes() == want.tobytes()
+    out.update(gather_tile=[16, 32, 16], gather_call_ms=float(np.median(times[1:])),
+               gather_upload_bytes=52 * n_nodes + 4 * n_pay,
+               gather_note="gather_call_ms is the whole host-array call: node and payload upload, the two scan "
+                           "kernels over every node record, the status and tile read-back")
     del fields, others, out_dense
     torch.cuda.empty_cache()
     return out
@@ -672,7 +735,7 @@ def main():
     ap.add_argument("--batch-count", type=int, default=256)
     ap.add_argument("--serve-count", type=int, default=1000)
     ap.add_argument("--serve-rates", default="200,500,1000,2000")
-    ap.add_argument("--only", choices=["edt-heavy", "kernels", "mesh", "ikarus", "batch", "serve", "cfg1",
+    ap.add_argument("--only", choices=["edt-heavy", "kernels", "mesh", "ikarus", "batch", "serve", "cfg1", "multires",
                                        "pack-csg", "cpu"], default=None,
                     help="one section only: edt-heavy (thin-feature EDT), kernels (each hot kernel of the 512^3 "
                          "step three times plus a 1 GiB copy, for rocprofv3 counter runs), mesh, ikarus, batch "
@@ -691,6 +754,7 @@ def main():
     rates = tuple(float(r) for r in args.serve_rates.split(","))
     single = {"edt-heavy": lambda: {"edt_heavy": bench_edt_heavy(fe, torch)},
               "kernels": lambda: kernel_launches(fe, torch, args.n),
+              "multires": lambda: {"multires": bench_multires(fe, torch)},
               "mesh": lambda: {"mesh": bench_mesh(fe, torch, cpu=not args.no_cpu)},
               "ikarus": lambda: {"ikarus": bench_ikarus(fe, torch, cpu=not args.no_cpu)},
               "batch": lambda: {"batched": bench_batch(fe, torch, args.batch_count)},
@@ -805,6 +869,7 @@ def main():
         out["edt_heavy"] = bench_edt_heavy(fe, torch)
         out["mesh"] = bench_mesh(fe, torch, cpu=not args.no_cpu)
         out["ikarus"] = bench_ikarus(fe, torch, cpu=not args.no_cpu)
+        out["multires"] = bench_multires(fe, torch)
     if not args.no_cpu:
         cb = cpu_baseline(fe, rec, occ_host, n, env)
         cb["fp64_check"] = fp64_check(rec, dense_host)

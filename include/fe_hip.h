@@ -180,19 +180,22 @@ int fe_query_points(const void *desc, const float *points, int64_t n, float *out
 /* The two-level field of faltkarna_v1_multires.py:84-106 (build :254-363,
  * band :211-252, queries :405-441), from a recipe instead of a mesh.  bricks:
  * device table of 32-byte records {int32 bi, bj, bk, level; int64 tile
- * (offset of its (B*2^level + 1)^3 fp32 samples), chunk0 (its first 64-sample
- * chunk, ascending from 0)}; grid: host 72-byte record {double lo[3], h,
+ * (offset of its (B*2^level + 1)^3 fp32 samples), item0 (its first slab,
+ * ascending from 0; a brick with m samples per axis has ceil(m / r) slabs of
+ * r = max(1, 1024 / m^2) rows)}; grid: host 72-byte record {double lo[3], h,
  * margin; int32 B, nb[3], n[3], pad} (f64 origin and pitch, brick lattice,
  * coarse sample counts); slot / kind: device int32 per lattice brick (table
  * index or -1; -1 interior, 0 active, +1 exterior).  tiles: every active
- * brick's samples from one classified descriptor (its culled lists), level 0
+ * brick's samples from one classified descriptor (its culled lists, refined
+ * exactly per slab; nitems = all slabs), level 0
  * on the coarse lattice with indices clamped to the grid, level l at
  * x0 + f32(i) * (h / 2^l).  band: the face layers of the `fine` bricks (all
  * at one level) next to coarser active bricks, as
  * map_coordinates(order=1, mode="nearest") of those bricks.  query: signed
  * distance at device f64 points (n x 3) into device f64 out; brick_out
- * (optional, device int32 n x 4): brick and level per point. */
-int fe_multires_tiles(const void *desc, const void *bricks, int nbricks, int64_t nchunks, float *tiles, void *stream);
+ * (optional, device int32 n x 4): the (clipped) brick and its level (-1
+ * inactive) per point. */
+int fe_multires_tiles(const void *desc, const void *bricks, int nbricks, int64_t nitems, float *tiles, void *stream);
 int fe_multires_band(const void *grid, const void *bricks, const int32_t *slot, const int32_t *fine, int nfine,
                      float *tiles, void *stream);
 int fe_multires_query(const void *grid, const void *bricks, const int32_t *slot, const int32_t *kind,

@@ -170,8 +170,10 @@ class Field:
             g[:, ax] = (self.avstand(pa) - self.avstand(pb)) / np.maximum(pa[:, ax] - pb[:, ax], 1e-12)
         return g
 
-    def continuity(self, eps_frac=1.0e-3):
-        """(faces, points, max jump, bound) over the pitch-change faces."""
+    def continuity(self, eps_frac=1.0e-3, interior=False):
+        """(faces, points, max jump, bound) over the pitch-change faces;
+        interior: only the face lattice's interior points (the package's
+        bara_inre extension)."""
         faces, npts, mx, bound = 0, 0, 0.0, 0.0
         for lin in sorted(self.tiles):
             bi, bj, bk = self.brick_of_lin(lin)
@@ -188,7 +190,7 @@ class Field:
                     lvl_n = int(self.niva[tuple(nb)])
                     if lvl_n >= lvl:
                         continue
-                    idx = [np.arange(n_l + 1, dtype=np.float64)] * 3
+                    idx = [np.arange(1 if interior else 0, n_l if interior else n_l + 1, dtype=np.float64)] * 3
                     idx[ax] = np.array([0.0 if side < 0 else float(n_l)])
                     gi, gj, gk = np.meshgrid(*idx, indexing="ij")
                     pts = np.stack([gi.ravel(), gj.ravel(), gk.ravel()], axis=1) * h_f + b_origin

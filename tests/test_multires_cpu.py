@@ -77,7 +77,8 @@ def test_affine_field_reproduced_and_continuous():
 
 def interior_face_jumps(F, eps_frac=1e-6):
     """Largest jump over the interior points of every pitch-change face (edge
-    points can fall into an inactive diagonal brick, which answers +-margin)."""
+    points can fall into an inactive diagonal brick, which answers +-margin),
+    restated here from the face lattice as a check of the oracle's option."""
     mx = 0.0
     for lin in sorted(F.tiles):
         b = np.array(F.brick_of_lin(lin))
@@ -105,9 +106,10 @@ def test_band_closes_jumps_of_the_built_field(fine_axis):
     """Without the band the fine side's own samples disagree with the coarse
     side's interpolant by the interpolation error; with it the interior face
     points agree to fp32 rounding (1e-5 on values below 40)."""
-    before = interior_face_jumps(small_field(band=False, fine_axis=fine_axis)[0])
-    after = interior_face_jumps(small_field(band=True, fine_axis=fine_axis)[0])
+    F0, F1 = small_field(band=False, fine_axis=fine_axis)[0], small_field(band=True, fine_axis=fine_axis)[0]
+    before, after = interior_face_jumps(F0), interior_face_jumps(F1)
     assert before > 1e-3 and after < 1e-5, (before, after)
+    assert F0.continuity(1e-6, interior=True)[2] == before and F1.continuity(1e-6, interior=True)[2] == after
 
 
 def test_balance_rules_and_package_equal(fe):

@@ -84,10 +84,11 @@ def compare(fe, meta, rec, niva, band, what, rng):
     gp = pts[::7]
     assert M.gradient(G, gp).tobytes() == R.gradient(gp).tobytes(), what
     for eps in (1e-3, 1e-6):
-        c = M.kontinuitetscheck(G, eps_frac=eps)
-        faces, npts, mx, bound = R.continuity(eps_frac=eps)
-        assert (c["n_pitchbyten_faces"], c["n_provpunkter"], c["max_hopp_mm"], c["grans_mm"]) == \
-            (faces, npts, mx, bound), (what, eps)
+        for inner in (False, True):
+            c = M.kontinuitetscheck(G, eps_frac=eps, bara_inre=inner)
+            faces, npts, mx, bound = R.continuity(eps_frac=eps, interior=inner)
+            assert (c["n_pitchbyten_faces"], c["n_provpunkter"], c["max_hopp_mm"], c["grans_mm"]) == \
+                (faces, npts, mx, bound), (what, eps, inner)
     return G, R, levels
 
 
