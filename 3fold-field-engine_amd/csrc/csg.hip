@@ -48,7 +48,7 @@ struct FieldRef {  // one packed operand on the device
 };
 
 struct MOp {  // micro-op of the compiled program
-    int32_t kind;  // 0 read input a into slot dst, 1 union, 2 difference, 3 offset
+    int32_t kind;  // 0 read input a into slot dst (b: the read's index), 1 union, 2 difference, 3 offset
     int32_t dst, a, b;
     double off;
 };
@@ -81,16 +81,51 @@ __device__ uint32_t descend(const Node52* __restrict__ nodes, const uint32_t* __
     }
 }
 
-// Reads field f over the tile [tlo, thi) into slot (layout x,y,z with TY*TZ, TZ strides).
+// Deepest node of a tree whose subtree holds every leaf overlapping the box
+// [tlo, thi): from the root, down while exactly one child's box overlaps it.
+// Child indices are checked (in range, after the parent) before a program
+// runs, so the walk ends.
+__device__ int tile_start(const Node52* __restrict__ nodes, const int* tlo, const int* thi)
+{
+    int id = 0;
+    for (;;) {
+        const Node52& nd = nodes[id];
+        if (nd.children[0] < 0) return id;
+        const Node52& c0 = nodes[nd.children[0]];
+        const Node52& c1 = nodes[nd.children[1]];
+        const bool o0 = box_overlap(c0.lo, c0.shape, tlo, thi), o1 = box_overlap(c1.lo, c1.shape, tlo, thi);
+        if (o0 == o1) return id;
+        id = o0 ? nd.children[0] : nd.children[1];
+    }
+}
+
+// Start nodes of every (read, tile): one thread each, before the program runs,
+// so a tile's breadth-first read starts below the levels that hold it whole.
+__global__ __launch_bounds__(256) void k_csg_starts(const FieldRef* __restrict__ fields, const int* __restrict__ reads,
+                                                    int nreads, int nx, int ny, int nz, int gty, int gtz, int ntiles,
+                                                    int* __restrict__ starts)
+{
+    const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (g >= (int64_t)nreads * ntiles) return;
+    const int r = (int)(g / ntiles), t = (int)(g % ntiles);
+    const int tz = t % gtz, ty = (t / gtz) % gty, tx = t / (gtz * gty);
+    const int tlo[3] = {tx * TX, ty * TY, tz * TZ};
+    const int thi[3] = {min(tlo[0] + TX, nx), min(tlo[1] + TY, ny), min(tlo[2] + TZ, nz)};
+    starts[g] = tile_start(fields[reads[r]].nodes, tlo, thi);
+}
+
+// Reads field f over the tile [tlo, thi) into slot (layout x,y,z with TY*TZ, TZ
+// strides): a breadth-first walk from node `start` collects the leaves
+// overlapping the tile, then the workgroup copies each leaf's part.
 __device__ void read_tile(const FieldRef& F, const int* tlo, const int* thi, float* slot, int* cur, int* nxt, int* leaves,
-                          int* counters)
+                          int* counters, int start)
 {
     const int tid = threadIdx.x;
     if (tid == 0) {
         counters[0] = 1;  // frontier size
         counters[2] = 0;  // leaves
         counters[3] = 0;  // overflow
-        cur[0] = 0;
+        cur[0] = start;
     }
     __syncthreads();
     for (;;) {
@@ -164,7 +199,8 @@ __device__ __forceinline__ int ref_tile(int x, int y, int z, int nty, int ntz)
 __global__ __launch_bounds__(256) void k_csg_tiles(const FieldRef* __restrict__ fields, const MOp* __restrict__ prog,
                                                    int nops, int final_slot, int lds_slots, int spill_slots,
                                                    float* __restrict__ arena, int nx, int ny, int nz, int gty, int gtz,
-                                                   int ntiles, float* __restrict__ out, int* __restrict__ err_nf)
+                                                   int ntiles, const int* __restrict__ starts, float* __restrict__ out,
+                                                   int* __restrict__ err_nf)
 {
     extern __shared__ float slots[];
     __shared__ int cur[FRONT], nxt[FRONT], leaves[FRONT];
@@ -184,7 +220,8 @@ __global__ __launch_bounds__(256) void k_csg_tiles(const FieldRef* __restrict__ 
             const MOp m = prog[o];
             float* dst = slot(m.dst);
             if (m.kind == 0) {
-                read_tile(fields[m.a], tlo, thi, dst, cur, nxt, leaves, counters);
+                read_tile(fields[m.a], tlo, thi, dst, cur, nxt, leaves, counters,
+                          starts ? starts[(int64_t)m.b * ntiles + t] : 0);
                 continue;
             }
             // elementwise: dst may be one of the operands' slots
@@ -430,9 +467,12 @@ size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
 // Launch geometry of a compiled program on the current device.
 struct Plan {
     std::vector<MOp> ops;
+    std::vector<int> reads;  // field of each read micro-op, in read order
     int nslots = 0, final_slot = 0, lds_slots = 0, spill = 0, grid = 0, ntiles = 0, gty = 0, gtz = 0;
-    size_t lds = 0, ws_refs = 0, ws_ops = 0, ws_flags = 0, ws_arena = 0, ws_total = 0;
+    bool starts = false;     // the start-node map fits (CSG_STARTS_MAX bytes)
+    size_t lds = 0, ws_refs = 0, ws_ops = 0, ws_flags = 0, ws_reads = 0, ws_starts = 0, ws_arena = 0, ws_total = 0;
 };
+constexpr size_t CSG_STARTS_MAX = size_t(1) << 28;  // start-node map bytes; beyond, reads start at the root
 
 hipError_t make_plan(int32_t fields, const Instr24* program, int32_t n, const std::vector<int>& last,
                      const int32_t* shape, Plan& P)
@@ -462,7 +502,16 @@ hipError_t make_plan(int32_t fields, const Instr24* program, int32_t n, const st
     P.ws_ops = align256(sizeof(MOp) * P.ops.size());
     P.ws_flags = 256;
     P.ws_arena = (size_t)P.grid * P.spill * TS * 4;
-    P.ws_total = P.ws_refs + P.ws_ops + P.ws_flags + P.ws_arena;
+    for (MOp& m : P.ops)
+        if (m.kind == 0) {
+            m.b = (int)P.reads.size();
+            P.reads.push_back(m.a);
+        }
+    P.ws_reads = align256(sizeof(int) * std::max<size_t>(P.reads.size(), 1));
+    const size_t map = (size_t)4 * P.reads.size() * (size_t)P.ntiles;
+    P.starts = map <= CSG_STARTS_MAX;
+    P.ws_starts = P.starts ? align256(map) : 0;
+    P.ws_total = P.ws_refs + P.ws_ops + P.ws_flags + P.ws_reads + P.ws_starts + P.ws_arena;
     return hipSuccess;
 }
 
@@ -474,15 +523,24 @@ int run_plan(const Plan& P, const std::vector<FieldRef>& refs, const int32_t* sh
     FieldRef* dref = reinterpret_cast<FieldRef*>(ws);
     MOp* dops = reinterpret_cast<MOp*>(ws + P.ws_refs);
     int* flag = reinterpret_cast<int*>(ws + P.ws_refs + P.ws_ops);
-    float* arena = reinterpret_cast<float*>(ws + P.ws_refs + P.ws_ops + P.ws_flags);
+    int* dreads = reinterpret_cast<int*>(ws + P.ws_refs + P.ws_ops + P.ws_flags);
+    int* dstarts = P.starts ? reinterpret_cast<int*>(ws + P.ws_refs + P.ws_ops + P.ws_flags + P.ws_reads) : nullptr;
+    float* arena = reinterpret_cast<float*>(ws + P.ws_refs + P.ws_ops + P.ws_flags + P.ws_reads + P.ws_starts);
     const int none = INT_MAX;
     int nf = INT_MAX;
     FE_TRY(hipMemcpyAsync(dref, refs.data(), sizeof(FieldRef) * refs.size(), hipMemcpyHostToDevice, st));
     FE_TRY(hipMemcpyAsync(dops, P.ops.data(), sizeof(MOp) * P.ops.size(), hipMemcpyHostToDevice, st));
     FE_TRY(hipMemcpyAsync(flag, &none, sizeof none, hipMemcpyHostToDevice, st));
+    if (dstarts && !P.reads.empty()) {
+        FE_TRY(hipMemcpyAsync(dreads, P.reads.data(), sizeof(int) * P.reads.size(), hipMemcpyHostToDevice, st));
+        const int64_t n = (int64_t)P.reads.size() * P.ntiles;
+        hipLaunchKernelGGL(k_csg_starts, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, dref, dreads,
+                           (int)P.reads.size(), shape[0], shape[1], shape[2], P.gty, P.gtz, P.ntiles, dstarts);
+        FE_TRY(hipGetLastError());
+    }
     hipLaunchKernelGGL(k_csg_tiles, dim3((unsigned)P.grid), dim3(256), P.lds, st, dref, dops, (int)P.ops.size(),
                        P.final_slot, P.lds_slots, P.spill, arena, shape[0], shape[1], shape[2], P.gty, P.gtz, P.ntiles,
-                       output, flag);
+                       dstarts, output, flag);
     FE_TRY(hipGetLastError());
     FE_TRY(hipMemcpyAsync(&nf, flag, sizeof nf, hipMemcpyDeviceToHost, st));
     FE_TRY(hipStreamSynchronize(st));
