@@ -69,6 +69,93 @@ __global__ __launch_bounds__(256) void k_pack_inv(PNode* __restrict__ arena, con
     }
 }
 
+// Run lengths of equal bits along each axis: r_a(p) = how many samples from p
+// on (along +a) hold p's bits, so a box is invariant along a exactly when
+// every sample of its lower a-face has r_a >= the box's extent along a.
+// uint16 (axes <= 4096).  z: a wave per (x, y) row, 64 z per ballot, rows
+// walked from the end with the next break carried.
+__global__ __launch_bounds__(256) void k_runs_z(const uint32_t* __restrict__ d, int nx, int ny, int nz,
+                                                uint16_t* __restrict__ rz)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t rows = (int64_t)nx * ny;
+    const int64_t nw = (int64_t)gridDim.x * 4;
+    for (int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); row < rows; row += nw) {
+        const uint32_t* line = d + row * nz;
+        uint16_t* out = rz + row * nz;
+        int carry = nz - 1;  // first break at or after the chunk after this one
+        for (int c0 = ((nz - 1) >> 6) << 6; c0 >= 0; c0 -= 64) {
+            const int z = c0 + lane;
+            bool brk = false;
+            if (z < nz) brk = z == nz - 1 || line[z] != line[z + 1];
+            const uint64_t m = __ballot(brk);
+            const uint64_t mine = lane ? m >> lane : m;
+            const int nb = mine ? z + __builtin_ctzll(mine) : carry;
+            if (z < nz) out[z] = (uint16_t)(nb - z + 1);
+            if (m) carry = c0 + __builtin_ctzll(m);
+        }
+    }
+}
+
+// y and x: a thread per column, walked from the end (adjacent threads hold
+// adjacent z, so every step's loads and stores are coalesced).
+__global__ __launch_bounds__(256) void k_runs_yx(const uint32_t* __restrict__ d, int n_outer, int n_line, int nz,
+                                                 int64_t s_outer, int64_t s_line, uint16_t* __restrict__ r)
+{
+    const int64_t cols = (int64_t)n_outer * nz;
+    for (int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x; c < cols; c += (int64_t)gridDim.x * 256) {
+        const int64_t base = (c / nz) * s_outer + c % nz;
+        int64_t o = base + (int64_t)(n_line - 1) * s_line;
+        uint32_t next = d[o];
+        int run = 1;
+        r[o] = 1;
+#pragma unroll 4
+        for (int i = n_line - 2; i >= 0; --i) {
+            o -= s_line;
+            const uint32_t v = d[o];
+            run = v == next ? run + 1 : 1;
+            next = v;
+            r[o] = (uint16_t)run;
+        }
+    }
+}
+
+// The three invariance flags of every node of one level from its faces'
+// run lengths: z from r_z on the face z = lo, y from r_y on y = lo, x from
+// r_x on x = lo.  S parts per node over the concatenated faces.
+__global__ __launch_bounds__(256) void k_pack_inv_faces(PNode* __restrict__ arena, const int64_t* __restrict__ lv, int S,
+                                                        const uint16_t* __restrict__ rx, const uint16_t* __restrict__ ry,
+                                                        const uint16_t* __restrict__ rz, int64_t sx, int64_t sy)
+{
+    const int64_t start = lv[0], n = lv[1];
+    for (int64_t it = blockIdx.x; it < n * S; it += gridDim.x) {
+        PNode& P = arena[start + it / S];
+        const int part = (int)(it % S);
+        const int hx = P.shape[0], hy = P.shape[1], hz = P.shape[2];
+        const int64_t fz = (int64_t)hx * hy, fy = (int64_t)hx * hz, fx = (int64_t)hy * hz;
+        const int64_t cnt = fz + fy + fx;
+        const int64_t b = cnt * part / S, e = cnt * (part + 1) / S;
+        const int64_t o = (int64_t)P.lo[0] * sx + (int64_t)P.lo[1] * sy + P.lo[2];
+        bool ex = true, ey = true, ez = true;
+        for (int64_t l = b + threadIdx.x; l < e; l += 256) {
+            if (l < fz) {
+                const int y = (int)(l % hy), x = (int)(l / hy);
+                ez = ez && rz[o + x * sx + y * sy] >= hz;
+            } else if (l < fz + fy) {
+                const int64_t k = l - fz;
+                const int z = (int)(k % hz), x = (int)(k / hz);
+                ey = ey && ry[o + x * sx + z] >= hy;
+            } else {
+                const int64_t k = l - fz - fy;
+                const int z = (int)(k % hz), y = (int)(k / hz);
+                ex = ex && rx[o + y * sy + z] >= hx;
+            }
+        }
+        const int m = (__ballot(!ex) ? 0 : 1) | (__ballot(!ey) ? 0 : 2) | (__ballot(!ez) ? 0 : 4);
+        if ((threadIdx.x & 63) == 0 && m != 7) atomicAnd(&P.inv, m);
+    }
+}
+
 __device__ void init_node(PNode& c, const int* lo, const int* shape)
 {
     for (int a = 0; a < 3; ++a) {
@@ -233,6 +320,8 @@ struct Packed {
 
 // Work buffers of pack_device kept between calls (per device, grown as
 // needed): the candidate-node arena, the level ranges and three counters.
+constexpr size_t PACK_RUNS_MAX = size_t(8) << 30;  // bytes of run lengths (6 per sample) held for a pack
+
 struct PackArena {
     char* p = nullptr;
     size_t n = 0;
@@ -285,14 +374,21 @@ Packed* pack_device(const float* data, int nx, int ny, int nz, int leaf, hipStre
     for (int attempt = 0; attempt < 2; ++attempt) {
         const int64_t cap = attempt == 0 ? 4 * pieces + 1024 : 2 * (int64_t)depth * pieces + 1024;
         const size_t lv_bytes = (size_t)(depth + 2) * 16, ctl_bytes = 32;
+        const size_t nodes_bytes = ((size_t)cap * sizeof(PNode) + 255) & ~(size_t)255;
+        // run lengths (2 bytes per sample and axis) when they fit; else each
+        // level scans its nodes' samples
+        const bool runs = (size_t)6 * (size_t)N <= PACK_RUNS_MAX;
         char* base = nullptr;
-        if (hipError_t e = pack_arena(ctl_bytes + lv_bytes + (size_t)cap * sizeof(PNode), &base)) {
+        if (hipError_t e = pack_arena(ctl_bytes + lv_bytes + nodes_bytes + (runs ? (size_t)6 * N : 0), &base)) {
             fe_note_error(e, "pack arena");
             return nullptr;
         }
         unsigned long long* ctl = reinterpret_cast<unsigned long long*>(base);  // [0] cursor [1] overflow [2] non-finite
         int64_t* lv = reinterpret_cast<int64_t*>(base + ctl_bytes);            // per level {start, count}
         PNode* arena = reinterpret_cast<PNode*>(base + ctl_bytes + lv_bytes);
+        uint16_t* rz = runs ? reinterpret_cast<uint16_t*>(base + ctl_bytes + lv_bytes + nodes_bytes) : nullptr;
+        uint16_t* ry = runs ? rz + N : nullptr;
+        uint16_t* rx = runs ? ry + N : nullptr;
         PNode root;
         std::memset(&root, 0, sizeof root);
         root.shape[0] = nx;
@@ -319,11 +415,28 @@ Packed* pack_device(const float* data, int nx, int ny, int nz, int leaf, hipStre
         hipLaunchKernelGGL(k_nonfinite, dim3((unsigned)std::min<int64_t>((N + 255) / 256, 4096)), dim3(256), 0, st, d, N,
                            ctl + 2);
         const unsigned grid_nodes = 2048;
+        if (runs) {
+            hipLaunchKernelGGL(k_runs_z, dim3((unsigned)std::min<int64_t>(((int64_t)nx * ny + 3) / 4, 65536)), dim3(256),
+                               0, st, d, nx, ny, nz, rz);
+            const int64_t cy = (int64_t)nx * nz, cx = (int64_t)ny * nz;
+            hipLaunchKernelGGL(k_runs_yx, dim3((unsigned)std::min<int64_t>((cy + 255) / 256, 65536)), dim3(256), 0, st, d,
+                               nx, ny, nz, sx, sy, ry);
+            hipLaunchKernelGGL(k_runs_yx, dim3((unsigned)std::min<int64_t>((cx + 255) / 256, 65536)), dim3(256), 0, st, d,
+                               ny, nx, nz, sy, sx, rx);
+        }
         for (int l = 0; l < depth; ++l) {
-            // parts per node: about 16 K samples each at this level's node size
+            // parts per node: about 16 K samples each at this level's node
+            // size (its faces' samples with run lengths)
             const int64_t per = std::max<int64_t>(1, N >> std::min(l, 62));
-            const int S = (int)std::min<int64_t>(4096, (per + 16383) / 16384);
-            hipLaunchKernelGGL(k_pack_inv, dim3(grid_nodes * 4), dim3(256), 0, st, arena, lv + 2 * l, S, d, sx, sy);
+            if (runs) {
+                const double face = 3.0 * std::pow((double)per, 2.0 / 3.0);
+                const int S = (int)std::min(4096.0, std::max(1.0, std::ceil(face / 16384.0)));
+                hipLaunchKernelGGL(k_pack_inv_faces, dim3(grid_nodes * 4), dim3(256), 0, st, arena, lv + 2 * l, S, rx, ry,
+                                   rz, sx, sy);
+            } else {
+                const int S = (int)std::min<int64_t>(4096, (per + 16383) / 16384);
+                hipLaunchKernelGGL(k_pack_inv, dim3(grid_nodes * 4), dim3(256), 0, st, arena, lv + 2 * l, S, d, sx, sy);
+            }
             hipLaunchKernelGGL(k_pack_decide, dim3(grid_nodes), dim3(256), 0, st, arena, lv + 2 * l, leaf, cap, ctl);
             hipLaunchKernelGGL(k_pack_level, dim3(1), dim3(1), 0, st, lv + 2 * l, ctl, cap);
         }
@@ -378,24 +491,34 @@ __device__ __forceinline__ int64_t node_offset(const Node52& n)
     return (int64_t)(((uint64_t)(uint32_t)n.off_hi << 32) | (uint32_t)n.off_lo);
 }
 
-// Leaf lookup by tree descent (FE/adaptive_sample_blocks_v1.py:103-128).
-__device__ __forceinline__ uint32_t lookup(const Node52* __restrict__ nodes, const uint32_t* __restrict__ payload, int x,
-                                           int y, int z)
+// Leaf of (x, y, z) by tree descent (FE/adaptive_sample_blocks_v1.py:103-128):
+// into the first child when its box holds the point, else the second.
+__device__ __forceinline__ int leaf_of(const Node52* __restrict__ nodes, int x, int y, int z)
 {
     int id = 0;
     for (;;) {
         const Node52& nd = nodes[id];
-        if (nd.children[0] < 0) {
-            const int64_t lx = nd.stored[0] == 1 ? 0 : x - nd.lo[0];
-            const int64_t ly = nd.stored[1] == 1 ? 0 : y - nd.lo[1];
-            const int64_t lz = nd.stored[2] == 1 ? 0 : z - nd.lo[2];
-            return payload[node_offset(nd) + (lx * nd.stored[1] + ly) * nd.stored[2] + lz];
-        }
+        if (nd.children[0] < 0) return id;
         const Node52& l = nodes[nd.children[0]];
         const bool in = x >= l.lo[0] && x < l.lo[0] + l.shape[0] && y >= l.lo[1] && y < l.lo[1] + l.shape[1] &&
                         z >= l.lo[2] && z < l.lo[2] + l.shape[2];
         id = in ? nd.children[0] : nd.children[1];
     }
+}
+
+// Bits of (x, y, z) in leaf nd (an axis stored once reads its one sample).
+__device__ __forceinline__ uint32_t leaf_bits(const Node52& nd, const uint32_t* __restrict__ payload, int x, int y, int z)
+{
+    const int64_t lx = nd.stored[0] == 1 ? 0 : x - nd.lo[0];
+    const int64_t ly = nd.stored[1] == 1 ? 0 : y - nd.lo[1];
+    const int64_t lz = nd.stored[2] == 1 ? 0 : z - nd.lo[2];
+    return payload[node_offset(nd) + (lx * nd.stored[1] + ly) * nd.stored[2] + lz];
+}
+
+__device__ __forceinline__ uint32_t lookup(const Node52* __restrict__ nodes, const uint32_t* __restrict__ payload, int x,
+                                           int y, int z)
+{
+    return leaf_bits(nodes[leaf_of(nodes, x, y, z)], payload, x, y, z);
 }
 
 __global__ __launch_bounds__(256) void k_samples_at(const Node52* __restrict__ nodes, const uint32_t* __restrict__ payload,
@@ -432,11 +555,17 @@ __global__ __launch_bounds__(256) void k_query(const Node52* __restrict__ nodes,
         w[a][0] = 1.0 - t;
         w[a][1] = 1.0 - w[a][0];
     }
+    // one descent for the lower corner; a corner inside that leaf's box reads
+    // it directly (the pack's tree splits every node into two children that
+    // partition it, so the corner's own descent would end in the same leaf)
+    const Node52& base = nodes[leaf_of(nodes, lower[0], lower[1], lower[2])];
     double acc = 0.0;
     for (int h = 0; h < 8; ++h) {
         const int bx = (h >> 2) & 1, by = (h >> 1) & 1, bz = h & 1;
         const int qx = min(lower[0] + bx, nx - 1), qy = min(lower[1] + by, ny - 1), qz = min(lower[2] + bz, nz - 1);
-        const uint32_t bits = lookup(nodes, payload, qx, qy, qz);
+        const bool same = qx >= base.lo[0] && qx < base.lo[0] + base.shape[0] && qy >= base.lo[1] &&
+                          qy < base.lo[1] + base.shape[1] && qz >= base.lo[2] && qz < base.lo[2] + base.shape[2];
+        const uint32_t bits = same ? leaf_bits(base, payload, qx, qy, qz) : lookup(nodes, payload, qx, qy, qz);
         double coeff = (double)__uint_as_float(bits);
         coeff *= 1.0;
         coeff *= w[0][bx];
