@@ -51,6 +51,7 @@ SIGNATURES = {
     "fe_multires_band": (ct.c_int, [P, P, P, P, ct.c_int, P, P]),
     "fe_multires_query": (ct.c_int, [P, P, P, P, P, P, I64, P, P, P]),
     "fe_query_points": (ct.c_int, [P, P, I64, P, P]),
+    "fe_copy_segments": (ct.c_int, [P, ct.c_int, P]),
     # leaf-box pack, lookups, CSG, tile reader
     "adaptive_pack": (P, [P, P, ct.c_int]),
     "adaptive_nodes": (I64, [P]),

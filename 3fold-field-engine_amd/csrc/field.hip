@@ -725,6 +725,43 @@ extern "C" int fe_dense_tiles(const float* dense, int nx, int ny, int nz, int B,
     return FE_OK;
 }
 
+// ---------------------------------------------------------------- segment copies
+// Many device-to-device copies in one launch (a field server moving every
+// built field of a batch out of the batch's pools): segment s copies bytes
+// [src, src + n) to dst, or clears dst when src is null.  blockIdx.y picks
+// the segment, blockIdx.x strides over it; 4-byte words when the segment
+// allows (every array of a field is 4-byte aligned except the uint16 lists).
+struct CopySeg {
+    const char* src;
+    char* dst;
+    int64_t n;
+};
+static_assert(sizeof(CopySeg) == 24, "segment record");
+
+__global__ __launch_bounds__(256) void k_copy_segments(const CopySeg* __restrict__ segs)
+{
+    const CopySeg g = segs[blockIdx.y];
+    const int64_t stride = (int64_t)gridDim.x * 256, t0 = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (((reinterpret_cast<uintptr_t>(g.src) | reinterpret_cast<uintptr_t>(g.dst) | (uintptr_t)g.n) & 3) == 0) {
+        const uint32_t* s = reinterpret_cast<const uint32_t*>(g.src);
+        uint32_t* d = reinterpret_cast<uint32_t*>(g.dst);
+        for (int64_t i = t0; i < g.n / 4; i += stride) d[i] = s ? s[i] : 0u;
+    } else {
+        for (int64_t i = t0; i < g.n; i += stride) g.dst[i] = g.src ? g.src[i] : (char)0;
+    }
+}
+
+// nseg segments (device table of {src, dst, bytes} records) in one launch.
+extern "C" int fe_copy_segments(const void* segs, int nseg, void* stream) noexcept
+{
+    if (nseg < 0 || nseg > 65535 || (nseg && !segs)) return FE_INVALID;
+    if (nseg == 0) return FE_OK;
+    hipLaunchKernelGGL(k_copy_segments, dim3(256, (unsigned)nseg), dim3(256), 0, static_cast<hipStream_t>(stream),
+                       static_cast<const CopySeg*>(segs));
+    FE_TRY(hipGetLastError());
+    return FE_OK;
+}
+
 // Point SDF queries on one resident field (desc: device pointer to its
 // descriptor; pts: device float32 (n,3); out: device float32 (n)).
 extern "C" int fe_query_points(const void* desc, const float* pts, int64_t n, float* out, void* stream) noexcept

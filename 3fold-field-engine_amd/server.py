@@ -31,50 +31,68 @@ def _torch():
     return torch
 
 
-class ResidentField:
-    """A built field kept on the device: SparseField data plus a descriptor.
-    n_active: the build's active-block count when the caller has read it
-    already; upload=False leaves the descriptor to the caller (see
-    resident_fields)."""
+SEG = np.dtype([("src", "<u8"), ("dst", "<u8"), ("n", "<i8")])  # fe_copy_segments record
 
-    def __init__(self, plan, j, stream, n_active=None, upload=True):
+
+class ResidentField:
+    """A built field kept on the device: SparseField data plus a descriptor,
+    all in one device buffer (`store`; the arrays are views into it).
+    n_active: the build's active-block count when the caller has read it
+    already.  copy=False / upload=False leave the copies (`segments`) and the
+    descriptor upload to the caller, which batches them (resident_fields)."""
+
+    _PIECES = (("kind", 4), ("ids", 4), ("tiles", 4), ("cma", 4), ("slot", 4), ("list_len", 4), ("lists", 2),
+               ("prims", 1), ("counts", 4))
+
+    def __init__(self, plan, j, stream, n_active=None, upload=True, copy=True):
         torch = _torch()
         meta, rec = plan.items[j]
         self.meta, self.recipe = meta, rec
         na = plan.n_active(j) if n_active is None else n_active
         self.n_active = na
-        B = plan.B
+        self.B = plan.B
+        off = sum(len(r) for _, r in plan.items[:j])
+        src = {"kind": plan.kind(j), "ids": plan.active_ids(j, na), "tiles": plan.tiles(j, na),
+               "cma": plan.corner_min_abs(j), "slot": plan._slice(plan.i32, j, "slot"),
+               "list_len": plan._slice(plan.i32, j, "list_len"), "lists": plan._slice(plan.u16, j, "lists"),
+               "prims": plan.prims[off * 32:(off + len(rec)) * 32], "counts": plan.counts(j)}
+        layout, at = {}, 0
+        for name, _ in self._PIECES:
+            nbytes = src[name].numel() * src[name].element_size()
+            layout[name] = (at, nbytes)
+            at += (nbytes + 255) // 256 * 256
+        dtypes = {"kind": torch.int32, "ids": torch.int32, "tiles": torch.float32, "cma": torch.float32,
+                  "slot": torch.int32, "list_len": torch.int32, "lists": torch.int16, "prims": torch.uint8,
+                  "counts": torch.int32}
         with torch.cuda.stream(stream):
-            self.kind = plan.kind(j).clone()
-            self.ids = plan.active_ids(j, na).clone()
-            self.tiles = plan.tiles(j, na).clone()
-            self.cma = plan.corner_min_abs(j).clone()
-            self.slot = plan._slice(plan.i32, j, "slot").clone()
-            self.list_len = plan._slice(plan.i32, j, "list_len").clone()
-            self.lists = plan._slice(plan.u16, j, "lists").clone()
-            n_prims = len(rec)
-            off = sum(len(r) for _, r in plan.items[:j])
-            self.prims = plan.prims[off * 32:(off + n_prims) * 32].clone()
-            # [n_active, 0, 0, 0], filled on the device (no host round trip)
-            self.counts = torch.zeros(4, dtype=torch.int32, device=plan.device)
-            self.counts[:1].copy_(plan.counts(j)[:1])
+            self.store = torch.empty(max(at, 256), dtype=torch.uint8, device=plan.device)
+        base = self.store.data_ptr()
+        segs = []
+        for name, _ in self._PIECES:
+            o, nbytes = layout[name]
+            setattr(self, name, self.store[o:o + nbytes].view(dtypes[name]))
+            if name == "counts":
+                # [n_active, 0, 0, 0]: the build's count, then zeros (no host round trip)
+                segs += [(src[name].data_ptr(), base + o, 4), (0, base + o + 4, nbytes - 4)]
+            elif nbytes:
+                segs.append((src[name].data_ptr(), base + o, nbytes))
+        self.segments = segs
         d = plan.desc_host[j:j + 1].copy()
         d["prims"], d["kind"], d["cma"] = self.prims.data_ptr(), self.kind.data_ptr(), self.cma.data_ptr()
         d["lists"], d["list_len"], d["ids"] = self.lists.data_ptr(), self.list_len.data_ptr(), self.ids.data_ptr()
         d["slot"], d["tiles"], d["counts"] = self.slot.data_ptr(), self.tiles.data_ptr(), self.counts.data_ptr()
         d["tile_cap"] = na
         self.desc_host = d
-        self.B = B
         self.desc = None
+        if copy:
+            copy_segments(segs, plan.device, stream)
         if upload:
             with torch.cuda.stream(stream):
                 self._set_desc(torch.from_numpy(d.view(np.uint8).copy()).to(plan.device, non_blocking=False))
 
     def _set_desc(self, desc):
         self.desc = desc
-        self.nbytes = sum(t.numel() * t.element_size() for t in
-                          (self.kind, self.ids, self.tiles, self.cma, self.slot, self.list_len, self.lists,
-                           self.prims, self.counts, self.desc))
+        self.nbytes = self.store.numel() + desc.numel()
 
     def query_device(self, points, out=None, stream=None):
         """Point SDF at a device float32 (n,3) tensor."""
@@ -92,15 +110,29 @@ class ResidentField:
                              stats=F._stats(self.meta, kind, self.n_active))
 
 
+def copy_segments(segs, device, stream):
+    """Device copies [(src address, dst address, bytes)] (src 0: clear) in
+    launches of at most 65535 segments, on `stream`."""
+    torch = _torch()
+    for i in range(0, len(segs), 65535):
+        part = np.array(segs[i:i + 65535], SEG)
+        with torch.cuda.stream(stream):
+            table = torch.from_numpy(part.view(np.uint8).copy()).to(device, non_blocking=False)
+        F._check(_lib.lib().fe_copy_segments(_lib.ptr(table), len(part), _lib.stream_handle(stream)),
+                 "segment copies")
+
+
 def resident_fields(plan, stream):
-    """ResidentFields of every build of a finished plan, with one device-to-host
-    read of the active-block counts and one upload of the descriptors, both on
+    """ResidentFields of every build of a finished plan: one device-to-host
+    read of the active-block counts, one launch copying every field's arrays
+    out of the plan's pools, and one upload of the descriptors, all on
     `stream` (so they wait for this plan's work only)."""
     torch = _torch()
     n = len(plan.items)
     with torch.cuda.stream(stream):
         nas = torch.cat([plan.counts(j)[:1] for j in range(n)]).cpu().tolist()
-    fields = [ResidentField(plan, j, stream, n_active=nas[j], upload=False) for j in range(n)]
+    fields = [ResidentField(plan, j, stream, n_active=nas[j], upload=False, copy=False) for j in range(n)]
+    copy_segments([g for f in fields for g in f.segments], plan.device, stream)
     descs = np.concatenate([f.desc_host for f in fields])
     size = descs.dtype.itemsize
     with torch.cuda.stream(stream):

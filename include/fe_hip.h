@@ -175,6 +175,12 @@ int fe_dense_tiles(const float *dense, int nx, int ny, int nz, int B, const int3
  * the per-batch eval_batch launch of ikarus_v1/eval_warp.py:478-489 for a
  * built field. */
 int fe_query_points(const void *desc, const float *points, int64_t n, float *out, void *stream);
+/* No counterpart in the reference (its builds are one per call): `nseg`
+ * device-to-device copies in one launch, for a server moving every built
+ * field of a batch out of the batch's pools.  segs: device table of 24-byte
+ * records {const void *src; void *dst; int64 bytes}; a null src clears dst.
+ * nseg <= 65535. */
+int fe_copy_segments(const void *segs, int nseg, void *stream);
 
 /* ---------------------------------------------------------------- per-block pitch */
 /* The two-level field of faltkarna_v1_multires.py:84-106 (build :254-363,
