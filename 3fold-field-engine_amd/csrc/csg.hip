@@ -121,6 +121,27 @@ __device__ void read_tile(const FieldRef& F, const int* tlo, const int* thi, flo
                           int* counters, int start)
 {
     const int tid = threadIdx.x;
+    const Node52& sn = F.nodes[start];
+    if (sn.children[0] < 0) {  // the tile lies in one leaf: no walk (start is uniform)
+        int lo[3], hi[3];
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = max(tlo[a], sn.lo[a]);
+            hi[a] = min(thi[a], sn.lo[a] + sn.shape[a]);
+        }
+        const int ex = hi[0] - lo[0], ey = hi[1] - lo[1], ez = hi[2] - lo[2];
+        if (ex > 0 && ey > 0 && ez > 0) {
+            const int64_t off = noff(sn);
+            for (int j = tid; j < ex * ey * ez; j += 256) {
+                const int z = lo[2] + j % ez, y = lo[1] + (j / ez) % ey, x = lo[0] + j / (ez * ey);
+                const int64_t sx = sn.stored[0] == 1 ? 0 : x - sn.lo[0], sy = sn.stored[1] == 1 ? 0 : y - sn.lo[1],
+                              sz = sn.stored[2] == 1 ? 0 : z - sn.lo[2];
+                slot[((x - tlo[0]) * TY + (y - tlo[1])) * TZ + (z - tlo[2])] =
+                    __uint_as_float(F.payload[off + (sx * sn.stored[1] + sy) * sn.stored[2] + sz]);
+            }
+        }
+        __syncthreads();
+        return;
+    }
     if (tid == 0) {
         counters[0] = 1;  // frontier size
         counters[2] = 0;  // leaves
