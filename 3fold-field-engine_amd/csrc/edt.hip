@@ -17,10 +17,12 @@
 //           a chunk of positions into registers and build the exact lower
 //           envelope of the chunk's parabolas as a stack of packed 32-bit
 //           entries in LDS (pop test by an exact product comparison, no
-//           division); the chunk envelopes of a line are merged in log2(T)
-//           rounds; every surviving entry is written at the first position
-//           where it is the minimum, and each position takes the last entry
-//           written at or before it.  Constant lines are returned as they are.
+//           division).  With every axis <= 1024 (k_edt_cand) each thread then
+//           finds a minimiser at its chunk's first position by a pruned
+//           search over the chunk envelopes and walks only the envelopes that
+//           hold the minimisers of its positions; with a longer axis
+//           (k_edt_lines) the chunk envelopes of a line are merged in log2(T)
+//           rounds and walked.  Constant lines are returned as they are.
 //   signed  FE/edt_persistent_pair_v1/edt.cu:36-42 semantics with IEEE
 //           correctly rounded f64 sqrt/mul and f32 sub/mul.
 //
@@ -347,10 +349,109 @@ __device__ __forceinline__ bool keep_mid(int sa, int32_t Fa, int sb, int32_t Fb,
     return above<SMALL>(Fc - Fb, sb - sa, Fb - Fa, sc - sb);
 }
 
-// One exact lower-envelope pass over W lines x L positions.  Thread (line, c)
-// owns chunk c (C consecutive positions) of one line; the W lines of a tile are
-// adjacent along z, so every load and store of a warp-wide step touches W
-// consecutive int32 words per position.
+// Loads positions [p0, p1) of one line into v[0 .. p1-p0) and the sentinel
+// into the rest of v (such a site never enters a hull).  ZSUM: the y pass
+// over z summaries (k_edt_zsum), rebuilding the squared z distance of (x =
+// outer, y = p, z) from the two words of its 16-voxel z tile; otherwise an
+// int32 plane read at `ps` with `axis_stride` between positions.
+template <int CMAX, bool ZSUM>
+__device__ __forceinline__ void load_chunk(int32_t (&v)[CMAX], const int32_t* __restrict__ src, const EdtVol& V,
+                                           int plane, int outer, int z, int L, int p0, int p1, int64_t lbase,
+                                           int64_t axis_stride)
+{
+    if constexpr (ZSUM) {
+        const int nz = V.nz, j = z & 15;
+        const int32_t sentinel = V.sentinel;
+        const int tz = (nz + 15) >> 4;
+        const int64_t ys = (L + 3) & ~3;
+        const uint32_t* pa = reinterpret_cast<const uint32_t*>(src) + V.data + (int64_t)plane * V.plane +
+                             ((int64_t)outer * tz + (z >> 4)) * ys + p0;
+        const int64_t bofs = (int64_t)V.nx * tz * ys;
+        const uint32_t lowm = (2u << j) - 1u;
+        auto expand = [&](uint32_t a, uint32_t b) {
+            const uint32_t bits = a & 0xffffu;
+            const uint32_t lo = bits & lowm, hi = bits >> j;
+            const int left = lo ? j - 31 + __builtin_clz(lo) : j + (int)(a >> 16);
+            const int right = hi ? __builtin_ctz(hi) : 15 - j + (int)b;
+            const int d = min(left, right);
+            return min((int32_t)__umul24((uint32_t)d, (uint32_t)d), sentinel);
+        };
+        // 16-byte loads when both word runs start 16-byte aligned and the
+        // chunk length is a multiple of 4 (CMAX is one in every tile shape)
+        static_assert(CMAX % 4 == 0, "tile shapes hold whole 4-word groups");
+        if (p1 - p0 == CMAX && ((reinterpret_cast<uintptr_t>(pa) & 15) | (bofs & 3)) == 0) {
+#pragma unroll
+            for (int i = 0; i < CMAX; i += 4) {  // a whole chunk: no per-position test
+                const uint4 a = *reinterpret_cast<const uint4*>(pa + i);
+                const uint4 b = *reinterpret_cast<const uint4*>(pa + bofs + i);
+                v[i] = expand(a.x, b.x);
+                v[i + 1] = expand(a.y, b.y);
+                v[i + 2] = expand(a.z, b.z);
+                v[i + 3] = expand(a.w, b.w);
+            }
+        } else if (((reinterpret_cast<uintptr_t>(pa) & 15) | ((bofs | (p1 - p0)) & 3)) == 0) {
+#pragma unroll
+            for (int i = 0; i < CMAX; i += 4) {
+                if (p0 + i < p1) {
+                    const uint4 a = *reinterpret_cast<const uint4*>(pa + i);
+                    const uint4 b = *reinterpret_cast<const uint4*>(pa + bofs + i);
+                    v[i] = expand(a.x, b.x);
+                    v[i + 1] = expand(a.y, b.y);
+                    v[i + 2] = expand(a.z, b.z);
+                    v[i + 3] = expand(a.w, b.w);
+                } else {
+                    v[i] = v[i + 1] = v[i + 2] = v[i + 3] = sentinel;
+                }
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < CMAX; ++i) v[i] = p0 + i < p1 ? expand(pa[i], pa[bofs + i]) : sentinel;
+        }
+    } else {
+        const int32_t* ps = src + lbase + (int64_t)p0 * axis_stride;
+        if (p1 - p0 == CMAX) {  // a whole chunk: no per-position test
+#pragma unroll
+            for (int i = 0; i < CMAX; ++i) {
+                v[i] = *ps;
+                ps += axis_stride;
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < CMAX; ++i) {
+                v[i] = p0 + i < p1 ? *ps : V.sentinel;
+                ps += axis_stride;
+            }
+        }
+    }
+}
+
+// Streaming stores of v[0 .. p1-p0) to positions [p0, p1) of one line: a
+// pass's output is far larger than the caches.
+template <int CMAX>
+__device__ __forceinline__ void store_chunk(const int32_t (&v)[CMAX], int32_t* __restrict__ dst, int64_t lbase,
+                                            int p0, int p1, int64_t axis_stride)
+{
+    int32_t* pd = dst + lbase + (int64_t)p0 * axis_stride;
+    if (p1 - p0 == CMAX) {  // a whole chunk: no per-position test
+#pragma unroll
+        for (int i = 0; i < CMAX; ++i) {
+            __builtin_nontemporal_store(v[i], pd);
+            pd += axis_stride;
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < CMAX; ++i) {
+            if (p0 + i < p1) __builtin_nontemporal_store(v[i], pd);
+            pd += axis_stride;
+        }
+    }
+}
+
+// One exact lower-envelope pass over W lines x L positions, for launches with
+// an axis above 1024 (entries (F << 6) | offset, int64 hull tests).  Thread
+// (line, c) owns chunk c (C consecutive positions) of one line; the W lines of
+// a tile are adjacent along z, so every load and store of a warp-wide step
+// touches W consecutive int32 words per position.
 //  1. the thread loads its C values into registers and builds the lower hull
 //     of the points (s, f(s)+s^2) of its chunk (the parabola envelope) as a
 //     stack of packed entries in LDS, keeping the top two in registers;
@@ -360,20 +461,17 @@ __device__ __forceinline__ bool keep_mid(int sa, int32_t Fa, int sb, int32_t Fb,
 //     found (merging lower hulls of x-separated point sets), with the entries
 //     next to the bridge cached in registers;
 //  3. every thread walks the merged envelope over its positions (the minimum
-//     is unimodal along the envelope and its site moves right with p) and
-//     stores its results from registers.  When every axis is <= 1024 the
-//     surviving entries of a line are first compacted into one ascending
-//     array, so the walk is a plain index increment with the next entry
-//     cached; otherwise it steps through the chunk stacks.
+//     is unimodal along the envelope and its site moves right with p),
+//     stepping through the chunk stacks, and stores its results from
+//     registers.
 // A line whose values are all equal is its own result (g = f).  Sites holding
 // the sentinel (no site reached yet) are left out of the hulls: such a site
 // contributes sentinel + (p-s)^2 >= sentinel, with equality only at its own
 // position, so the result is min(envelope of the other sites, sentinel).
 // LDS: kmax*W*T packed entries (kmax = the largest chunk length of the launch)
-// plus four bytes per thread, eight bytes per line and (COMPACT) a start bit
-// per position of the longest line.  ZSUM: the y pass
+// plus four bytes per thread and eight bytes per line.  ZSUM: the y pass
 // reading z summaries (k_edt_zsum) instead of int32 planes.
-template <int W, int T, int CMAX, bool COMPACT, bool ZSUM>
+template <int W, int T, int CMAX, bool ZSUM>
 __global__ __launch_bounds__(W* T) void k_edt_lines(const int32_t* __restrict__ src, int32_t* __restrict__ dst,
                                                     const EdtVol* __restrict__ table, int nvol, EdtVol one, int axis,
                                                     uint32_t tiles, uint32_t grid, int region)
@@ -410,14 +508,10 @@ __global__ __launch_bounds__(W* T) void k_edt_lines(const int32_t* __restrict__ 
     uint8_t* rsb = lsb + NT;                                  // [NT] nearest nonempty chunk >= c
     int32_t* lval = reinterpret_cast<int32_t*>(rsb + NT);     // [W] first value of the line
     int32_t* lflat = lval + W;                                // [W] line is constant
-    uint32_t* smask = reinterpret_cast<uint32_t*>(lflat + W);  // [W][MW] start positions (COMPACT)
 
     const int tid = threadIdx.x;
     const int line = tid % W, c = tid / W;
     const int C = (L + T - 1) / T;
-    const int MW = (L + 31) >> 5;  // mask words per line
-    if constexpr (COMPACT)
-        for (int i = tid; i < W * MW; i += NT) smask[i] = 0u;  // published by the barriers below
     const int p0 = min(c * C, L), p1 = min(p0 + C, L);
     const bool active = line < nlines;
     // slot of chunk b of this line: rotated within the W-slot row so that the
@@ -425,66 +519,14 @@ __global__ __launch_bounds__(W* T) void k_edt_lines(const int32_t* __restrict__ 
     auto slot = [&](int b) { return b * W + (line ^ (b & (W - 1))); };
     auto entry = [&](int k, int b, int& s, int32_t& F) {
         const uint32_t e = stk[k * NT + slot(b)];
-        if constexpr (COMPACT) {
-            s = (int)(e & 1023u);
-            F = (int32_t)(e >> 10);
-        } else {
-            s = b * C + (int)(e & 63u);
-            F = (int32_t)(e >> 6);
-        }
+        s = b * C + (int)(e & 63u);
+        F = (int32_t)(e >> 6);
     };
     const int me = slot(c);
     const int64_t lbase = base + line;
 
     int32_t v[CMAX];
-    if constexpr (ZSUM) {
-        // y pass over z summaries: rebuild the squared z distance of (x = outer,
-        // y = p, z) from the two words of its 16-voxel z tile
-        if (active) {
-            const int z = zt * W + line, j = z & 15;
-            const int tz = (nz + 15) >> 4;
-            const int64_t ys = (L + 3) & ~3;
-            const uint32_t* pa = reinterpret_cast<const uint32_t*>(src) + V.data + (int64_t)plane * V.plane +
-                                 ((int64_t)outer * tz + (z >> 4)) * ys + p0;
-            const int64_t bofs = (int64_t)V.nx * tz * ys;
-            const uint32_t lowm = (2u << j) - 1u;
-            auto expand = [&](uint32_t a, uint32_t b) {
-                const uint32_t bits = a & 0xffffu;
-                const uint32_t lo = bits & lowm, hi = bits >> j;
-                const int left = lo ? j - 31 + __builtin_clz(lo) : j + (int)(a >> 16);
-                const int right = hi ? __builtin_ctz(hi) : 15 - j + (int)b;
-                const int d = min(left, right);
-                return min((int32_t)__umul24((uint32_t)d, (uint32_t)d), sentinel);
-            };
-            // 16-byte loads when both word runs start 16-byte aligned and the
-            // chunk length is a multiple of 4 (CMAX is one in every tile shape)
-            static_assert(CMAX % 4 == 0, "tile shapes hold whole 4-word groups");
-            if (((reinterpret_cast<uintptr_t>(pa) & 15) | ((bofs | (p1 - p0)) & 3)) == 0) {
-#pragma unroll
-                for (int i = 0; i < CMAX; i += 4) {
-                    if (p0 + i < p1) {
-                        const uint4 a = *reinterpret_cast<const uint4*>(pa + i);
-                        const uint4 b = *reinterpret_cast<const uint4*>(pa + bofs + i);
-                        v[i] = expand(a.x, b.x);
-                        v[i + 1] = expand(a.y, b.y);
-                        v[i + 2] = expand(a.z, b.z);
-                        v[i + 3] = expand(a.w, b.w);
-                    }
-                }
-            } else {
-#pragma unroll
-                for (int i = 0; i < CMAX; ++i)
-                    if (p0 + i < p1) v[i] = expand(pa[i], pa[bofs + i]);
-            }
-        }
-    } else if (active) {
-        const int32_t* ps = src + lbase + (int64_t)p0 * axis_stride;
-#pragma unroll
-        for (int i = 0; i < CMAX; ++i) {
-            if (p0 + i < p1) v[i] = *ps;
-            ps += axis_stride;
-        }
-    }
+    if (active) load_chunk<CMAX, ZSUM>(v, src, V, plane, outer, zt * W + line, L, p0, p1, lbase, axis_stride);
 
     // 0. constant lines (all zero, all at the sentinel, ...) are their own
     // result: found first, their threads skip the hull, merge and output.
@@ -519,7 +561,7 @@ __global__ __launch_bounds__(W* T) void k_edt_lines(const int32_t* __restrict__ 
             if (fq < sentinel) {
                 const int32_t Fq = fq + (int32_t)__umul24((uint32_t)q, (uint32_t)q);
                 while (k >= 1) {
-                    if (above<COMPACT>(Fq - F_top, s_top - s_sec, F_top - F_sec, q - s_top)) break;
+                    if (above<false>(Fq - F_top, s_top - s_sec, F_top - F_sec, q - s_top)) break;
                     --k;
                     top -= NT;
                     s_top = s_sec;
@@ -528,7 +570,7 @@ __global__ __launch_bounds__(W* T) void k_edt_lines(const int32_t* __restrict__ 
                 }
                 ++k;
                 top += NT;
-                *top = pack_entry<COMPACT>(Fq, q, i);
+                *top = pack_entry<false>(Fq, q, i);
                 s_sec = s_top;
                 F_sec = F_top;
                 s_top = q;
@@ -574,7 +616,7 @@ __global__ __launch_bounds__(W* T) void k_edt_lines(const int32_t* __restrict__ 
                             if (pa < a0) break;
                             entry(bhi[slot(pa)] - 1, pa, s2, F2);
                         }
-                        if (keep_mid<COMPACT>(s2, F2, s1, F1, s3, F3)) break;
+                        if (keep_mid<false>(s2, F2, s1, F1, s3, F3)) break;
                         if (--th == tlo) {
                             bhi[slot(ta)] = (uint8_t)th;
                             ta = pa;
@@ -594,7 +636,7 @@ __global__ __launch_bounds__(W* T) void k_edt_lines(const int32_t* __restrict__ 
                             if (nb >= a2) break;
                             entry(blo[slot(nb)], nb, s4, F4);
                         }
-                        if (keep_mid<COMPACT>(s1, F1, s3, F3, s4, F4)) break;
+                        if (keep_mid<false>(s1, F1, s3, F3, s4, F4)) break;
                         changed = true;
                         if (++hl == hhi) {
                             blo[slot(hb)] = (uint8_t)hl;
@@ -613,175 +655,385 @@ __global__ __launch_bounds__(W* T) void k_edt_lines(const int32_t* __restrict__ 
         if (4 * h <= CPW) wave_sync();  // the next round (2h) stays within the wave
         else __syncthreads();
     }
-    if constexpr (COMPACT) {
-        // 3. Each thread computes, for the surviving entries of its chunk, the
-        // first integer position where the entry is the envelope minimum (one
-        // exact floor division with the neighbouring envelope entry), writes
-        // the entry there and marks the position in the line's start mask;
-        // every position then takes the entry at the last start at or before
-        // it.  No data-dependent loops over positions.
-        const int lo = blo[me], cnt = bhi[me] - lo;
-        uint32_t pred = 0xffffffffu, succ = 0xffffffffu;
-        if (busy) {
+    {
+        int b = c;
+        while (b >= 0 && bhi[slot(b)] == blo[slot(b)]) --b;
+        lsb[me] = (uint8_t)(b < 0 ? 255 : b);
+        b = c;
+        while (b < T && bhi[slot(b)] == blo[slot(b)]) ++b;
+        rsb[me] = (uint8_t)(b >= T ? 255 : b);
+    }
+    __syncthreads();
+
+    // 3. walk the merged envelope; results replace v
+    if (active && p0 < p1 && !lflat[line]) {
+        int cb = bhi[me] > blo[me] ? c : (lsb[me] != 255 ? lsb[me] : rsb[me]);
+        if (cb == 255) {
 #pragma unroll
-            for (int i = 0; i < CMAX; ++i)
-                if (i < cnt) v[i] = (int32_t)stk[(lo + i) * NT + me];
-            int b = c - 1;
-            while (b >= 0 && bhi[slot(b)] == blo[slot(b)]) --b;
-            if (b >= 0) pred = stk[(bhi[slot(b)] - 1) * NT + slot(b)];
-            b = c + 1;
-            while (b < T && bhi[slot(b)] == blo[slot(b)]) ++b;
-            if (b < T) succ = stk[blo[slot(b)] * NT + slot(b)];
-        }
-        __syncthreads();  // the hull stacks are free from here on
-        // at[p*W + line]: entry whose interval starts at position p; the
-        // line's start positions are bits of smask (zeroed at kernel start)
-        uint32_t* at = stk;
-        // first integer p where entry b is strictly below entry a (a left of b):
-        // floor((Fb - Fa) / (2 (sb - sa))) + 1, exact: |Fb - Fa| < 2^22 and the
-        // reciprocal estimate is within 0.4 of the quotient, so one integer
-        // correction step gives the floor.
-        auto start = [](uint32_t a, uint32_t b) {
-            const int32_t n = (int32_t)(b >> 10) - (int32_t)(a >> 10);
-            const int32_t d = 2 * ((int32_t)(b & 1023u) - (int32_t)(a & 1023u));
-            int32_t q = (int32_t)floorf((float)n * __builtin_amdgcn_rcpf((float)d));
-            const int32_t r = n - q * d;
-            q += r < 0 ? -1 : (r >= d ? 1 : 0);
-            return q + 1;
-        };
-        uint32_t* lmask = smask + line * MW;
-        if (busy && cnt > 0) {
-            int xlo = pred == 0xffffffffu ? 0 : min(max(start(pred, (uint32_t)v[0]), 0), L);
-#pragma unroll
-            for (int i = 0; i < CMAX; ++i)
-                if (i < cnt) {
-                    const uint32_t e = (uint32_t)v[i];
-                    const uint32_t nx = i + 1 < cnt ? (uint32_t)v[min(i + 1, CMAX - 1)] : succ;
-                    const int xhi = nx == 0xffffffffu ? L : min(max(start(e, nx), 0), L);
-                    if (xlo < xhi) {
-                        at[xlo * W + line] = e;
-                        atomicOr(lmask + (xlo >> 5), 1u << (xlo & 31));
+            for (int i = 0; i < CMAX; ++i) v[i] = sentinel;
+        } else {
+            int clo = blo[slot(cb)], chi = bhi[slot(cb)];
+            int ci = cb >= c ? clo : chi - 1;
+            int cs;
+            int32_t cF;
+            entry(ci, cb, cs, cF);
+            // next entry to the right; false at the end of the envelope
+            auto next = [&](int& nb, int& ni, int& nlo, int& nhi) {
+                nb = cb;
+                ni = ci + 1;
+                nlo = clo;
+                nhi = chi;
+                if (ni < chi) return true;
+                if (cb + 1 >= T) return false;
+                nb = rsb[slot(cb + 1)];
+                if (nb == 255) return false;
+                nlo = blo[slot(nb)];
+                nhi = bhi[slot(nb)];
+                ni = nlo;
+                return true;
+            };
+            bool moved = false;
+            for (;;) {  // at p0: right while not worse
+                int nb, ni, nlo, nhi, ns;
+                int32_t nF;
+                if (!next(nb, ni, nlo, nhi)) break;
+                entry(ni, nb, ns, nF);
+                if (nF - 2 * p0 * ns > cF - 2 * p0 * cs) break;
+                cb = nb; ci = ni; clo = nlo; chi = nhi; cs = ns; cF = nF;
+                moved = true;
+            }
+            if (!moved) {
+                for (;;) {  // left while strictly better
+                    int pb = cb, pi = ci - 1, plo = clo, phi = chi;
+                    if (pi < clo) {
+                        if (cb == 0) break;
+                        pb = lsb[slot(cb - 1)];
+                        if (pb == 255) break;
+                        plo = blo[slot(pb)];
+                        phi = bhi[slot(pb)];
+                        pi = phi - 1;
                     }
-                    xlo = max(xlo, xhi);
+                    int ps;
+                    int32_t pF;
+                    entry(pi, pb, ps, pF);
+                    if (pF - 2 * p0 * ps >= cF - 2 * p0 * cs) break;
+                    cb = pb; ci = pi; clo = plo; chi = phi; cs = ps; cF = pF;
                 }
-        }
-        __syncthreads();
-        if (busy && p0 < p1) {
-            // the entry in force at p0 is the one starting at the last start
-            // at or before p0 (a busy line's envelope is nonempty and its first
-            // entry starts at 0); later starts within the chunk come from a
-            // 64-bit window of the mask (chunks hold <= 32 positions)
-            int w = p0 >> 5;
-            uint32_t bits = lmask[w] & ((2u << (p0 & 31)) - 1u);
-            while (!bits && w > 0) bits = lmask[--w];
-            uint32_t e = bits ? at[((w << 5) + 31 - __builtin_clz(bits)) * W + line] : 0xffffffffu;
-            const int w0 = p0 >> 5;
-            const uint64_t win = (uint64_t)lmask[w0] | ((uint64_t)(w0 + 1 < MW ? lmask[w0 + 1] : 0u) << 32);
+            }
 #pragma unroll
             for (int i = 0; i < CMAX; ++i) {
                 const int p = p0 + i;
                 if (p < p1) {
-                    if (i > 0 && ((win >> (p - (w0 << 5))) & 1ull)) e = at[p * W + line];
-                    const int es = (int)(e & 1023u);
-                    const int32_t eF = (int32_t)(e >> 10);
-                    const int32_t g = eF - (int32_t)__umul24(2u * p, es) + (int32_t)__umul24(p, p);
-                    v[i] = e == 0xffffffffu ? sentinel : min(g, sentinel);
-                }
-            }
-        }
-    } else {
-        {
-            int b = c;
-            while (b >= 0 && bhi[slot(b)] == blo[slot(b)]) --b;
-            lsb[me] = (uint8_t)(b < 0 ? 255 : b);
-            b = c;
-            while (b < T && bhi[slot(b)] == blo[slot(b)]) ++b;
-            rsb[me] = (uint8_t)(b >= T ? 255 : b);
-        }
-        __syncthreads();
-
-        // 3. walk the merged envelope; results replace v
-        if (active && p0 < p1 && !lflat[line]) {
-            int cb = bhi[me] > blo[me] ? c : (lsb[me] != 255 ? lsb[me] : rsb[me]);
-            if (cb == 255) {
-    #pragma unroll
-                for (int i = 0; i < CMAX; ++i) v[i] = sentinel;
-            } else {
-                int clo = blo[slot(cb)], chi = bhi[slot(cb)];
-                int ci = cb >= c ? clo : chi - 1;
-                int cs;
-                int32_t cF;
-                entry(ci, cb, cs, cF);
-                // next entry to the right; false at the end of the envelope
-                auto next = [&](int& nb, int& ni, int& nlo, int& nhi) {
-                    nb = cb;
-                    ni = ci + 1;
-                    nlo = clo;
-                    nhi = chi;
-                    if (ni < chi) return true;
-                    if (cb + 1 >= T) return false;
-                    nb = rsb[slot(cb + 1)];
-                    if (nb == 255) return false;
-                    nlo = blo[slot(nb)];
-                    nhi = bhi[slot(nb)];
-                    ni = nlo;
-                    return true;
-                };
-                bool moved = false;
-                for (;;) {  // at p0: right while not worse
-                    int nb, ni, nlo, nhi, ns;
-                    int32_t nF;
-                    if (!next(nb, ni, nlo, nhi)) break;
-                    entry(ni, nb, ns, nF);
-                    if (nF - 2 * p0 * ns > cF - 2 * p0 * cs) break;
-                    cb = nb; ci = ni; clo = nlo; chi = nhi; cs = ns; cF = nF;
-                    moved = true;
-                }
-                if (!moved) {
-                    for (;;) {  // left while strictly better
-                        int pb = cb, pi = ci - 1, plo = clo, phi = chi;
-                        if (pi < clo) {
-                            if (cb == 0) break;
-                            pb = lsb[slot(cb - 1)];
-                            if (pb == 255) break;
-                            plo = blo[slot(pb)];
-                            phi = bhi[slot(pb)];
-                            pi = phi - 1;
-                        }
-                        int ps;
-                        int32_t pF;
-                        entry(pi, pb, ps, pF);
-                        if (pF - 2 * p0 * ps >= cF - 2 * p0 * cs) break;
-                        cb = pb; ci = pi; clo = plo; chi = phi; cs = ps; cF = pF;
+                    for (;;) {
+                        int nb, ni, nlo, nhi, ns;
+                        int32_t nF;
+                        if (!next(nb, ni, nlo, nhi)) break;
+                        entry(ni, nb, ns, nF);
+                        if (nF - 2 * p * ns > cF - 2 * p * cs) break;
+                        cb = nb; ci = ni; clo = nlo; chi = nhi; cs = ns; cF = nF;
                     }
-                }
-    #pragma unroll
-                for (int i = 0; i < CMAX; ++i) {
-                    const int p = p0 + i;
-                    if (p < p1) {
-                        for (;;) {
-                            int nb, ni, nlo, nhi, ns;
-                            int32_t nF;
-                            if (!next(nb, ni, nlo, nhi)) break;
-                            entry(ni, nb, ns, nF);
-                            if (nF - 2 * p * ns > cF - 2 * p * cs) break;
-                            cb = nb; ci = ni; clo = nlo; chi = nhi; cs = ns; cF = nF;
-                        }
-                        v[i] = min(cF - 2 * p * cs + p * p, sentinel);
-                    }
+                    v[i] = min(cF - 2 * p * cs + p * p, sentinel);
                 }
             }
         }
     }
-    if (active) {
-        // streaming stores: a pass's output is far larger than the caches
-        int32_t* pd = dst + lbase + (int64_t)p0 * axis_stride;
+    if (active) store_chunk<CMAX>(v, dst, lbase, p0, p1, axis_stride);
+}
+
+// The lower-envelope pass without a merge of the chunk hulls (every axis <=
+// 1024, so entries are (F << 10) | s).  Tiles, chunks and hulls are those of
+// k_edt_lines.  With h(p, s) = f(s) + (p - s)^2:
+//  0. the owner of each chunk hull also records the hull's minimum at its
+//     chunk's first position p_b and at the next chunk's first position
+//     p_{b+1} (along a hull the values at one position fall and then rise, so
+//     short scans from its two ends find them), and each line keeps a bit
+//     mask of its nonempty chunks;
+//  1. each thread finds one minimiser a_c of its line at p_c = c*C, visiting
+//     the nonempty chunks outward from its own.  A chunk b > c cannot beat
+//     its minimum at p_b plus (p_b - p_c)^2 at p_c, a chunk b < c its minimum
+//     at p_{b+1} plus (p_c - p_{b+1})^2 (for s >= p_b >= p,
+//     (s - p)^2 >= (s - p_b)^2 + (p_b - p)^2), and no chunk at distance >= d
+//     the line's smallest f plus d^2, which ends the walk outward; a visited
+//     chunk's minimum at p_c is found by walking its hull from the minimum
+//     at its boundary nearest p_c;
+//  2. the positions p_c <= p < p_{c+1} of the chunk take the minimum over the
+//     sites from a_c to a_{c+1}: for s < a_c and p > p_c,
+//     h(p, s) - h(p, a_c) = [h(p_c, s) - h(p_c, a_c)] + 2 (p - p_c)(a_c - s) > 0,
+//     and likewise every site beyond a_{c+1} loses to it before p_{c+1}, so
+//     every minimiser at p lies in [a_c, a_{c+1}].  The nonempty chunks
+//     holding those sites (the first from a_c's entry on, the last up to
+//     a_{c+1}'s) are each walked left to right along the positions.
+// Four block barriers in all, none in data-dependent rounds.  Constant lines
+// are their own result and sentinel sites stay out of the hulls, as in
+// k_edt_lines.  LDS: kmax*W*T entries, 12 bytes per thread, 12 per line.
+template <int W, int T, int CMAX, bool ZSUM>
+__global__ __launch_bounds__(W* T, CMAX <= 16 ? 8 : (CMAX <= 24 ? 6 : 4)) void k_edt_cand(
+    const int32_t* __restrict__ src, int32_t* __restrict__ dst, const EdtVol* __restrict__ table, int nvol,
+    EdtVol one, int axis, uint32_t tiles, uint32_t grid, int region)
+{
+    extern __shared__ uint32_t lds[];
+    constexpr int NT = W * T;
+    static_assert((W & (W - 1)) == 0 && CMAX <= 32 && T <= 32, "tile shape");
+    const uint32_t tile = xcd_tile(blockIdx.x, grid);
+    if (tile >= tiles) return;  // uniform per block
+    const int pass = axis == 1 ? 1 : 2;
+    const EdtVol V = load_vol(table, nvol, one, pass, tile);
+    const uint32_t local = tile - V.first[pass];
+    const int nz = V.nz;
+    const int32_t sentinel = V.sentinel;
+    const int64_t yz = (int64_t)V.ny * nz;
+    const int L = axis == 1 ? V.ny : V.nx;
+    const int64_t axis_stride = axis == 1 ? (int64_t)nz : yz;
+    const int64_t outer_stride = axis == 1 ? yz : (int64_t)nz;
+    // tile numbering as in k_edt_lines: z tile fastest, then plane, then outer
+    const int ztiles = (nz + W - 1) / W;
+    const int zt = local % ztiles;
+    const int rest = local / ztiles;
+    const int nplanes = V.planes > 1 ? V.planes : 1;
+    const int outer = rest / nplanes, plane = rest % nplanes;
+    const int64_t base = V.data + (int64_t)plane * V.plane + (int64_t)outer * outer_stride + (int64_t)zt * W;
+    const int nlines = min(W, nz - zt * W);
+
+    uint32_t* stk = lds;                                       // [kmax][NT] packed hull entries (region words)
+    int32_t* lv = reinterpret_cast<int32_t*>(stk + region);   // [NT] hull minimum at p_b
+    int32_t* rv = lv + NT;                                     // [NT] hull minimum at p_{b+1}
+    int32_t* lmin = rv + NT;                                   // [W] smallest value of the line
+    int32_t* lmax = lmin + W;                                  // [W] largest value of the line
+    uint32_t* lmask = reinterpret_cast<uint32_t*>(lmax + W);  // [W] nonempty chunks of the line
+    uint16_t* info = reinterpret_cast<uint16_t*>(lmask + W);  // [NT] size | argmin at p_b << 6 | at p_{b+1} << 11
+    uint16_t* bnd = info + NT;                                 // [NT] minimiser at p_c: chunk << 8 | entry
+
+    const int tid = threadIdx.x;
+    const int line = tid % W, c = tid / W;
+    const int C = (L + T - 1) / T;
+    const int p0 = min(c * C, L), p1 = min(p0 + C, L), np = p1 - p0;
+    const bool active = line < nlines;
+    // slot of chunk b of this line, rotated within the W-slot row (banks)
+    auto slot = [&](int b) { return b * W + (line ^ (b & (W - 1))); };
+    const int me = slot(c);
+    const int64_t lbase = base + line;
+    if (c == 0) {
+        lmin[line] = INT_MAX;
+        lmax[line] = INT_MIN;
+        lmask[line] = 0u;
+    }
+
+    int32_t v[CMAX];
+    if (active) load_chunk<CMAX, ZSUM>(v, src, V, plane, outer, zt * W + line, L, p0, p1, lbase, axis_stride);
+    int32_t mn = INT_MAX, mx = INT_MIN;
+    if (active && np == CMAX) {
 #pragma unroll
         for (int i = 0; i < CMAX; ++i) {
-            if (p0 + i < p1) __builtin_nontemporal_store(v[i], pd);
-            pd += axis_stride;
+            mn = min(mn, v[i]);
+            mx = max(mx, v[i]);
+        }
+    } else if (active) {
+#pragma unroll
+        for (int i = 0; i < CMAX; ++i)
+            if (i < np) {
+                mn = min(mn, v[i]);
+                mx = max(mx, v[i]);
+            }
+    }
+    __syncthreads();  // the line records are initialised
+    if (active && np > 0) {
+        atomicMin(lmin + line, mn);
+        atomicMax(lmax + line, mx);
+    }
+    __syncthreads();
+    const int32_t fline = lmin[line];
+    const bool busy = active && fline != lmax[line];
+
+    // value of entry e at position p, less p^2
+    auto at = [](uint32_t e, int p) { return (int32_t)(e >> 10) - (int32_t)__umul24(2u * (uint32_t)p, e & 1023u); };
+    // 0. hull of this chunk and its minima at p_c and p_{c+1}
+    if (busy) {
+        int k = -1, s_top = 0, s_sec = 0;
+        int32_t F_top = 0, F_sec = 0;
+        uint32_t* top = stk + me - NT;  // stack slot k lives at stk[k * NT + me]
+#pragma unroll
+        for (int i = 0; i < CMAX; ++i) {
+            const int q = p0 + i;
+            const int32_t fq = v[i];  // the sentinel past the chunk end
+            if (fq < sentinel) {
+                const int32_t Fq = fq + (int32_t)__umul24((uint32_t)q, (uint32_t)q);
+                if (k >= 1 && !above<true>(Fq - F_top, s_top - s_sec, F_top - F_sec, q - s_top)) {
+                    do {  // pops are rare: the loop stays off the common path
+                        --k;
+                        top -= NT;
+                        s_top = s_sec;
+                        F_top = F_sec;
+                        if (k >= 1) {
+                            const uint32_t e = stk[(k - 1) * NT + me];
+                            s_sec = (int)(e & 1023u);
+                            F_sec = (int32_t)(e >> 10);
+                        }
+                    } while (k >= 1 && !above<true>(Fq - F_top, s_top - s_sec, F_top - F_sec, q - s_top));
+                }
+                ++k;
+                top += NT;
+                *top = pack_entry<true>(Fq, q, i);
+                s_sec = s_top;
+                F_sec = F_top;
+                s_top = q;
+                F_top = Fq;
+            }
+        }
+        const int m = k + 1;
+        uint32_t inf = (uint32_t)m;
+        if (m > 0) {
+            int li = 0;  // minimum at p_c: from the first entry on
+            int32_t cur = at(stk[me], p0);
+            while (li + 1 < m) {
+                const int32_t nxt = at(stk[(li + 1) * NT + me], p0);
+                if (nxt >= cur) break;
+                ++li;
+                cur = nxt;
+            }
+            lv[me] = cur + p0 * p0;
+            int ri = k;  // minimum at p_{c+1}: from the last entry back (the top two are in registers)
+            cur = F_top - (int32_t)__umul24(2u * (uint32_t)p1, (uint32_t)s_top);
+            if (k >= 1) {
+                const int32_t sec = F_sec - (int32_t)__umul24(2u * (uint32_t)p1, (uint32_t)s_sec);
+                if (sec <= cur) {
+                    ri = k - 1;
+                    cur = sec;
+                    while (ri > 0) {
+                        const int32_t nxt = at(stk[(ri - 1) * NT + me], p1);
+                        if (nxt > cur) break;
+                        --ri;
+                        cur = nxt;
+                    }
+                }
+            }
+            rv[me] = cur + p1 * p1;
+            inf |= (uint32_t)li << 6 | (uint32_t)ri << 11;
+            atomicOr(lmask + line, 1u << c);
+        }
+        info[me] = (uint16_t)inf;
+    }
+    __syncthreads();  // hulls, their minima and the masks published
+
+    const uint32_t mask = busy ? lmask[line] : 0u;
+    // 1. a minimiser at p_c
+    if (busy && np > 0) {
+        const int p = p0, pp = p * p;
+        int32_t best = INT_MAX;  // h(p_c, best site)
+        uint32_t sel = 0;
+        if (mask >> c & 1u) {
+            best = lv[me];
+            sel = (uint32_t)c << 8 | (uint32_t)(info[me] >> 6 & 31u);
+        }
+        for (uint32_t mr = c + 1 < 32 ? mask >> (c + 1) << (c + 1) : 0u; mr; mr &= mr - 1u) {
+            const int b = __builtin_ctz(mr);  // nonempty chunks to the right, nearest first
+            const int d = b * C - p;          // distance to the chunk's first position
+            if (fline + d * d >= best) break;
+            const int sb = slot(b);
+            if (lv[sb] + d * d >= best) continue;
+            int i = info[sb] >> 6 & 31;  // from the minimum at p_b leftwards
+            int32_t cur = at(stk[i * NT + sb], p);
+            while (i > 0) {
+                const int32_t nxt = at(stk[(i - 1) * NT + sb], p);
+                if (nxt > cur) break;
+                --i;
+                cur = nxt;
+            }
+            if (cur + pp < best) {
+                best = cur + pp;
+                sel = (uint32_t)b << 8 | (uint32_t)i;
+            }
+        }
+        for (uint32_t ml = mask & ((1u << c) - 1u); ml; ml ^= 1u << (31 - __builtin_clz(ml))) {
+            const int b = 31 - __builtin_clz(ml);  // nonempty chunks to the left, nearest first
+            const int d = p - (b * C + C - 1);     // distance to the chunk's last position
+            if (fline + d * d >= best) break;
+            const int sb = slot(b);
+            const int e = p - (b + 1) * C;  // distance to p_{b+1}
+            if (rv[sb] + e * e >= best) continue;
+            const uint32_t inf = info[sb];
+            const int m = (int)(inf & 63u);
+            int i = (int)(inf >> 11 & 31u);  // from the minimum at p_{b+1} rightwards
+            int32_t cur = rv[sb] - pp;       // exact when p_{b+1} = p_c
+            if (e > 0) {
+                cur = at(stk[i * NT + sb], p);
+                while (i + 1 < m) {
+                    const int32_t nxt = at(stk[(i + 1) * NT + sb], p);
+                    if (nxt >= cur) break;
+                    ++i;
+                    cur = nxt;
+                }
+            }
+            if (cur + pp < best) {
+                best = cur + pp;
+                sel = (uint32_t)b << 8 | (uint32_t)i;
+            }
+        }
+        bnd[me] = (uint16_t)sel;
+    }
+    __syncthreads();  // minimisers published
+
+    // 2. the chunk's positions over the hulls holding the sites a_c .. a_{c+1}
+    // (a constant line's value is its minimum, so no loaded value stays live)
+#pragma unroll
+    for (int j = 0; j < CMAX; ++j) v[j] = busy ? sentinel : fline;
+    if (busy && np > 0) {
+        const uint32_t s0 = bnd[me];
+        const int b0 = (int)(s0 >> 8), i0 = (int)(s0 & 255u);
+        int b1 = 31 - __builtin_clz(mask), i1 = 255;  // the last chunk of the line: every site from a_c on
+        if (p1 < L) {
+            const uint32_t s1 = bnd[slot(c + 1)];
+            b1 = (int)(s1 >> 8);
+            i1 = (int)(s1 & 255u);
+        }
+        uint32_t cm = mask & (b1 >= 31 ? ~0u : (2u << b1) - 1u) & ~((1u << b0) - 1u);
+        for (; cm; cm &= cm - 1u) {
+            const int b = __builtin_ctz(cm);
+            const int sb = slot(b);
+            const uint32_t inf = info[sb];
+            const int hi = min(b == b1 ? i1 : 31, (int)(inf & 63u) - 1);
+            // first entry: a_c's, else the chunk's minimum at the boundary
+            // nearest p_c, else (chunks to the right) its first
+            const int i_start = b == b0 ? i0 : (b < c ? (int)(inf >> 11 & 31u) : (b == c ? (int)(inf >> 6 & 31u) : 0));
+            int i = i_start;
+            const uint32_t e = stk[i * NT + sb];
+            int s2 = (int)(e & 1023u) << 1;
+            int32_t cur = at(e, p0);
+            // the next entry, or none: a value that no position reaches
+            // (stays put, as its 2s is 0)
+            int sn2 = 0;
+            int32_t nxt = INT_MAX;
+            if (i < hi) {
+                const uint32_t en = stk[(i + 1) * NT + sb];
+                sn2 = (int)(en & 1023u) << 1;
+                nxt = at(en, p0);
+            }
+            // 2p and p^2 as recurrences: kept opaque so that the compiler does
+            // not hoist CMAX precomputed copies out of the chunk loop
+            int p2 = 2 * p0, pp = p0 * p0;
+            asm volatile("" : "+v"(p2), "+v"(pp));
+            // every position of v, also past the chunk end (never stored)
+#pragma unroll
+            for (int j = 0; j < CMAX; ++j) {
+                while (nxt <= cur) {  // the minimum moves right with p
+                    s2 = sn2;
+                    cur = nxt;
+                    sn2 = 0;
+                    nxt = INT_MAX;
+                    if (++i < hi) {
+                        const uint32_t e2 = stk[(i + 1) * NT + sb];
+                        sn2 = (int)(e2 & 1023u) << 1;
+                        nxt = (int32_t)(e2 >> 10) - (int32_t)__umul24((uint32_t)p2, e2 & 1023u);
+                    }
+                }
+                v[j] = min(v[j], cur + pp);
+                cur -= s2;  // values at p + 1
+                nxt -= sn2;
+                pp += p2 + 1;
+                p2 += 2;
+            }
         }
     }
+    if (active) store_chunk<CMAX>(v, dst, lbase, p0, p1, axis_stride);
 }
 
 // ---------------------------------------------------------------- signed output
@@ -800,38 +1052,63 @@ __global__ __launch_bounds__(256) void k_edt_signed(const int32_t* __restrict__ 
 
 struct LineCfg {
     int W, T, CMAX;
+    bool cand;  // k_edt_cand (every axis <= 1024), else k_edt_lines
 };
 
 // Tile shape by line length: W lines (W*4 contiguous bytes per position),
 // T threads per line, CMAX >= ceil(L/T) positions per thread (registers).
-LineCfg pick_cfg(int L)
+// compact: every axis of every volume of the launch is <= 1024.  xpass: the
+// x pass, whose positions lie a whole plane apart, takes 32 lines up to 512
+// positions, so that every access of a position is one 128-byte piece; the
+// y pass keeps 16 (its loads are contiguous summaries and its stores 16 lines
+// a row apart, where the smaller tile's occupancy pays more).
+LineCfg pick_cfg(int L, bool compact, bool xpass)
 {
-    if (L <= 128) return {32, 8, 16};
-    if (L <= 256) return {32, 16, 16};
-    if (L <= 512) return {16, 32, 16};
-    if (L <= 768) return {16, 32, 24};
-    if (L <= 1024) return {16, 32, 32};
-    if (L <= 2048) return {8, 64, 32};
-    return {8, 64, 64};
+    if (compact) {
+        const int w = xpass ? 32 : 16;
+        if (L <= 128) return {w, 8, 16, true};
+        if (L <= 256) return {w, 16, 16, true};
+        if (L <= 512) return {w, 32, 16, true};
+        if (L <= 768) return {16, 32, 24, true};
+        return {16, 32, 32, true};
+    }
+    if (L <= 2048) return {8, 64, 32, false};
+    return {8, 64, 64, false};
 }
 
-template <int W, int T, int CMAX, bool COMPACT, bool ZSUM>
+template <int W, int T, int CMAX, bool ZSUM>
 hipError_t launch_lines_t(const int32_t* src, int32_t* dst, const EdtVol* table, int nvol, const EdtVol& one,
                           int axis, int Lmax, uint64_t tiles, hipStream_t st)
 {
     const uint32_t grid = round8(tiles);
     const int kmax = (Lmax + T - 1) / T;  // <= CMAX by the choice of shape
-    // hull stacks; with COMPACT the same words later hold Lmax rows of W start entries
-    const int region = kmax * W * T;
-    const size_t lds = (size_t)region * 4 + (size_t)W * T * 4 + (size_t)W * 8 +
-                       (COMPACT ? (size_t)W * ((Lmax + 31) / 32) * 4 : 0);
-    const void* fn = reinterpret_cast<const void*>(&k_edt_lines<W, T, CMAX, COMPACT, ZSUM>);
+    const int region = kmax * W * T;      // hull stacks
+    const size_t lds = (size_t)region * 4 + (size_t)W * T * 4 + (size_t)W * 8;
+    const void* fn = reinterpret_cast<const void*>(&k_edt_lines<W, T, CMAX, ZSUM>);
     if (lds > 65536) {
         hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL((k_edt_lines<W, T, CMAX, COMPACT, ZSUM>), dim3(grid), dim3(W * T), lds, st, src, dst, table,
-                       nvol, one, axis, (uint32_t)tiles, grid, region);
+    hipLaunchKernelGGL((k_edt_lines<W, T, CMAX, ZSUM>), dim3(grid), dim3(W * T), lds, st, src, dst, table, nvol, one,
+                       axis, (uint32_t)tiles, grid, region);
+    return hipGetLastError();
+}
+
+template <int W, int T, int CMAX, bool ZSUM>
+hipError_t launch_cand_t(const int32_t* src, int32_t* dst, const EdtVol* table, int nvol, const EdtVol& one, int axis,
+                         int Lmax, uint64_t tiles, hipStream_t st)
+{
+    const uint32_t grid = round8(tiles);
+    const int kmax = (Lmax + T - 1) / T;  // <= CMAX by the choice of shape
+    const int region = kmax * W * T;      // hull stacks
+    const size_t lds = (size_t)region * 4 + (size_t)W * T * 12 + (size_t)W * 12;
+    const void* fn = reinterpret_cast<const void*>(&k_edt_cand<W, T, CMAX, ZSUM>);
+    if (lds > 65536) {
+        hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL((k_edt_cand<W, T, CMAX, ZSUM>), dim3(grid), dim3(W * T), lds, st, src, dst, table, nvol, one,
+                       axis, (uint32_t)tiles, grid, region);
     return hipGetLastError();
 }
 
@@ -842,23 +1119,25 @@ hipError_t launch_lines_t(const int32_t* src, int32_t* dst, const EdtVol* table,
 hipError_t launch_lines(const int32_t* src, int32_t* dst, const EdtVol* table, int nvol, const EdtVol& one, int axis,
                         int Lmax, uint64_t tiles, bool compact, bool zsum, hipStream_t st)
 {
-    const LineCfg c = pick_cfg(Lmax);
-#define FE_LINES_Z(w, t, cm, cp)                                                                             \
-    return zsum ? launch_lines_t<w, t, cm, cp, true>(src, dst, table, nvol, one, axis, Lmax, tiles, st)     \
-                : launch_lines_t<w, t, cm, cp, false>(src, dst, table, nvol, one, axis, Lmax, tiles, st);
-#define FE_LINES(w, t, cm)                                                                                   \
-    if (c.W == w && c.T == t && c.CMAX == cm) {                                                              \
-        if (compact) { FE_LINES_Z(w, t, cm, true) }                                                          \
-        FE_LINES_Z(w, t, cm, false)                                                                          \
-    }
-    FE_LINES(32, 8, 16)
-    FE_LINES(32, 16, 16)
-    FE_LINES(16, 32, 16)
-    FE_LINES(16, 32, 24)
-    FE_LINES(16, 32, 32)
-#undef FE_LINES
-    if (c.W == 8 && c.T == 64 && c.CMAX == 32) { FE_LINES_Z(8, 64, 32, false) }
-    if (c.W == 8 && c.T == 64 && c.CMAX == 64) { FE_LINES_Z(8, 64, 64, false) }
+    const LineCfg c = pick_cfg(Lmax, compact, axis == 0);
+#define FE_CAND(w, t, cm)                                                                                    \
+    if (c.cand && c.W == w && c.T == t && c.CMAX == cm)                                                      \
+        return zsum ? launch_cand_t<w, t, cm, true>(src, dst, table, nvol, one, axis, Lmax, tiles, st)      \
+                    : launch_cand_t<w, t, cm, false>(src, dst, table, nvol, one, axis, Lmax, tiles, st);
+    FE_CAND(32, 8, 16)
+    FE_CAND(32, 16, 16)
+    FE_CAND(32, 32, 16)
+    FE_CAND(16, 8, 16)
+    FE_CAND(16, 16, 16)
+    FE_CAND(16, 32, 16)
+    FE_CAND(16, 32, 24)
+    FE_CAND(16, 32, 32)
+#undef FE_CAND
+#define FE_LINES_Z(w, t, cm)                                                                                 \
+    return zsum ? launch_lines_t<w, t, cm, true>(src, dst, table, nvol, one, axis, Lmax, tiles, st)         \
+                : launch_lines_t<w, t, cm, false>(src, dst, table, nvol, one, axis, Lmax, tiles, st);
+    if (!c.cand && c.W == 8 && c.T == 64 && c.CMAX == 32) { FE_LINES_Z(8, 64, 32) }
+    if (!c.cand && c.W == 8 && c.T == 64 && c.CMAX == 64) { FE_LINES_Z(8, 64, 64) }
 #undef FE_LINES_Z
     return hipErrorInvalidValue;
 }
@@ -950,8 +1229,8 @@ extern "C" int fe_edt_device(const uint8_t* mask, int nx, int ny, int nz, int pl
     const EdtVol one = single_vol(nx, ny, nz, (int64_t)nx * ny * nz, planes);
     uint64_t tiles[3];
     const bool zsum = nz >= ZSUM_MIN_NZ;
-    vol_tiles(nx, ny, nz, planes, pick_cfg(ny).W, pick_cfg(nx).W, zsum, tiles);
     const bool compact = nx <= 1024 && ny <= 1024 && nz <= 1024;
+    vol_tiles(nx, ny, nz, planes, pick_cfg(ny, compact, false).W, pick_cfg(nx, compact, true).W, zsum, tiles);
     FE_TRY(run_passes(mask, work, out, nullptr, 0, one, planes, ny, nx, tiles, compact, zsum,
                       static_cast<hipStream_t>(stream)));
     return FE_OK;
@@ -1023,14 +1302,14 @@ extern "C" int fe_edt_axis_pass(const void* src, int32_t* dst, int nx, int ny, i
     const EdtVol one = single_vol(nx, ny, nz, (int64_t)nx * ny * nz, planes);
     uint64_t tiles[3];
     const bool zsum = nz >= ZSUM_MIN_NZ;
-    vol_tiles(nx, ny, nz, planes, pick_cfg(ny).W, pick_cfg(nx).W, zsum, tiles);
+    const bool compact = nx <= 1024 && ny <= 1024 && nz <= 1024;
+    vol_tiles(nx, ny, nz, planes, pick_cfg(ny, compact, false).W, pick_cfg(nx, compact, true).W, zsum, tiles);
     if (axis == 2) {
         launch_zpass(static_cast<const uint8_t*>(src), dst, nullptr, 0, one, planes, tiles[0], zsum, st);
         FE_TRY(hipGetLastError());
         return FE_OK;
     }
     const int32_t* a = static_cast<const int32_t*>(src);
-    const bool compact = nx <= 1024 && ny <= 1024 && nz <= 1024;
     if (axis == 1) FE_TRY(launch_lines(a, dst, nullptr, 0, one, 1, ny, tiles[1], compact, zsum, st));
     else FE_TRY(launch_lines(a, dst, nullptr, 0, one, 0, nx, tiles[2], compact, false, st));
     return FE_OK;
@@ -1088,7 +1367,8 @@ extern "C" void* fe_edt_batch_create(const int32_t* dims, const int64_t* mask_of
         std::sort(ranges.begin(), ranges.end());
         for (size_t i = 1; i < ranges.size(); ++i)
             if (ranges[i].first < ranges[i - 1].second) return nullptr;
-        const int Wy = pick_cfg(Ly).W, Wx = pick_cfg(Lx).W;
+        const bool compact = Lx <= 1024 && Ly <= 1024 && Lz <= 1024;
+        const int Wy = pick_cfg(Ly, compact, false).W, Wx = pick_cfg(Lx, compact, true).W;
         const bool zsum = zmin >= ZSUM_MIN_NZ;
         uint64_t tot[3] = {0, 0, 0};
         for (int v = 0; v < nvol; ++v) {
@@ -1107,7 +1387,7 @@ extern "C" void* fe_edt_batch_create(const int32_t* dims, const int64_t* mask_of
         b->planes = planes;
         b->Lmax_y = Ly;
         b->Lmax_x = Lx;
-        b->compact = Lx <= 1024 && Ly <= 1024 && Lz <= 1024;
+        b->compact = compact;
         b->zsum = zsum;
         for (int p = 0; p < 3; ++p) b->tiles[p] = tot[p];
         hipError_t e = hipMalloc(&b->table, sizeof(EdtVol) * (size_t)nvol);
