@@ -53,6 +53,32 @@ def event_ms(torch, fn, reps):
     return s.elapsed_time(e) / reps
 
 
+def edt_pass_ms(fe, torch, occ, pair, work, n, reps):
+    """Average durations of the z, y and x passes of the paired EDT, each timed
+    by HIP events on the launch stream inside the step's own sequence and
+    buffers (z: occupancy -> pair, y: pair -> work, x: work -> pair, as
+    fe_edt_device runs them), so that every pass sees the caches the step
+    leaves it; one untimed sequence first."""
+    L = fe._lib.lib()
+    st = fe._lib.stream_handle(None)
+    P = fe._lib.ptr
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    tot = [0.0, 0.0, 0.0]
+    for r in range(reps + 1):
+        ev[0].record()
+        L.fe_edt_axis_pass(P(occ), P(pair), n, n, n, 2, 2, st)
+        ev[1].record()
+        L.fe_edt_axis_pass(P(pair), P(work), n, n, n, 2, 1, st)
+        ev[2].record()
+        L.fe_edt_axis_pass(P(work), P(pair), n, n, n, 2, 0, st)
+        ev[3].record()
+        ev[3].synchronize()
+        if r:
+            for k in range(3):
+                tot[k] += ev[k].elapsed_time(ev[k + 1])
+    return tuple(t / reps for t in tot)
+
+
 def profiled_traffic(kernel, n):
     """HBM bytes per launch of `kernel` at n^3 from the rocprofv3 FETCH_SIZE /
     WRITE_SIZE runs summarised in profiles/hbm_traffic_<n>_pmc.txt (counters
@@ -812,10 +838,7 @@ def main():
     edt_ms = event_ms(torch, edt_part, reps)
     build_ms = event_ms(torch, plan.launch, reps)
     recon_ms = event_ms(torch, lambda: plan.reconstruct(0, dense, occ), reps)
-    pz = event_ms(torch, lambda: L.fe_edt_axis_pass(fe._lib.ptr(occ), fe._lib.ptr(work), n, n, n, 2, 2, st), reps)
-    py = event_ms(torch, lambda: L.fe_edt_axis_pass(fe._lib.ptr(work), fe._lib.ptr(pair), n, n, n, 2, 1, st), reps)
-    px = event_ms(torch, lambda: L.fe_edt_axis_pass(fe._lib.ptr(pair), fe._lib.ptr(work), n, n, n, 2, 0, st), reps)
-    torch.cuda.synchronize()
+    pz, py, px = edt_pass_ms(fe, torch, occ, pair, work, n, reps)
     B3 = 512
     kernels = {
         # name: (ms, algorithmic bytes per launch)
