@@ -72,3 +72,17 @@ __device__ __forceinline__ uint32_t xcd_tile(uint32_t bid, uint32_t grid)
 }
 
 static inline uint32_t round8(uint64_t n) { return (uint32_t)((n + 7) & ~(uint64_t)7); }
+
+// Logical tiles dealt to the XCDs in groups of G consecutive tiles,
+// round-robin: every XCD gets work from the whole tile range, which keeps the
+// XCDs balanced when the work per tile varies along the range (the solid of a
+// part often fills only its middle), while G neighbouring tiles, which may
+// share cache lines, stay in one L2.  `grid` must be a multiple of 8*G.
+template <uint32_t G>
+__device__ __forceinline__ uint32_t xcd_group_tile(uint32_t bid)
+{
+    const uint32_t q = bid >> 3, x = bid & 7u;
+    return (q / G) * (8u * G) + x * G + q % G;
+}
+
+static inline uint32_t round_up(uint64_t n, uint32_t m) { return (uint32_t)((n + m - 1) / m * m); }
