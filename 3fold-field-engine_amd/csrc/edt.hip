@@ -169,11 +169,14 @@ __device__ __forceinline__ uint32_t nonzero_nibble(uint32_t w)
 template <bool PAIR>
 __global__ __launch_bounds__(ZSUM_ROWS) void k_edt_zsum(const uint8_t* __restrict__ masks, int32_t* __restrict__ outs,
                                                         const EdtVol* __restrict__ table, int nvol, EdtVol one,
-                                                        int site_one, uint32_t tiles, uint32_t grid)
+                                                        int site_one, uint32_t tiles)
 {
     __shared__ __align__(16) uint8_t stage[ZSUM_ROWS * ZSUM_STRIDE];
     const int lane = threadIdx.x;
-    const uint32_t tile = xcd_tile(blockIdx.x, grid);
+    // blocks of 64 rows dealt to the XCDs in groups of 64, round-robin (the
+    // y pass then reads the summaries 0.04 ms faster at 512^3 than after
+    // contiguous ranges per XCD)
+    const uint32_t tile = xcd_group_tile<64>(blockIdx.x);  // grid: a multiple of 512
     if (tile >= tiles) return;  // uniform per block
     const EdtVol V = load_vol(table, nvol, one, 0, tile);
     const int64_t rows = (int64_t)V.nx * V.ny;
@@ -1170,13 +1173,13 @@ void launch_zpass(const uint8_t* masks, int32_t* out, const EdtVol* table, int n
     const int site_one = planes == 2 ? 1 : 0;
     const dim3 grid((uint32_t)tiles), block(256);
     if (zsum) {
-        const uint32_t g8 = round8(tiles);
+        const uint32_t gz = round_up(tiles, 512);  // whole groups of 64 blocks on each XCD
         if (planes == 2)
-            hipLaunchKernelGGL(k_edt_zsum<true>, dim3(g8), dim3(ZSUM_ROWS), 0, st, masks, out, table, nvol, one,
-                               site_one, (uint32_t)tiles, g8);
+            hipLaunchKernelGGL(k_edt_zsum<true>, dim3(gz), dim3(ZSUM_ROWS), 0, st, masks, out, table, nvol, one,
+                               site_one, (uint32_t)tiles);
         else
-            hipLaunchKernelGGL(k_edt_zsum<false>, dim3(g8), dim3(ZSUM_ROWS), 0, st, masks, out, table, nvol, one,
-                               site_one, (uint32_t)tiles, g8);
+            hipLaunchKernelGGL(k_edt_zsum<false>, dim3(gz), dim3(ZSUM_ROWS), 0, st, masks, out, table, nvol, one,
+                               site_one, (uint32_t)tiles);
     } else {
         if (planes == 2) hipLaunchKernelGGL(k_edt_zpass<true>, grid, block, 0, st, masks, out, table, nvol, one, site_one);
         else hipLaunchKernelGGL(k_edt_zpass<false>, grid, block, 0, st, masks, out, table, nvol, one, site_one);
