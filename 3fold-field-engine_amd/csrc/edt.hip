@@ -34,6 +34,7 @@
 #include <cmath>
 #include <cstring>
 #include <mutex>
+#include <type_traits>
 #include <utility>
 #include <vector>
 
@@ -772,7 +773,7 @@ __global__ __launch_bounds__(W* T) void k_edt_lines(const int32_t* __restrict__ 
 //     a_{c+1}'s) are each walked left to right along the positions.
 // Four block barriers in all, none in data-dependent rounds.  Constant lines
 // are their own result and sentinel sites stay out of the hulls, as in
-// k_edt_lines.  LDS: kmax*W*T entries, 12 bytes per thread, 12 per line.
+// k_edt_lines.  LDS: kmax*W*T entries, 12 bytes per thread, 16 per line.
 template <int W, int T, int CMAX, bool ZSUM>
 __global__ __launch_bounds__(W* T, CMAX <= 16 ? 8 : (CMAX <= 24 ? 6 : 4)) void k_edt_cand(
     const int32_t* __restrict__ src, int32_t* __restrict__ dst, const EdtVol* __restrict__ table, int nvol,
@@ -780,7 +781,14 @@ __global__ __launch_bounds__(W* T, CMAX <= 16 ? 8 : (CMAX <= 24 ? 6 : 4)) void k
 {
     extern __shared__ uint32_t lds[];
     constexpr int NT = W * T;
-    static_assert((W & (W - 1)) == 0 && CMAX <= 32 && T <= 32, "tile shape");
+    static_assert((W & (W - 1)) == 0 && CMAX <= 32 && T <= 64, "tile shape");
+    // a bit per chunk of a line; helpers: lowest and highest set bit
+    using Mask = typename std::conditional<(T > 32), unsigned long long, uint32_t>::type;
+    constexpr int MB = 8 * (int)sizeof(Mask);
+    auto low_bit = [](Mask m) { return MB == 64 ? __builtin_ctzll((unsigned long long)m) : __builtin_ctz((uint32_t)m); };
+    auto high_bit = [](Mask m) {
+        return MB == 64 ? 63 - __builtin_clzll((unsigned long long)m) : 31 - __builtin_clz((uint32_t)m);
+    };
     const uint32_t tile = xcd_group_tile<32>(blockIdx.x);  // grid: a multiple of 256
     if (tile >= tiles) return;  // uniform per block
     const int pass = axis == 1 ? 1 : 2;
@@ -807,7 +815,8 @@ __global__ __launch_bounds__(W* T, CMAX <= 16 ? 8 : (CMAX <= 24 ? 6 : 4)) void k
     int32_t* rv = lv + NT;                                     // [NT] hull minimum at p_{b+1}
     int32_t* lmin = rv + NT;                                   // [W] smallest value of the line
     int32_t* lmax = lmin + W;                                  // [W] largest value of the line
-    uint32_t* lmask = reinterpret_cast<uint32_t*>(lmax + W);  // [W] nonempty chunks of the line
+    Mask* lmask = reinterpret_cast<Mask*>(lmax + W);          // [W] nonempty chunks of the line (8-byte aligned:
+                                                               // region, 2*NT and 2*W words are even)
     uint16_t* info = reinterpret_cast<uint16_t*>(lmask + W);  // [NT] size | argmin at p_b << 6 | at p_{b+1} << 11
     uint16_t* bnd = info + NT;                                 // [NT] minimiser at p_c: chunk << 8 | entry
 
@@ -823,7 +832,7 @@ __global__ __launch_bounds__(W* T, CMAX <= 16 ? 8 : (CMAX <= 24 ? 6 : 4)) void k
     if (c == 0) {
         lmin[line] = INT_MAX;
         lmax[line] = INT_MIN;
-        lmask[line] = 0u;
+        lmask[line] = 0;
     }
 
     int32_t v[CMAX];
@@ -916,24 +925,24 @@ __global__ __launch_bounds__(W* T, CMAX <= 16 ? 8 : (CMAX <= 24 ? 6 : 4)) void k
             }
             rv[me] = cur + p1 * p1;
             inf |= (uint32_t)li << 6 | (uint32_t)ri << 11;
-            atomicOr(lmask + line, 1u << c);
+            atomicOr(lmask + line, (Mask)1 << c);
         }
         info[me] = (uint16_t)inf;
     }
     __syncthreads();  // hulls, their minima and the masks published
 
-    const uint32_t mask = busy ? lmask[line] : 0u;
+    const Mask mask = busy ? lmask[line] : (Mask)0;
     // 1. a minimiser at p_c
     if (busy && np > 0) {
         const int p = p0, pp = p * p;
         int32_t best = INT_MAX;  // h(p_c, best site)
         uint32_t sel = 0;
-        if (mask >> c & 1u) {
+        if (mask >> c & 1) {
             best = lv[me];
             sel = (uint32_t)c << 8 | (uint32_t)(info[me] >> 6 & 31u);
         }
-        for (uint32_t mr = c + 1 < 32 ? mask >> (c + 1) << (c + 1) : 0u; mr; mr &= mr - 1u) {
-            const int b = __builtin_ctz(mr);  // nonempty chunks to the right, nearest first
+        for (Mask mr = c + 1 < T ? mask >> (c + 1) << (c + 1) : (Mask)0; mr; mr &= mr - 1) {
+            const int b = low_bit(mr);  // nonempty chunks to the right, nearest first
             const int d = b * C - p;          // distance to the chunk's first position
             if (fline + d * d >= best) break;
             const int sb = slot(b);
@@ -951,8 +960,8 @@ __global__ __launch_bounds__(W* T, CMAX <= 16 ? 8 : (CMAX <= 24 ? 6 : 4)) void k
                 sel = (uint32_t)b << 8 | (uint32_t)i;
             }
         }
-        for (uint32_t ml = mask & ((1u << c) - 1u); ml; ml ^= 1u << (31 - __builtin_clz(ml))) {
-            const int b = 31 - __builtin_clz(ml);  // nonempty chunks to the left, nearest first
+        for (Mask ml = mask & (((Mask)1 << c) - 1); ml; ml ^= (Mask)1 << high_bit(ml)) {
+            const int b = high_bit(ml);  // nonempty chunks to the left, nearest first
             const int d = p - (b * C + C - 1);     // distance to the chunk's last position
             if (fline + d * d >= best) break;
             const int sb = slot(b);
@@ -987,15 +996,15 @@ __global__ __launch_bounds__(W* T, CMAX <= 16 ? 8 : (CMAX <= 24 ? 6 : 4)) void k
     if (busy && np > 0) {
         const uint32_t s0 = bnd[me];
         const int b0 = (int)(s0 >> 8), i0 = (int)(s0 & 255u);
-        int b1 = 31 - __builtin_clz(mask), i1 = 255;  // the last chunk of the line: every site from a_c on
+        int b1 = high_bit(mask), i1 = 255;  // the last chunk of the line: every site from a_c on
         if (p1 < L) {
             const uint32_t s1 = bnd[slot(c + 1)];
             b1 = (int)(s1 >> 8);
             i1 = (int)(s1 & 255u);
         }
-        uint32_t cm = mask & (b1 >= 31 ? ~0u : (2u << b1) - 1u) & ~((1u << b0) - 1u);
-        for (; cm; cm &= cm - 1u) {
-            const int b = __builtin_ctz(cm);
+        Mask cm = mask & (b1 >= MB - 1 ? ~(Mask)0 : ((Mask)2 << b1) - 1) & ~(((Mask)1 << b0) - 1);
+        for (; cm; cm &= cm - 1) {
+            const int b = low_bit(cm);
             const int sb = slot(b);
             const uint32_t inf = info[sb];
             const int hi = min(b == b1 ? i1 : 31, (int)(inf & 63u) - 1);
@@ -1077,8 +1086,11 @@ LineCfg pick_cfg(int L, bool compact, bool xpass)
         if (L <= 128) return {w, 8, 16, true};
         if (L <= 256) return {w, 16, 16, true};
         if (L <= 512) return {w, 32, 16, true};
-        if (L <= 768) return {16, 32, 24, true};
-        return {16, 32, 32, true};
+        // longer lines: more chunks of 16 rather than longer chunks, which
+        // keeps 64 VGPRs and 6-8 waves per SIMD (768^3 thin-feature pair:
+        // y 4.0 -> 3.4 ms, x 4.0 -> 3.2 ms against 32 chunks of 24)
+        if (L <= 768) return {16, 48, 16, true};
+        return {16, 64, 16, true};
     }
     if (L <= 2048) return {8, 64, 32, false};
     return {8, 64, 64, false};
@@ -1109,7 +1121,9 @@ hipError_t launch_cand_t(const int32_t* src, int32_t* dst, const EdtVol* table, 
     const uint32_t grid = round_up(tiles, 256);  // whole groups of 32 tiles on each XCD
     const int kmax = (Lmax + T - 1) / T;  // <= CMAX by the choice of shape
     const int region = kmax * W * T;      // hull stacks
-    const size_t lds = (size_t)region * 4 + (size_t)W * T * 12 + (size_t)W * 12;
+    // hulls; minima at both boundaries (8 bytes per thread), line extremes
+    // (8 per line), line masks (8 per line), info and minimisers (4 per thread)
+    const size_t lds = (size_t)region * 4 + (size_t)W * T * 12 + (size_t)W * 16;
     const void* fn = reinterpret_cast<const void*>(&k_edt_cand<W, T, CMAX, ZSUM>);
     if (lds > 65536) {
         hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -1138,8 +1152,8 @@ hipError_t launch_lines(const int32_t* src, int32_t* dst, const EdtVol* table, i
     FE_CAND(16, 8, 16)
     FE_CAND(16, 16, 16)
     FE_CAND(16, 32, 16)
-    FE_CAND(16, 32, 24)
-    FE_CAND(16, 32, 32)
+    FE_CAND(16, 48, 16)
+    FE_CAND(16, 64, 16)
 #undef FE_CAND
 #define FE_LINES_Z(w, t, cm)                                                                                 \
     return zsum ? launch_lines_t<w, t, cm, true>(src, dst, table, nvol, one, axis, Lmax, tiles, st)         \

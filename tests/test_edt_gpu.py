@@ -52,7 +52,9 @@ def test_pair_cases_persistent(fe, golden, gpu):
 
 def random_masks(rng):
     for shape in ((1, 1, 2), (3, 70, 5), (130, 9, 11), (17, 300, 3), (64, 64, 64), (5, 6, 1100), (1100, 3, 4),
-                  (2, 1030, 7), (4, 5, 16), (20, 30, 17), (9, 8, 31), (3, 2, 4096)):
+                  (2, 1030, 7), (4, 5, 16), (20, 30, 17), (9, 8, 31), (3, 2, 4096),
+                  # every axis <= 1024 with lines of 513..1024 positions (48 and 64 chunks per line)
+                  (3, 900, 5), (1000, 4, 6), (600, 7, 9), (5, 1000, 17), (8, 700, 40)):
         yield shape, rng.random(shape) < rng.uniform(0.01, 0.9)
     m = np.zeros((90, 80, 70), bool)
     m[45, 40, 35] = True
