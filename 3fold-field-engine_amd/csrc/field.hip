@@ -483,18 +483,17 @@ __global__ __launch_bounds__(256) void k_checksum(const FieldDesc* __restrict__ 
 // Point SDF of a resident field: the recipe's fp32 fold at arbitrary points
 // (the same arithmetic as the stored tiles), evaluated over the culled list
 // of the superblock containing the point; points outside every superblock
-// box evaluate the whole recipe.  One thread per point.
-__global__ __launch_bounds__(256) void k_query_points(const FieldDesc* __restrict__ desc, const float* __restrict__ pts,
-                                                      int64_t n, float* __restrict__ out)
+// box, or in a superblock that kept no list, evaluate the whole recipe.
+
+// The superblock whose list holds point (x, y, z) and that list (-1, nullptr:
+// the whole recipe).
+__device__ __forceinline__ int query_list(const FieldDesc& D, float x, float y, float z, const uint16_t*& list,
+                                          int& nl)
 {
-    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
-    const FieldDesc& D = *desc;
-    const float x = pts[3 * i], y = pts[3 * i + 1], z = pts[3 * i + 2];
+    list = nullptr;
+    nl = D.nprims;
     const float span = (float)(D.SB * D.B) * D.pitch;
     const int sx = (int)floorf((x - D.x0) / span), sy = (int)floorf((y - D.y0) / span), sz = (int)floorf((z - D.z0) / span);
-    const uint16_t* list = nullptr;
-    int nl = D.nprims;
     if (sx >= 0 && sy >= 0 && sz >= 0 && sx < D.nsbx && sy < D.nsby && sz < D.nsbz) {
         // the point must lie in the box the list was culled for
         const int bx0 = sx * D.SB, by0 = sy * D.SB, bz0 = sz * D.SB, B = D.B;
@@ -506,8 +505,23 @@ __global__ __launch_bounds__(256) void k_query_points(const FieldDesc* __restric
         if (len >= 0 && x >= lx && x <= ux && y >= ly && y <= uy && z >= lz && z <= uz) {
             list = D.lists + (int64_t)sb * D.cap;
             nl = len;
+            return sb;
         }
     }
+    return -1;
+}
+
+// One thread per point, each over its own list (small queries).
+__global__ __launch_bounds__(256) void k_query_points(const FieldDesc* __restrict__ desc, const float* __restrict__ pts,
+                                                      int64_t n, float* __restrict__ out)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const FieldDesc& D = *desc;
+    const float x = pts[3 * i], y = pts[3 * i + 1], z = pts[3 * i + 2];
+    const uint16_t* list;
+    int nl;
+    query_list(D, x, y, z, list, nl);
     float f = 0.0f;
     for (int j = 0; j < nl; ++j) {
         const Prim p = D.prims[list ? list[j] : j];
@@ -515,6 +529,110 @@ __global__ __launch_bounds__(256) void k_query_points(const FieldDesc* __restric
         f = j == 0 ? v : fold(p.op, f, v);
     }
     out[i] = f;
+}
+
+// Large queries: the points are ordered by superblock first (FE_QBINS bins
+// of superblock ids, one more for the whole recipe; a bin may mix
+// superblocks, which costs only uniformity), so that a wave's lanes share one
+// or two lists: each list then runs once for the wave with its primitive
+// records read by scalar loads and one primitive formula at a time, where
+// lanes of scattered points would step through every kind of formula and
+// gather their own records.  Values are bit-identical to k_query_points.
+#define FE_QBINS 8192
+#define FE_QSORT_MIN 16384  // fewer points: one thread per point
+
+__global__ __launch_bounds__(256) void k_qbin_zero(int32_t* __restrict__ cnt)
+{
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t < FE_QBINS + 2) cnt[t] = 0;
+}
+
+__global__ __launch_bounds__(256) void k_qbin_count(const FieldDesc* __restrict__ desc, const float* __restrict__ pts,
+                                                    int64_t n, int32_t* __restrict__ cnt, int32_t* __restrict__ bin,
+                                                    int32_t* __restrict__ rank)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const uint16_t* list;
+    int nl;
+    const int sb = query_list(*desc, pts[3 * i], pts[3 * i + 1], pts[3 * i + 2], list, nl);
+    const int b = sb < 0 ? FE_QBINS : sb % FE_QBINS;
+    bin[i] = b;
+    rank[i] = atomicAdd(cnt + b, 1);
+}
+
+// Exclusive scan of the FE_QBINS + 1 bin counts, in place (one workgroup).
+__global__ __launch_bounds__(1024) void k_qbin_scan(int32_t* __restrict__ cnt)
+{
+    constexpr int NB = FE_QBINS + 1, PER = (NB + 1023) / 1024;
+    __shared__ int part[1024];
+    const int t = threadIdx.x;
+    int loc[PER], s = 0;
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        const int idx = t * PER + k;
+        loc[k] = idx < NB ? cnt[idx] : 0;
+        s += loc[k];
+    }
+    part[t] = s;
+    __syncthreads();
+    for (int d = 1; d < 1024; d <<= 1) {
+        const int v = t >= d ? part[t - d] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    int base = part[t] - s;
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        const int idx = t * PER + k;
+        if (idx < NB) {
+            cnt[idx] = base;
+            base += loc[k];
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_qbin_place(const int32_t* __restrict__ off, const int32_t* __restrict__ bin,
+                                                    const int32_t* __restrict__ rank, int64_t n,
+                                                    int32_t* __restrict__ perm)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    perm[off[bin[i]] + rank[i]] = (int32_t)i;
+}
+
+__global__ __launch_bounds__(256) void k_query_sorted(const FieldDesc* __restrict__ desc, const float* __restrict__ pts,
+                                                      const int32_t* __restrict__ perm, int64_t n,
+                                                      float* __restrict__ out)
+{
+    const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const bool valid = k < n;
+    const FieldDesc& D = *desc;
+    const int64_t i = valid ? perm[k] : 0;
+    float x = 0.0f, y = 0.0f, z = 0.0f;
+    int key = -2;  // -2: no point; -1: the whole recipe; else the superblock
+    if (valid) {
+        x = pts[3 * i];
+        y = pts[3 * i + 1];
+        z = pts[3 * i + 2];
+        const uint16_t* list;
+        int nl;
+        key = query_list(D, x, y, z, list, nl);
+    }
+    uint64_t todo = __ballot(valid);
+    float f = 0.0f;
+    while (todo) {  // the distinct lists of the wave, one at a time, all lanes running
+        const int lead = __builtin_amdgcn_readlane(key, (int)__builtin_ctzll(todo));
+        const bool mine = valid && key == lead;
+        const uint16_t* list = lead >= 0 ? D.lists + (int64_t)lead * D.cap : nullptr;
+        const int nl = lead >= 0 ? D.list_len[lead] : D.nprims;
+        float v = 0.0f;
+        eval_points<1>(D.prims, list, nl, &x, &y, &z, &v);
+        if (mine) f = v;
+        todo &= ~__ballot(mine);
+    }
+    if (valid) out[i] = f;
 }
 
 // ---------------------------------------------------------------- dense-source classify
@@ -768,8 +886,30 @@ extern "C" int fe_query_points(const void* desc, const float* pts, int64_t n, fl
 {
     if (!desc || !pts || !out || n < 0) return FE_INVALID;
     if (n == 0) return FE_OK;
-    hipLaunchKernelGGL(k_query_points, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, static_cast<hipStream_t>(stream),
-                       static_cast<const FieldDesc*>(desc), pts, n, out);
-    FE_TRY(hipGetLastError());
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const FieldDesc* d = static_cast<const FieldDesc*>(desc);
+    const dim3 grid((unsigned)((n + 255) / 256));
+    if (n < FE_QSORT_MIN || n > INT32_MAX) {
+        hipLaunchKernelGGL(k_query_points, grid, dim3(256), 0, st, d, pts, n, out);
+        FE_TRY(hipGetLastError());
+        return FE_OK;
+    }
+    // stream-ordered scratch (pooled by the runtime): bin counts / offsets,
+    // then per point its bin, its rank in the bin and the sorted order
+    int32_t* buf = nullptr;
+    FE_TRY(hipMallocAsync(reinterpret_cast<void**>(&buf), sizeof(int32_t) * ((size_t)FE_QBINS + 2 + 3 * (size_t)n), st));
+    int32_t* cnt = buf;
+    int32_t* bin = cnt + FE_QBINS + 2;
+    int32_t* rank = bin + n;
+    int32_t* perm = rank + n;
+    hipLaunchKernelGGL(k_qbin_zero, dim3((FE_QBINS + 2 + 255) / 256), dim3(256), 0, st, cnt);
+    hipLaunchKernelGGL(k_qbin_count, grid, dim3(256), 0, st, d, pts, n, cnt, bin, rank);
+    hipLaunchKernelGGL(k_qbin_scan, dim3(1), dim3(1024), 0, st, cnt);
+    hipLaunchKernelGGL(k_qbin_place, grid, dim3(256), 0, st, cnt, bin, rank, n, perm);
+    hipLaunchKernelGGL(k_query_sorted, grid, dim3(256), 0, st, d, pts, perm, n, out);
+    const hipError_t e = hipGetLastError();
+    const hipError_t f = hipFreeAsync(buf, st);
+    FE_TRY(e);
+    FE_TRY(f);
     return FE_OK;
 }

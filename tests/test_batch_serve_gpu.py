@@ -60,7 +60,7 @@ def test_server_results_alone_and_in_mix(fe, gpu):
             assert same(srv.result(rid, timeout=120).sparse_field(), alone), rid
         rng = np.random.default_rng(3)
         for rid, (m, r) in zip(rids, metas):
-            pts = rng.uniform(-2, m.nx + 2, (4097, 3)).astype(np.float32)
+            pts = rng.uniform(-2, m.nx + 2, (20001, 3)).astype(np.float32)
             got = srv.query(rid, pts)
             ref = O.recipe_eval_points(np.ascontiguousarray(r).view(O.PRIM), pts)
             assert got.tobytes() == ref.tobytes(), rid

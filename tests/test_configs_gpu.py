@@ -142,11 +142,14 @@ def test_list_fallbacks(fe, gpu, digests, name):
     check(sparse_digests(fe.klassificera_och_evaluera_recept(meta, rec)), ref, name + " plain")
     # the server's query kernel falls back to the whole recipe where the list overflowed
     import oracle as O
-    pts = np.random.default_rng(9).uniform(30.0, 66.0, (4096, 3)).astype(np.float32)
+    # (20,000 points: the superblock-sorted query path; 4096: one thread per point)
+    pts = np.random.default_rng(9).uniform(30.0, 66.0, (20000, 3)).astype(np.float32)
     import torch
     f = fe.server.resident_fields(plan, torch.cuda.current_stream())[0]
     got = f.query_device(torch.from_numpy(pts).to(gpu)).cpu().numpy()
     assert got.tobytes() == O.recipe_eval_points(np.ascontiguousarray(rec).view(O.PRIM), pts).tobytes()
+    few = f.query_device(torch.from_numpy(pts[:4096]).to(gpu)).cpu().numpy()
+    assert few.tobytes() == got[:4096].tobytes()
 
 
 def test_crowded_cases_in_one_plan(fe, gpu, digests):
@@ -210,7 +213,7 @@ def test_cfg5_served_under_poisson(fe, gpu, digests):
             check(sparse_digests(srv.result(rid).sparse_field()), ref[k], f"cfg5 request {k}")
         rng = np.random.default_rng(55)
         for k in (0, len(rids) - 1):
-            pts = rng.uniform(0.0, 511.0, (4096, 3)).astype(np.float32)
+            pts = rng.uniform(0.0, 511.0, (20000, 3)).astype(np.float32)
             got = srv.query(rids[k], pts)
             assert got.tobytes() == O.recipe_eval_points(np.ascontiguousarray(recs[k]).view(O.PRIM), pts).tobytes()
     finally:
