@@ -990,9 +990,12 @@ __global__ __launch_bounds__(W* T, CMAX <= 16 ? 8 : (CMAX <= 24 ? 6 : 4)) void k
     __syncthreads();  // minimisers published
 
     // 2. the chunk's positions over the hulls holding the sites a_c .. a_{c+1}
-    // (a constant line's value is its minimum, so no loaded value stays live)
+    // (a constant line's value is its minimum, so no loaded value stays live).
+    // Every hull adds the same p^2 at a position, so the walks take minima of
+    // the values less p^2 and p^2 is added once, at the end.
+    constexpr int32_t far = INT_MAX - (1 << 21);  // + p^2 (p < 1024) stays finite
 #pragma unroll
-    for (int j = 0; j < CMAX; ++j) v[j] = busy ? sentinel : fline;
+    for (int j = 0; j < CMAX; ++j) v[j] = busy ? far : fline;
     if (busy && np > 0) {
         const uint32_t s0 = bnd[me];
         const int b0 = (int)(s0 >> 8), i0 = (int)(s0 & 255u);
@@ -1024,10 +1027,10 @@ __global__ __launch_bounds__(W* T, CMAX <= 16 ? 8 : (CMAX <= 24 ? 6 : 4)) void k
                 sn2 = (int)(en & 1023u) << 1;
                 nxt = at(en, p0);
             }
-            // 2p and p^2 as recurrences: kept opaque so that the compiler does
-            // not hoist CMAX precomputed copies out of the chunk loop
-            int p2 = 2 * p0, pp = p0 * p0;
-            asm volatile("" : "+v"(p2), "+v"(pp));
+            // 2p as a recurrence: kept opaque so that the compiler does not
+            // hoist CMAX precomputed copies out of the chunk loop
+            int p2 = 2 * p0;
+            asm volatile("" : "+v"(p2));
             // every position of v, also past the chunk end (never stored)
 #pragma unroll
             for (int j = 0; j < CMAX; ++j) {
@@ -1042,12 +1045,18 @@ __global__ __launch_bounds__(W* T, CMAX <= 16 ? 8 : (CMAX <= 24 ? 6 : 4)) void k
                         nxt = (int32_t)(e2 >> 10) - (int32_t)__umul24((uint32_t)p2, e2 & 1023u);
                     }
                 }
-                v[j] = min(v[j], cur + pp);
+                v[j] = min(v[j], cur);
                 cur -= s2;  // values at p + 1
                 nxt -= sn2;
-                pp += p2 + 1;
                 p2 += 2;
             }
+        }
+        int pp = p0 * p0, p2 = 2 * p0;
+#pragma unroll
+        for (int j = 0; j < CMAX; ++j) {
+            v[j] = min(v[j] + pp, sentinel);
+            pp += p2 + 1;
+            p2 += 2;
         }
     }
     if (active) store_chunk<CMAX>(v, dst, lbase, p0, p1, axis_stride);
