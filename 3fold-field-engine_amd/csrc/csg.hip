@@ -11,13 +11,15 @@
 // and inside a tile the operand reads (status 3) precede the instructions
 // (status 4).
 //
-// MI355X design: one workgroup per output tile of 4x8x64 samples.  The host
+// MI355X design: one wave per output tile of 1x8x64 samples.  The host
 // compiles the program into micro-ops over LDS slots (linear-scan liveness,
-// every input read once right before its first use).  An input read walks the
-// field's box tree breadth-first with the whole workgroup to list the leaves
-// overlapping the tile, then copies their intersections into the slot with
-// coalesced reads along z.  Each instruction then runs over the slot arrays
-// in LDS; only the final value is written to HBM.
+// every input read once right before its first use).  Before the program, a
+// wave per leaf appends the leaf to the lists of the tiles it overlaps; an
+// input read then copies the parts of its tile's listed leaves into the slot
+// with coalesced reads along z (a tile whose list overflows, or a launch whose
+// lists would not fit, walks the field's box tree breadth-first with the whole
+// workgroup instead).  Each instruction then runs over the slot arrays in
+// LDS; only the final value is written to HBM.
 #include "fe_common.hpp"
 #include <algorithm>
 #include <climits>
@@ -54,8 +56,21 @@ struct MOp {  // micro-op of the compiled program
 };
 static_assert(sizeof(MOp) == 24, "micro-op");
 
-constexpr int TX = 4, TY = 8, TZ = 64, TS = TX * TY * TZ;  // 2048 samples per tile
-constexpr int FRONT = 1024;                                  // BFS frontier / leaf list capacity
+// Output tiles of 1 x 8 x 64 samples, one wave each (many tiles in flight per
+// CU hide the dependent loads of a read); leaf lists and start nodes belong to
+// list tiles of LX = 4 output tiles along x.
+constexpr int TX = 1, TY = 8, TZ = 64, TS = TX * TY * TZ;  // 512 samples per tile
+constexpr int LX = 4;                                       // list tile: LX x TY x TZ samples
+constexpr int NTH = 64;                                     // threads per workgroup
+constexpr int FRONT = 256;                                  // BFS frontier / leaf list capacity
+
+// A load through a pointer known to be in device memory (global, not generic,
+// instructions for pointers read out of descriptor tables)
+template <class T>
+__device__ __forceinline__ T gld(const T* p)
+{
+    return *(const __attribute__((address_space(1))) T*)p;
+}
 
 __device__ __forceinline__ bool box_overlap(const int* lo, const int* shape, const int* tlo, const int* thi)
 {
@@ -99,46 +114,223 @@ __device__ int tile_start(const Node52* __restrict__ nodes, const int* tlo, cons
     }
 }
 
-// Start nodes of every (read, tile): one thread each, before the program runs,
-// so a tile's breadth-first read starts below the levels that hold it whole.
+// Leaf lists of every (read, list tile), LCAP words each: word 0 counts the
+// leaves overlapping the list tile, words 1 .. LCAP-1 hold the first LCAP-1 of
+// them (in no particular order: the leaves of a checked operand are
+// disjoint).  A count >= LCAP marks a list that did not fit; the reads of its
+// tiles walk the tree.
+constexpr int LCAP = 64;
+
+// Start nodes of every (read, list tile): one thread each, before the program
+// runs, so a tile's breadth-first read starts below the levels that hold it
+// whole.  With leaf lists (built first) only the tiles whose list overflowed
+// walk the tree.
 __global__ __launch_bounds__(256) void k_csg_starts(const FieldRef* __restrict__ fields, const int* __restrict__ reads,
-                                                    int nreads, int nx, int ny, int nz, int gty, int gtz, int ntiles,
-                                                    int* __restrict__ starts)
+                                                    int nreads, int nx, int ny, int nz, int gty, int gtz, int nlt,
+                                                    const int* __restrict__ lists, int* __restrict__ starts)
 {
     const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (g >= (int64_t)nreads * ntiles) return;
-    const int r = (int)(g / ntiles), t = (int)(g % ntiles);
+    if (g >= (int64_t)nreads * nlt) return;
+    if (lists && lists[g * LCAP] < LCAP) return;  // never read
+    const int r = (int)(g / nlt), t = (int)(g % nlt);
     const int tz = t % gtz, ty = (t / gtz) % gty, tx = t / (gtz * gty);
-    const int tlo[3] = {tx * TX, ty * TY, tz * TZ};
-    const int thi[3] = {min(tlo[0] + TX, nx), min(tlo[1] + TY, ny), min(tlo[2] + TZ, nz)};
+    const int tlo[3] = {tx * LX, ty * TY, tz * TZ};
+    const int thi[3] = {min(tlo[0] + LX, nx), min(tlo[1] + TY, ny), min(tlo[2] + TZ, nz)};
     starts[g] = tile_start(fields[reads[r]].nodes, tlo, thi);
 }
 
+// Zeroes the count word of every leaf list.
+__global__ __launch_bounds__(256) void k_csg_list_clear(int* __restrict__ lists, int64_t n)
+{
+    const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (g < n) lists[g * LCAP] = 0;
+}
+
+// One wave per (read, node record): a leaf appends itself to the lists of the
+// list tiles its box (clipped to the grid) overlaps, one list tile per lane (a
+// leaf of the exterior may cover thousands of them).
+__global__ __launch_bounds__(256) void k_csg_lists(const FieldRef* __restrict__ fields, const int* __restrict__ reads,
+                                                   int nx, int ny, int nz, int gty, int gtz, int nlt,
+                                                   int* __restrict__ lists)
+{
+    const int r = blockIdx.y;
+    const FieldRef F = fields[reads[r]];
+    const int64_t i = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (i >= F.count) return;
+    const Node52& nd = F.nodes[i];
+    if (nd.children[0] >= 0) return;
+    const int n[3] = {nx, ny, nz}, T[3] = {LX, TY, TZ};
+    int t0[3], nt[3];
+    for (int a = 0; a < 3; ++a) {
+        const int64_t lo = max((int64_t)nd.lo[a], (int64_t)0);
+        const int64_t hi = min((int64_t)nd.lo[a] + nd.shape[a], (int64_t)n[a]);
+        if (hi <= lo) return;
+        t0[a] = (int)(lo / T[a]);
+        nt[a] = (int)((hi - 1) / T[a]) - t0[a] + 1;
+    }
+    int* base = lists + (int64_t)r * nlt * LCAP;
+    const int total = nt[0] * nt[1] * nt[2];  // <= 1024 * 512 * 64
+    for (int k = lane; k < total; k += 64) {
+        const int tz = t0[2] + k % nt[2], ty = t0[1] + (k / nt[2]) % nt[1], tx = t0[0] + k / (nt[2] * nt[1]);
+        int* L = base + ((int64_t)(tx * gty + ty) * gtz + tz) * LCAP;
+        const int c = atomicAdd(L, 1);
+        if (c < LCAP - 1) L[1 + c] = (int)i;
+    }
+}
+
+// Copies the part of leaf nd inside the tile [tlo, thi) into the slot.  A part
+// has at most TX*TY = 8 rows (x, y) of at most TZ = 64 samples along z: lanes
+// run along z in groups of the next power of two >= the row length, the
+// groups over the rows, so the only division is row / ey (ey <= 8, by a
+// multiply and shift, exact for rows < 2^11).
+__device__ __forceinline__ void copy_leaf_part(const Node52& nd, const uint32_t* __restrict__ payload, const int* tlo,
+                                               const int* thi, float* slot)
+{
+    int lo[3], hi[3];
+    for (int a = 0; a < 3; ++a) {
+        lo[a] = max(tlo[a], nd.lo[a]);
+        hi[a] = min(thi[a], nd.lo[a] + nd.shape[a]);
+    }
+    const int ex = hi[0] - lo[0], ey = hi[1] - lo[1], ez = hi[2] - lo[2];
+    if (ex <= 0 || ey <= 0 || ez <= 0) return;
+    const int lg = ez <= 1 ? 0 : 32 - __clz(ez - 1);  // group width 2^lg >= ez
+    const int zl = (int)threadIdx.x & ((1 << lg) - 1);
+    if (zl >= ez) return;
+    const int s1 = nd.stored[1], s2 = nd.stored[2];
+    const bool bx = nd.stored[0] == 1, by = s1 == 1;
+    const int z = lo[2] + zl;
+    const int64_t sz = s2 == 1 ? 0 : z - nd.lo[2];
+    const uint32_t* src = payload + noff(nd) + sz;
+    float* dst = slot + (z - tlo[2]);
+    const uint32_t inv = (65536u + (uint32_t)ey - 1u) / (uint32_t)ey;  // ceil(2^16 / ey)
+    const int rows = ex * ey;
+    for (int r = (int)threadIdx.x >> lg; r < rows; r += NTH >> lg) {
+        const int rx = (int)(((uint32_t)r * inv) >> 16), ry = r - rx * ey;
+        const int x = lo[0] + rx, y = lo[1] + ry;
+        const int64_t sx = bx ? 0 : x - nd.lo[0], sy = by ? 0 : y - nd.lo[1];
+        dst[((x - tlo[0]) * TY + (y - tlo[1])) * TZ] = __uint_as_float(src[(sx * s1 + sy) * s2]);
+    }
+}
+
+// A leaf record as a read of a listed tile uses it: lane l holds leaf l's.
+struct LeafRec {
+    int lo0, lo1, lo2, sh0, sh1, sh2, st0, st1, st2, offl, offh;
+};
+
+// Loads the records of the nl <= 63 leaves of a list tile (lane l: the l-th
+// listed record id in `myid`) at once.
+__device__ __forceinline__ LeafRec load_leaves(const FieldRef& F, int myid, int nl)
+{
+    LeafRec q = {0, 0, 0, 0, 0, 0, 1, 1, 1, 0, 0};
+    if ((int)threadIdx.x < nl) {
+        const int32_t* nd = reinterpret_cast<const int32_t*>(F.nodes + myid);
+        q.lo0 = gld(nd); q.lo1 = gld(nd + 1); q.lo2 = gld(nd + 2);
+        q.sh0 = gld(nd + 3); q.sh1 = gld(nd + 4); q.sh2 = gld(nd + 5);
+        q.st0 = gld(nd + 6); q.st1 = gld(nd + 7); q.st2 = gld(nd + 8);
+        q.offl = gld(nd + 11); q.offh = gld(nd + 12);
+    }
+    return q;
+}
+
+// Finds the leaf of each of the TY samples (x, y, z) of lane z in the tile
+// [tlo, thi) and loads the samples together into v.  Each lane first derives
+// its own leaf's terms for the tile (overlap, rows as a bit mask, z range,
+// address of row tlo[1] less the z term); the leaves that touch the tile then
+// broadcast those terms in turn (few scalar instructions per leaf: the scalar
+// unit is shared by a CU's waves) and the rows are selected without branches.
+// Returns the rows found (bit k: row k); samples no leaf covers (an operand
+// with gaps, whose status already fails the program) are left out.
+__device__ __forceinline__ uint32_t gather_leaves(const FieldRef& F, const LeafRec& q, int nl, const int* tlo,
+                                                  const int* thi, uint32_t* v)
+{
+    static_assert(NTH == 64 && TX == 1 && TZ == 64, "one wave per tile, a lane per z");
+    const int lane = threadIdx.x;
+    const int x = tlo[0];
+    bool use = false;
+    uint32_t rm = 0;
+    int zlo = 0, zhi = 0, rs = 0, zs = 0;
+    int64_t base = 0;
+    if (lane < nl) {
+        const int y0 = max(q.lo1, tlo[1]), y1 = min(q.lo1 + q.sh1, thi[1]);
+        use = x >= q.lo0 && x < q.lo0 + q.sh0 && y0 < y1 && q.lo2 < thi[2] && q.lo2 + q.sh2 > tlo[2];
+        if (use) rm = ((1u << (y1 - tlo[1])) - 1u) & ~((1u << (y0 - tlo[1])) - 1u);
+        rs = q.st1 == 1 ? 0 : q.st2;
+        zs = q.st2 == 1 ? 0 : 1;
+        zlo = q.lo2;
+        zhi = q.lo2 + q.sh2;
+        const int64_t off = (int64_t)(((uint64_t)(uint32_t)q.offh << 32) | (uint32_t)q.offl);
+        base = off + (int64_t)(q.st0 == 1 ? 0 : x - q.lo0) * q.st1 * q.st2 + (int64_t)(tlo[1] - q.lo1) * rs -
+               (int64_t)zs * q.lo2;
+    }
+    const int z = tlo[2] + lane;
+    const bool zin = z < thi[2];
+    int64_t addr[TY];
+    uint32_t have = 0;
+#pragma unroll
+    for (int r = 0; r < TY; ++r) addr[r] = 0;
+    for (uint64_t todo = __ballot(use); todo; todo &= todo - 1) {
+        const int l = __builtin_ctzll(todo);
+        const int ZL = __builtin_amdgcn_readlane(zlo, l), ZH = __builtin_amdgcn_readlane(zhi, l);
+        const int RS = __builtin_amdgcn_readlane(rs, l), ZS = __builtin_amdgcn_readlane(zs, l);
+        const uint32_t RM = (uint32_t)__builtin_amdgcn_readlane((int)rm, l);
+        const int64_t B = (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(base >> 32), l) << 32) |
+                                    (uint32_t)__builtin_amdgcn_readlane((int)base, l));
+        const uint32_t lm = zin && z >= ZL && z < ZH ? RM : 0u;
+        int64_t a = B + (int64_t)ZS * z;
+        have |= lm;
+#pragma unroll
+        for (int r = 0; r < TY; ++r) {
+            addr[r] = (lm >> r & 1u) ? a : addr[r];
+            a += RS;
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < TY; ++r) v[r] = (have >> r & 1u) ? gld(F.payload + addr[r]) : 0u;
+    return have;
+}
+
+// Writes gathered rows into a slot.
+__device__ __forceinline__ void put_rows(float* slot, const uint32_t* v, uint32_t have)
+{
+#pragma unroll
+    for (int r = 0; r < TY; ++r)
+        if (have >> r & 1u) slot[r * TZ + threadIdx.x] = __uint_as_float(v[r]);
+}
+
+// Reads field F over the tile [tlo, thi) from the nl <= 63 leaves of its list
+// tile: the lanes load the records at once, then each lane loads its TY
+// samples together, so a read waits for two loads in a row whatever the
+// number of leaves.
+__device__ __forceinline__ void read_listed(const FieldRef& F, const int* tlo, const int* thi, float* slot, int myid,
+                                           int nl)
+{
+    const LeafRec q = load_leaves(F, myid, nl);
+    uint32_t v[TY];
+    const uint32_t have = gather_leaves(F, q, nl, tlo, thi, v);
+    put_rows(slot, v, have);
+}
+
 // Reads field f over the tile [tlo, thi) into slot (layout x,y,z with TY*TZ, TZ
-// strides): a breadth-first walk from node `start` collects the leaves
-// overlapping the tile, then the workgroup copies each leaf's part.
-__device__ void read_tile(const FieldRef& F, const int* tlo, const int* thi, float* slot, int* cur, int* nxt, int* leaves,
-                          int* counters, int start)
+// strides): the workgroup copies the part of every leaf on the tile's list,
+// or, without a list that fits, a breadth-first walk from node `start`
+// collects the leaves overlapping the tile first.
+__device__ __forceinline__ void read_tile(const FieldRef& F, const int* tlo, const int* thi, float* slot, int* cur, int* nxt, int* leaves,
+                          int* counters, int start, const int* __restrict__ list)
 {
     const int tid = threadIdx.x;
+    if (list) {
+        const int myid = tid < LCAP - 1 ? list[1 + tid] : 0;  // loaded with the count
+        const int nl = list[0];                                // uniform
+        if (nl < LCAP) {
+            read_listed(F, tlo, thi, slot, myid, nl);
+            __syncthreads();
+            return;
+        }
+    }
     const Node52& sn = F.nodes[start];
     if (sn.children[0] < 0) {  // the tile lies in one leaf: no walk (start is uniform)
-        int lo[3], hi[3];
-        for (int a = 0; a < 3; ++a) {
-            lo[a] = max(tlo[a], sn.lo[a]);
-            hi[a] = min(thi[a], sn.lo[a] + sn.shape[a]);
-        }
-        const int ex = hi[0] - lo[0], ey = hi[1] - lo[1], ez = hi[2] - lo[2];
-        if (ex > 0 && ey > 0 && ez > 0) {
-            const int64_t off = noff(sn);
-            for (int j = tid; j < ex * ey * ez; j += 256) {
-                const int z = lo[2] + j % ez, y = lo[1] + (j / ez) % ey, x = lo[0] + j / (ez * ey);
-                const int64_t sx = sn.stored[0] == 1 ? 0 : x - sn.lo[0], sy = sn.stored[1] == 1 ? 0 : y - sn.lo[1],
-                              sz = sn.stored[2] == 1 ? 0 : z - sn.lo[2];
-                slot[((x - tlo[0]) * TY + (y - tlo[1])) * TZ + (z - tlo[2])] =
-                    __uint_as_float(F.payload[off + (sx * sn.stored[1] + sy) * sn.stored[2] + sz]);
-            }
-        }
+        copy_leaf_part(sn, F.payload, tlo, thi, slot);
         __syncthreads();
         return;
     }
@@ -154,7 +346,7 @@ __device__ void read_tile(const FieldRef& F, const int* tlo, const int* thi, flo
         if (n == 0 || counters[3]) break;
         if (tid == 0) counters[1] = 0;
         __syncthreads();
-        for (int i = tid; i < n; i += 256) {
+        for (int i = tid; i < n; i += NTH) {
             const Node52& nd = F.nodes[cur[i]];
             if (nd.children[0] < 0) {
                 const int k = atomicAdd(&counters[2], 1);
@@ -173,36 +365,19 @@ __device__ void read_tile(const FieldRef& F, const int* tlo, const int* thi, flo
             }
         }
         __syncthreads();
-        for (int i = tid; i < counters[1] && i < FRONT; i += 256) cur[i] = nxt[i];
+        for (int i = tid; i < counters[1] && i < FRONT; i += NTH) cur[i] = nxt[i];
         if (tid == 0) counters[0] = counters[1];
         __syncthreads();
     }
     const int dx = thi[0] - tlo[0], dy = thi[1] - tlo[1], dz = thi[2] - tlo[2];
     if (counters[3]) {  // too many boxes for the lists: per-sample descent
-        for (int j = tid; j < dx * dy * dz; j += 256) {
+        for (int j = tid; j < dx * dy * dz; j += NTH) {
             const int z = j % dz, y = (j / dz) % dy, x = j / (dz * dy);
             slot[(x * TY + y) * TZ + z] = __uint_as_float(descend(F.nodes, F.payload, tlo[0] + x, tlo[1] + y, tlo[2] + z));
         }
     } else {
         const int nl = counters[2];
-        for (int l = 0; l < nl; ++l) {
-            const Node52& nd = F.nodes[leaves[l]];
-            int lo[3], hi[3];
-            for (int a = 0; a < 3; ++a) {
-                lo[a] = max(tlo[a], nd.lo[a]);
-                hi[a] = min(thi[a], nd.lo[a] + nd.shape[a]);
-            }
-            const int ex = hi[0] - lo[0], ey = hi[1] - lo[1], ez = hi[2] - lo[2];
-            if (ex <= 0 || ey <= 0 || ez <= 0) continue;
-            const int64_t off = noff(nd);
-            for (int j = tid; j < ex * ey * ez; j += 256) {
-                const int z = lo[2] + j % ez, y = lo[1] + (j / ez) % ey, x = lo[0] + j / (ez * ey);
-                const int64_t sx = nd.stored[0] == 1 ? 0 : x - nd.lo[0], sy = nd.stored[1] == 1 ? 0 : y - nd.lo[1],
-                              sz = nd.stored[2] == 1 ? 0 : z - nd.lo[2];
-                slot[((x - tlo[0]) * TY + (y - tlo[1])) * TZ + (z - tlo[2])] =
-                    __uint_as_float(F.payload[off + (sx * nd.stored[1] + sy) * nd.stored[2] + sz]);
-            }
-        }
+        for (int l = 0; l < nl; ++l) copy_leaf_part(F.nodes[leaves[l]], F.payload, tlo, thi, slot);
     }
     __syncthreads();
 }
@@ -217,10 +392,13 @@ __device__ __forceinline__ int ref_tile(int x, int y, int z, int nty, int ntz)
 // lds_slots live in LDS; the others (programs with more simultaneously live
 // values than LDS holds) in this workgroup's spill_slots * TS floats of
 // `arena`.  err_nf receives the first reference tile with a non-finite result.
-__global__ __launch_bounds__(256) void k_csg_tiles(const FieldRef* __restrict__ fields, const MOp* __restrict__ prog,
+template <bool SPILL>
+__global__ __launch_bounds__(NTH, 5) void k_csg_tiles(const FieldRef* __restrict__ fields, const MOp* __restrict__ prog,
                                                    int nops, int final_slot, int lds_slots, int spill_slots,
                                                    float* __restrict__ arena, int nx, int ny, int nz, int gty, int gtz,
-                                                   int ntiles, const int* __restrict__ starts, float* __restrict__ out,
+                                                   int ntiles, int nlt, int nreads, const int* __restrict__ reads,
+                                                   const int* __restrict__ starts,
+                                                   const int* __restrict__ lists, float* __restrict__ out,
                                                    int* __restrict__ err_nf)
 {
     extern __shared__ float slots[];
@@ -229,48 +407,129 @@ __global__ __launch_bounds__(256) void k_csg_tiles(const FieldRef* __restrict__ 
     __shared__ int bad_tile;
     const int tid = threadIdx.x;
     float* spill = arena + (size_t)blockIdx.x * spill_slots * TS;
-    auto slot = [&](int s) -> float* { return s < lds_slots ? slots + s * TS : spill + (size_t)(s - lds_slots) * TS; };
+    // without spill slots every slot is in LDS, and the accesses compile to
+    // LDS instructions rather than generic ones
+    auto slot = [&](int s) -> float* {
+        if constexpr (SPILL) return s < lds_slots ? slots + s * TS : spill + (size_t)(s - lds_slots) * TS;
+        else return slots + s * TS;
+    };
     const int nty = (ny + 31) >> 5, ntz = (nz + 15) >> 4;
+    const int lane = tid;  // one wave: lane = z offset, rows = y offsets
     if (tid == 0) bad_tile = INT_MAX;
     __syncthreads();
+    // leaf-list words of the first PRE reads, fetched one tile ahead; each
+    // tile then gathers those reads' samples together at its start, so that it
+    // waits for two loads in a row (leaf records, then samples) for all of them
+    constexpr int PRE = 2;
+    const int npre = lists ? min(nreads, PRE) : 0;
+    FieldRef pref[PRE];
+#pragma unroll
+    for (int r = 0; r < PRE; ++r)
+        if (r < npre) pref[r] = fields[reads[r]];
+    int nid[PRE], ncnt[PRE];
+#pragma unroll
+    for (int r = 0; r < PRE; ++r) {
+        nid[r] = 0;
+        ncnt[r] = LCAP;
+    }
+    auto fetch = [&](int t) {
+        const int tz = t % gtz, ty = (t / gtz) % gty, tx = t / (gtz * gty);
+        const int lt = ((tx / LX) * gty + ty) * gtz + tz;
+#pragma unroll
+        for (int r = 0; r < PRE; ++r)
+            if (r < npre) {
+                const int* L = lists + ((int64_t)r * nlt + lt) * LCAP;
+                nid[r] = lane < LCAP - 1 ? L[1 + lane] : 0;
+                ncnt[r] = L[0];
+            }
+    };
+    if ((int)blockIdx.x < ntiles) fetch(blockIdx.x);
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const int tz = t % gtz, ty = (t / gtz) % gty, tx = t / (gtz * gty);
         const int tlo[3] = {tx * TX, ty * TY, tz * TZ};
         const int thi[3] = {min(tlo[0] + TX, nx), min(tlo[1] + TY, ny), min(tlo[2] + TZ, nz)};
+        const int lt = ((tx / LX) * gty + ty) * gtz + tz;  // list tile
+        const int rows = thi[1] - tlo[1];                   // <= TY
+        const bool zin = tlo[2] + lane < thi[2];
+        int cid[PRE], ccnt[PRE];
+#pragma unroll
+        for (int r = 0; r < PRE; ++r) {
+            cid[r] = nid[r];
+            ccnt[r] = __builtin_amdgcn_readfirstlane(ncnt[r]);
+        }
+        if (t + (int)gridDim.x < ntiles) fetch(t + gridDim.x);
+        LeafRec q[PRE];
+#pragma unroll
+        for (int r = 0; r < PRE; ++r)
+            if (r < npre && ccnt[r] < LCAP) q[r] = load_leaves(pref[r], cid[r], ccnt[r]);
+        uint32_t gv[PRE][TY], gm[PRE];
+#pragma unroll
+        for (int r = 0; r < PRE; ++r)
+            gm[r] = r < npre && ccnt[r] < LCAP ? gather_leaves(pref[r], q[r], ccnt[r], tlo, thi, gv[r]) : 0u;
         for (int o = 0; o < nops; ++o) {
             const MOp m = prog[o];
             float* dst = slot(m.dst);
             if (m.kind == 0) {
-                read_tile(fields[m.a], tlo, thi, dst, cur, nxt, leaves, counters,
-                          starts ? starts[(int64_t)m.b * ntiles + t] : 0);
+                if (m.b < npre) {  // uniform
+                    int c = ccnt[0];
+#pragma unroll
+                    for (int r = 1; r < PRE; ++r)
+                        if (m.b == r) c = ccnt[r];
+                    if (c < LCAP) {  // gathered at the tile's start
+#pragma unroll
+                        for (int r = 0; r < PRE; ++r)
+                            if (m.b == r) put_rows(dst, gv[r], gm[r]);
+                        __syncthreads();
+                        continue;
+                    }
+                }
+                const int64_t g = (int64_t)m.b * nlt + lt;
+                read_tile(fields[m.a], tlo, thi, dst, cur, nxt, leaves, counters, starts ? starts[g] : 0,
+                          lists ? lists + g * LCAP : nullptr);
                 continue;
             }
-            // elementwise: dst may be one of the operands' slots
+            // elementwise over every sample of the slots (dst may be one of the
+            // operands' slots); samples outside the grid hold stale values,
+            // computed but never flagged nor stored
             const float* A = slot(m.a);
             const float* Bv = slot(m.b);
-            int bad = INT_MAX;
-            for (int j = tid; j < TS; j += 256) {
-                const int z = j % TZ, y = (j / TZ) % TY, x = j / (TZ * TY);
-                const int gx = tlo[0] + x, gy = tlo[1] + y, gz = tlo[2] + z;
-                if (gx >= thi[0] || gy >= thi[1] || gz >= thi[2]) continue;
-                const float a = A[j];
-                float r;
-                if (m.kind == 3) r = (float)((double)a - m.off);
-                else {
-                    const float b = m.kind == 1 ? Bv[j] : -Bv[j];
-                    r = m.kind == 1 ? (a < b ? a : b) : (a > b ? a : b);
+            uint32_t nf = 0;  // bit k: the sample of row k is not finite
+            if (m.kind == 3) {
+                const double d = m.off;
+#pragma unroll
+                for (int k = 0; k < TY; ++k) {
+                    const float r = (float)((double)A[k * TZ + lane] - d);
+                    dst[k * TZ + lane] = r;
+                    nf |= (uint32_t)!isfinite(r) << k;
                 }
-                if (!isfinite(r)) bad = min(bad, ref_tile(gx, gy, gz, nty, ntz));
-                dst[j] = r;
+            } else if (m.kind == 1) {
+#pragma unroll
+                for (int k = 0; k < TY; ++k) {
+                    const float a = A[k * TZ + lane], b = Bv[k * TZ + lane];
+                    const float r = a < b ? a : b;
+                    dst[k * TZ + lane] = r;
+                    nf |= (uint32_t)!isfinite(r) << k;
+                }
+            } else {
+#pragma unroll
+                for (int k = 0; k < TY; ++k) {
+                    const float a = A[k * TZ + lane], b = -Bv[k * TZ + lane];
+                    const float r = a > b ? a : b;
+                    dst[k * TZ + lane] = r;
+                    nf |= (uint32_t)!isfinite(r) << k;
+                }
             }
-            if (bad != INT_MAX) atomicMin(&bad_tile, bad);
+            nf &= zin ? (1u << rows) - 1u : 0u;
+            // the reference tile index grows with y, so a lane's first row decides
+            if (nf) atomicMin(&bad_tile, ref_tile(tlo[0], tlo[1] + __builtin_ctz(nf), tlo[2] + lane, nty, ntz));
             __syncthreads();
         }
         const float* F = slot(final_slot);
-        for (int j = tid; j < TS; j += 256) {
-            const int z = j % TZ, y = (j / TZ) % TY, x = j / (TZ * TY);
-            const int gx = tlo[0] + x, gy = tlo[1] + y, gz = tlo[2] + z;
-            if (gx < thi[0] && gy < thi[1] && gz < thi[2]) out[((int64_t)gx * ny + gy) * nz + gz] = F[j];
+        if (zin) {
+            float* o = out + ((int64_t)tlo[0] * ny + tlo[1]) * nz + tlo[2] + lane;
+#pragma unroll
+            for (int k = 0; k < TY; ++k)
+                if (k < rows) o[(int64_t)k * nz] = F[k * TZ + lane];
         }
         __syncthreads();  // the next tile overwrites the slots
     }
@@ -480,7 +739,7 @@ int compile_program(int32_t fields, const Instr24* program, int32_t n, const std
     return nslots;
 }
 
-constexpr int LDS_SLOTS_MAX = 16;  // 128 KiB of slots (+ 12 KiB of tree lists) per workgroup
+constexpr int LDS_SLOTS_MAX = 16;  // 32 KiB of slots (+ 3 KiB of tree lists) per workgroup
 constexpr int LDS_SLOTS_SPILL = 8;  // LDS slots of a program that spills to the arena
 
 size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
@@ -489,11 +748,15 @@ size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
 struct Plan {
     std::vector<MOp> ops;
     std::vector<int> reads;  // field of each read micro-op, in read order
-    int nslots = 0, final_slot = 0, lds_slots = 0, spill = 0, grid = 0, ntiles = 0, gty = 0, gtz = 0;
+    int nslots = 0, final_slot = 0, lds_slots = 0, spill = 0, grid = 0, ntiles = 0, nlt = 0, gty = 0, gtz = 0;
     bool starts = false;     // the start-node map fits (CSG_STARTS_MAX bytes)
-    size_t lds = 0, ws_refs = 0, ws_ops = 0, ws_flags = 0, ws_reads = 0, ws_starts = 0, ws_arena = 0, ws_total = 0;
+    bool lists = false;      // the leaf lists fit (CSG_LISTS_MAX bytes)
+    size_t lds = 0, ws_refs = 0, ws_ops = 0, ws_flags = 0, ws_reads = 0, ws_starts = 0, ws_lists = 0, ws_arena = 0,
+           ws_total = 0;
 };
 constexpr size_t CSG_STARTS_MAX = size_t(1) << 28;  // start-node map bytes; beyond, reads start at the root
+constexpr size_t CSG_LISTS_MAX = size_t(1) << 28;   // leaf-list bytes (LCAP words per read and list tile); beyond,
+                                                    // every read walks the tree
 
 hipError_t make_plan(int32_t fields, const Instr24* program, int32_t n, const std::vector<int>& last,
                      const int32_t* shape, Plan& P)
@@ -509,15 +772,16 @@ hipError_t make_plan(int32_t fields, const Instr24* program, int32_t n, const st
     const int64_t nt = (int64_t)gtx * P.gty * P.gtz;
     if (nt > INT_MAX) return hipErrorInvalidValue;
     P.ntiles = (int)nt;
+    P.nlt = (int)((int64_t)((shape[0] + LX - 1) / LX) * P.gty * P.gtz);
     int dev = 0, cus = 0, per_cu = 0;
     hipError_t e;
     if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
     if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess) return e;
-    if (P.lds > 65536 &&
-        (e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_csg_tiles), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                 (int)P.lds)) != hipSuccess)
+    const void* fn = P.spill ? reinterpret_cast<const void*>(&k_csg_tiles<true>)
+                             : reinterpret_cast<const void*>(&k_csg_tiles<false>);
+    if (P.lds > 65536 && (e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)P.lds)) != hipSuccess)
         return e;
-    if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_csg_tiles, 256, P.lds)) != hipSuccess) return e;
+    if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, NTH, P.lds)) != hipSuccess) return e;
     P.grid = (int)std::min<int64_t>(nt, (int64_t)std::max(cus, 1) * std::max(per_cu, 1));
     P.ws_refs = align256(sizeof(FieldRef) * fields);
     P.ws_ops = align256(sizeof(MOp) * P.ops.size());
@@ -529,10 +793,13 @@ hipError_t make_plan(int32_t fields, const Instr24* program, int32_t n, const st
             P.reads.push_back(m.a);
         }
     P.ws_reads = align256(sizeof(int) * std::max<size_t>(P.reads.size(), 1));
-    const size_t map = (size_t)4 * P.reads.size() * (size_t)P.ntiles;
+    const size_t map = (size_t)4 * P.reads.size() * (size_t)P.nlt;
     P.starts = map <= CSG_STARTS_MAX;
     P.ws_starts = P.starts ? align256(map) : 0;
-    P.ws_total = P.ws_refs + P.ws_ops + P.ws_flags + P.ws_reads + P.ws_starts + P.ws_arena;
+    const size_t lmap = map * LCAP;
+    P.lists = !P.reads.empty() && P.reads.size() <= 65535 && lmap <= CSG_LISTS_MAX;
+    P.ws_lists = P.lists ? align256(lmap) : 0;
+    P.ws_total = P.ws_refs + P.ws_ops + P.ws_flags + P.ws_reads + P.ws_starts + P.ws_lists + P.ws_arena;
     return hipSuccess;
 }
 
@@ -546,22 +813,41 @@ int run_plan(const Plan& P, const std::vector<FieldRef>& refs, const int32_t* sh
     int* flag = reinterpret_cast<int*>(ws + P.ws_refs + P.ws_ops);
     int* dreads = reinterpret_cast<int*>(ws + P.ws_refs + P.ws_ops + P.ws_flags);
     int* dstarts = P.starts ? reinterpret_cast<int*>(ws + P.ws_refs + P.ws_ops + P.ws_flags + P.ws_reads) : nullptr;
-    float* arena = reinterpret_cast<float*>(ws + P.ws_refs + P.ws_ops + P.ws_flags + P.ws_reads + P.ws_starts);
+    int* dlists = P.lists ? reinterpret_cast<int*>(ws + P.ws_refs + P.ws_ops + P.ws_flags + P.ws_reads + P.ws_starts)
+                          : nullptr;
+    float* arena =
+        reinterpret_cast<float*>(ws + P.ws_refs + P.ws_ops + P.ws_flags + P.ws_reads + P.ws_starts + P.ws_lists);
     const int none = INT_MAX;
     int nf = INT_MAX;
     FE_TRY(hipMemcpyAsync(dref, refs.data(), sizeof(FieldRef) * refs.size(), hipMemcpyHostToDevice, st));
     FE_TRY(hipMemcpyAsync(dops, P.ops.data(), sizeof(MOp) * P.ops.size(), hipMemcpyHostToDevice, st));
     FE_TRY(hipMemcpyAsync(flag, &none, sizeof none, hipMemcpyHostToDevice, st));
-    if (dstarts && !P.reads.empty()) {
+    const int64_t n = (int64_t)P.reads.size() * P.nlt;
+    if ((dstarts || dlists) && n > 0)
         FE_TRY(hipMemcpyAsync(dreads, P.reads.data(), sizeof(int) * P.reads.size(), hipMemcpyHostToDevice, st));
-        const int64_t n = (int64_t)P.reads.size() * P.ntiles;
-        hipLaunchKernelGGL(k_csg_starts, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, dref, dreads,
-                           (int)P.reads.size(), shape[0], shape[1], shape[2], P.gty, P.gtz, P.ntiles, dstarts);
+    if (dlists && n > 0) {
+        int64_t most = 1;
+        for (int f : P.reads) most = std::max(most, refs[f].count);
+        hipLaunchKernelGGL(k_csg_list_clear, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, dlists, n);
+        hipLaunchKernelGGL(k_csg_lists, dim3((unsigned)((most + 3) / 4), (unsigned)P.reads.size()), dim3(256), 0,
+                           st, dref, dreads, shape[0], shape[1], shape[2], P.gty, P.gtz, P.nlt, dlists);
         FE_TRY(hipGetLastError());
     }
-    hipLaunchKernelGGL(k_csg_tiles, dim3((unsigned)P.grid), dim3(256), P.lds, st, dref, dops, (int)P.ops.size(),
-                       P.final_slot, P.lds_slots, P.spill, arena, shape[0], shape[1], shape[2], P.gty, P.gtz, P.ntiles,
-                       dstarts, output, flag);
+    if (dstarts && n > 0) {
+        hipLaunchKernelGGL(k_csg_starts, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, dref, dreads,
+                           (int)P.reads.size(), shape[0], shape[1], shape[2], P.gty, P.gtz, P.nlt, dlists, dstarts);
+        FE_TRY(hipGetLastError());
+    }
+    if (P.spill)
+        hipLaunchKernelGGL(k_csg_tiles<true>, dim3((unsigned)P.grid), dim3(NTH), P.lds, st, dref, dops,
+                           (int)P.ops.size(), P.final_slot, P.lds_slots, P.spill, arena, shape[0], shape[1], shape[2],
+                           P.gty, P.gtz, P.ntiles, P.nlt, (int)P.reads.size(), dreads, dstarts, dlists, output,
+                           flag);
+    else
+        hipLaunchKernelGGL(k_csg_tiles<false>, dim3((unsigned)P.grid), dim3(NTH), P.lds, st, dref, dops,
+                           (int)P.ops.size(), P.final_slot, P.lds_slots, P.spill, arena, shape[0], shape[1], shape[2],
+                           P.gty, P.gtz, P.ntiles, P.nlt, (int)P.reads.size(), dreads, dstarts, dlists, output,
+                           flag);
     FE_TRY(hipGetLastError());
     FE_TRY(hipMemcpyAsync(&nf, flag, sizeof nf, hipMemcpyDeviceToHost, st));
     FE_TRY(hipStreamSynchronize(st));

@@ -236,3 +236,77 @@ def test_tile_reader_codes_and_bits(fe, gpu):
         c2 = lib.csg_gather(nodes.ctypes.data, len(nodes), f.payload.ctypes.data, len(f.payload), lo.ctypes.data,
                             hi.ctypes.data, o2.ctypes.data)
         assert c1 == c2 and c1 != 0, mode
+
+
+def box_table(a, leaf):
+    """A valid leaf-box table over `a`: bisection of the longest axis down to
+    boxes of at most `leaf` samples, in pre-order; a leaf broadcasts every
+    axis its samples are constant along and stores the rest."""
+    rows, pay, used = [], [], [0]
+
+    def build(lo, sh):
+        i = len(rows)
+        rows.append(None)
+        if sh[0] * sh[1] * sh[2] <= leaf:
+            sub = a[lo[0]:lo[0] + sh[0], lo[1]:lo[1] + sh[1], lo[2]:lo[2] + sh[2]]
+            st = list(sh)
+            for ax in range(3):
+                first = np.take(sub, [0], axis=ax)
+                if np.array_equal(sub.view(np.uint32), np.broadcast_to(first, sub.shape).view(np.uint32)):
+                    st[ax] = 1
+                    sub = first
+            pay.append(np.ascontiguousarray(sub).ravel())
+            rows[i] = (lo, sh, st, (-1, -1), used[0])
+            used[0] += pay[-1].size
+            return i
+        ax = int(np.argmax(sh))
+        ls, rs, rl = list(sh), list(sh), list(lo)
+        ls[ax] = sh[ax] // 2
+        rs[ax] -= ls[ax]
+        rl[ax] += ls[ax]
+        c0 = build(lo, ls)
+        c1 = build(rl, rs)
+        rows[i] = (lo, sh, [0, 0, 0], (c0, c1), -1)
+        return i
+
+    build([0, 0, 0], list(a.shape))
+    nodes = np.zeros(len(rows), O.NODE)
+    for i, (lo, sh, st, ch, off) in enumerate(rows):
+        nodes[i] = (lo, sh, st, ch, off)
+    return nodes, np.concatenate(pay).astype(np.float32)
+
+
+@pytest.mark.parametrize("leaf", [1, 8, 32, 256, 1 << 20])
+def test_csg_every_tile_read_path(fe, gpu, leaf):
+    """Leaf sizes that send a tile's operand reads down every path of the
+    device reader: a few leaves per 4x8x64 list tile, gathered at the tile's
+    start (256 and one leaf for the whole grid), lists that overflow their 63
+    entries so the read walks the tree from its start node (32 and 8; at 8
+    the walk's leaves still fit the workgroup's list), and walks that overflow
+    that list into the per-sample descent (1); three operands, so one read
+    always comes after the gathered ones, and odd extents leave partial tiles
+    on every axis.  Device and oracle run the same tables."""
+    rng = np.random.default_rng(1000 + leaf)
+    shape = (9, 21, 133)
+    arrays = [rng.normal(size=shape).astype(np.float32) for _ in range(3)]
+    arrays[1][:, :, 40:90] = 0.5  # constant boxes broadcast every axis
+    arrays[2][:, 3:11, :] = arrays[2][:, 3:4, :]  # boxes constant along y
+    prog = encode([("union", 0, 1), ("difference", 3, 2), ("offset", 4, 0.25)])
+    tables = [box_table(a, leaf) for a in arrays]
+    code, want = O.csg_execute(tables, prog, shape)
+    assert code == 0
+    # the program in closed form (ties give the second operand)
+    a, b, c = arrays
+    u = np.where(a < b, a, b)
+    d = np.where(u > -c, u, -c)
+    assert want.tobytes() == (d.astype(np.float64) - 0.25).astype(np.float32).tobytes()
+    got = np.empty(shape, np.float32)
+    lib = fe._lib.lib()
+    nodes = (ct.c_void_p * 3)(*[t[0].ctypes.data for t in tables])
+    pays = (ct.c_void_p * 3)(*[t[1].ctypes.data for t in tables])
+    counts = np.array([len(t[0]) for t in tables], np.int64)
+    lens = np.array([len(t[1]) for t in tables], np.int64)
+    sh = np.array(shape, np.int32)
+    assert lib.csg_execute(nodes, counts.ctypes.data, pays, lens.ctypes.data, 3, sh.ctypes.data, prog.ctypes.data,
+                           len(prog), got.ctypes.data) == 0
+    assert got.tobytes() == want.tobytes(), leaf
