@@ -523,10 +523,20 @@ __global__ __launch_bounds__(256) void k_query_points(const FieldDesc* __restric
     int nl;
     query_list(D, x, y, z, list, nl);
     float f = 0.0f;
-    for (int j = 0; j < nl; ++j) {
-        const Prim p = D.prims[list ? list[j] : j];
-        const float v = prim_eval(p.kind, p.cx, p.cy, p.cz, p.a, p.b, p.c, x, y, z);
-        f = j == 0 ? v : fold(p.op, f, v);
+    if (nl > 0) {
+        // the next record and the index after it are in flight while a
+        // primitive evaluates (a thread's list is one chain of loads)
+        int inext = nl > 1 ? (list ? list[1] : 1) : 0;
+        Prim pn = D.prims[list ? list[0] : 0];
+        for (int j = 0; j < nl; ++j) {
+            const Prim p = pn;
+            if (j + 1 < nl) {
+                pn = D.prims[inext];
+                if (j + 2 < nl) inext = list ? list[j + 2] : j + 2;
+            }
+            const float v = prim_eval(p.kind, p.cx, p.cy, p.cz, p.a, p.b, p.c, x, y, z);
+            f = j == 0 ? v : fold(p.op, f, v);
+        }
     }
     out[i] = f;
 }
@@ -539,7 +549,11 @@ __global__ __launch_bounds__(256) void k_query_points(const FieldDesc* __restric
 // lanes of scattered points would step through every kind of formula and
 // gather their own records.  Values are bit-identical to k_query_points.
 #define FE_QBINS 8192
-#define FE_QSORT_MIN 16384  // fewer points: one thread per point
+// Fewer points: one thread per point.  A wave then runs one list per lane at
+// once, while sorted waves run each of their lists in turn, which pays only
+// where many points share a superblock (a 512^3 part's 2^20 uniform points:
+// 0.47 ms sorted against 1.06 ms; 10^5 points: 0.46 against 0.32 ms).
+#define FE_QSORT_MIN (1 << 18)
 
 __global__ __launch_bounds__(256) void k_qbin_zero(int32_t* __restrict__ cnt)
 {

@@ -142,8 +142,8 @@ def test_list_fallbacks(fe, gpu, digests, name):
     check(sparse_digests(fe.klassificera_och_evaluera_recept(meta, rec)), ref, name + " plain")
     # the server's query kernel falls back to the whole recipe where the list overflowed
     import oracle as O
-    # (20,000 points: the superblock-sorted query path; 4096: one thread per point)
-    pts = np.random.default_rng(9).uniform(30.0, 66.0, (20000, 3)).astype(np.float32)
+    # (300,000 points: the superblock-sorted query path; 4096: one thread per point)
+    pts = np.random.default_rng(9).uniform(30.0, 66.0, (300000, 3)).astype(np.float32)
     import torch
     f = fe.server.resident_fields(plan, torch.cuda.current_stream())[0]
     got = f.query_device(torch.from_numpy(pts).to(gpu)).cpu().numpy()
@@ -212,8 +212,8 @@ def test_cfg5_served_under_poisson(fe, gpu, digests):
         for k, rid in enumerate(rids):
             check(sparse_digests(srv.result(rid).sparse_field()), ref[k], f"cfg5 request {k}")
         rng = np.random.default_rng(55)
-        for k in (0, len(rids) - 1):
-            pts = rng.uniform(0.0, 511.0, (20000, 3)).astype(np.float32)
+        for k, npts in ((0, 300000), (len(rids) - 1, 20000)):  # sorted and per-thread query paths
+            pts = rng.uniform(0.0, 511.0, (npts, 3)).astype(np.float32)
             got = srv.query(rids[k], pts)
             assert got.tobytes() == O.recipe_eval_points(np.ascontiguousarray(recs[k]).view(O.PRIM), pts).tobytes()
     finally:
