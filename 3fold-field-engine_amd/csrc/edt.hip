@@ -868,13 +868,16 @@ __global__ __launch_bounds__(W* T, CMAX <= 16 ? 8 : (CMAX <= 24 ? 6 : 4)) void k
         int k = -1, s_top = 0, s_sec = 0;
         int32_t F_top = 0, F_sec = 0;
         uint32_t* top = stk + me - NT;  // stack slot k lives at stk[k * NT + me]
-#pragma unroll
-        for (int i = 0; i < CMAX; ++i) {
+        // site i of the chunk onto the stack.  CLEAN: every position of the
+        // chunk is a site (a full chunk without the sentinel), so no site test
+        // is needed, and from the third site on the stack holds two entries
+        auto add = [&](const int i, auto clean) {
+            constexpr bool CLEAN = decltype(clean)::value;
             const int q = p0 + i;
             const int32_t fq = v[i];  // the sentinel past the chunk end
-            if (fq < sentinel) {
+            if (CLEAN || fq < sentinel) {
                 const int32_t Fq = fq + (int32_t)__umul24((uint32_t)q, (uint32_t)q);
-                if (k >= 1 && !above<true>(Fq - F_top, s_top - s_sec, F_top - F_sec, q - s_top)) {
+                if ((CLEAN ? i >= 2 : k >= 1) && !above<true>(Fq - F_top, s_top - s_sec, F_top - F_sec, q - s_top)) {
                     do {  // pops are rare: the loop stays off the common path
                         --k;
                         top -= NT;
@@ -895,6 +898,13 @@ __global__ __launch_bounds__(W* T, CMAX <= 16 ? 8 : (CMAX <= 24 ? 6 : 4)) void k
                 s_top = q;
                 F_top = Fq;
             }
+        };
+        if (__all(np == CMAX && mx < sentinel)) {  // wave-uniform
+#pragma unroll
+            for (int i = 0; i < CMAX; ++i) add(i, std::true_type{});
+        } else {
+#pragma unroll
+            for (int i = 0; i < CMAX; ++i) add(i, std::false_type{});
         }
         const int m = k + 1;
         uint32_t inf = (uint32_t)m;
