@@ -75,11 +75,12 @@ __global__ __launch_bounds__(256) void k_pack_inv(PNode* __restrict__ arena, con
 // uint16 (axes <= 4096).  z: a wave per (x, y) row, 64 z per ballot, rows
 // walked from the end with the next break carried.
 __global__ __launch_bounds__(256) void k_runs_z(const uint32_t* __restrict__ d, int nx, int ny, int nz,
-                                                uint16_t* __restrict__ rz)
+                                                uint16_t* __restrict__ rz, unsigned long long* __restrict__ nonfinite)
 {
     const int lane = threadIdx.x & 63;
     const int64_t rows = (int64_t)nx * ny;
     const int64_t nw = (int64_t)gridDim.x * 4;
+    bool bad = false;  // a non-finite sample (this pass reads every sample once)
     for (int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); row < rows; row += nw) {
         const uint32_t* line = d + row * nz;
         uint16_t* out = rz + row * nz;
@@ -87,7 +88,11 @@ __global__ __launch_bounds__(256) void k_runs_z(const uint32_t* __restrict__ d, 
         for (int c0 = ((nz - 1) >> 6) << 6; c0 >= 0; c0 -= 64) {
             const int z = c0 + lane;
             bool brk = false;
-            if (z < nz) brk = z == nz - 1 || line[z] != line[z + 1];
+            if (z < nz) {
+                const uint32_t v = line[z];
+                bad = bad || (v & 0x7f800000u) == 0x7f800000u;
+                brk = z == nz - 1 || v != line[z + 1];
+            }
             const uint64_t m = __ballot(brk);
             const uint64_t mine = lane ? m >> lane : m;
             const int nb = mine ? z + __builtin_ctzll(mine) : carry;
@@ -95,6 +100,7 @@ __global__ __launch_bounds__(256) void k_runs_z(const uint32_t* __restrict__ d, 
             if (m) carry = c0 + __builtin_ctzll(m);
         }
     }
+    if (__ballot(bad) && lane == 0) atomicOr(nonfinite, 1ull);
 }
 
 // y and x: a thread per column, walked from the end (adjacent threads hold
@@ -137,17 +143,18 @@ __global__ __launch_bounds__(256) void k_pack_inv_faces(PNode* __restrict__ aren
         const int64_t b = cnt * part / S, e = cnt * (part + 1) / S;
         const int64_t o = (int64_t)P.lo[0] * sx + (int64_t)P.lo[1] * sy + P.lo[2];
         bool ex = true, ey = true, ez = true;
-        for (int64_t l = b + threadIdx.x; l < e; l += 256) {
-            if (l < fz) {
-                const int y = (int)(l % hy), x = (int)(l / hy);
+        // face positions stay below 3 * 4096^2 < 2^31: 32-bit division
+        for (uint32_t l = (uint32_t)b + threadIdx.x; l < (uint32_t)e; l += 256) {
+            if (l < (uint32_t)fz) {
+                const uint32_t x = l / (uint32_t)hy, y = l - x * (uint32_t)hy;
                 ez = ez && rz[o + x * sx + y * sy] >= hz;
-            } else if (l < fz + fy) {
-                const int64_t k = l - fz;
-                const int z = (int)(k % hz), x = (int)(k / hz);
+            } else if (l < (uint32_t)(fz + fy)) {
+                const uint32_t k = l - (uint32_t)fz;
+                const uint32_t x = k / (uint32_t)hz, z = k - x * (uint32_t)hz;
                 ey = ey && ry[o + x * sx + z] >= hy;
             } else {
-                const int64_t k = l - fz - fy;
-                const int z = (int)(k % hz), y = (int)(k / hz);
+                const uint32_t k = l - (uint32_t)(fz + fy);
+                const uint32_t y = k / (uint32_t)hz, z = k - y * (uint32_t)hz;
                 ex = ex && rx[o + y * sy + z] >= hx;
             }
         }
@@ -257,11 +264,36 @@ __global__ __launch_bounds__(256) void k_pack_top(PNode* __restrict__ arena, con
     }
 }
 
+// Copies rows (x, y) of a final leaf's stored box into the payload: lanes run
+// along z in groups of the next power of two >= the stored z extent (at most
+// 256), the groups of workgroup slot g (of gstep) over the rows, so a lane
+// divides once per row it copies (stored extents <= 4096).
+__device__ __forceinline__ void emit_rows(const PNode& P, uint32_t* __restrict__ payload, const uint32_t* __restrict__ d,
+                                          int64_t sx, int64_t sy, uint32_t g, uint32_t gstep)
+{
+    const uint32_t s1 = (uint32_t)P.stored[1], s2 = (uint32_t)P.stored[2];
+    const uint32_t rows = (uint32_t)P.stored[0] * s1;
+    const int lg = s2 >= 256 ? 8 : (s2 <= 1 ? 0 : 32 - __clz(s2 - 1));
+    const uint32_t zl = threadIdx.x & ((1u << lg) - 1u), G = 256u >> lg;
+    const int64_t o = (int64_t)P.lo[0] * sx + (int64_t)P.lo[1] * sy + P.lo[2];
+    uint32_t* dst = payload + P.off;
+    for (uint32_t r = g * G + (threadIdx.x >> lg); r < rows; r += gstep * G) {
+        const uint32_t x = r / s1, y = r - x * s1;
+        const uint32_t* src = d + o + x * sx + y * sy;
+        for (uint32_t z = zl; z < s2; z += 1u << lg) dst[(int64_t)r * s2 + z] = src[z];
+    }
+}
+
+#define EMIT_BIG 4096  // stored samples: larger leaves are copied by k_pack_emit_big
+
 // Writes the 52-byte record (as 13 words) of every node of the final tree in
-// one level and copies each final leaf's stored samples into the payload.
+// one level and copies each final leaf's stored samples into the payload (a
+// leaf of more than EMIT_BIG stored samples is listed in big, counted by
+// *nbig, for k_pack_emit_big).
 __global__ __launch_bounds__(256) void k_pack_emit(const PNode* __restrict__ arena, const int64_t* __restrict__ lv,
                                                    uint32_t* __restrict__ rec, uint32_t* __restrict__ payload,
-                                                   const uint32_t* __restrict__ d, int64_t sx, int64_t sy)
+                                                   const uint32_t* __restrict__ d, int64_t sx, int64_t sy,
+                                                   int64_t* __restrict__ big, unsigned long long* __restrict__ nbig)
 {
     const int64_t start = lv[0], n = lv[1];
     for (int64_t i = blockIdx.x; i < n; i += gridDim.x) {
@@ -285,17 +317,27 @@ __global__ __launch_bounds__(256) void k_pack_emit(const PNode* __restrict__ are
             }
             r[11] = (uint32_t)((uint64_t)off & 0xffffffffu);
             r[12] = (uint32_t)((uint64_t)off >> 32);
+            if (!P.split && P.reduced > EMIT_BIG) big[atomicAdd(nbig, 1ull)] = start + i;
         }
-        if (P.split) continue;
-        const int s1 = P.stored[1], s2 = P.stored[2];
-        const int64_t o = (int64_t)P.lo[0] * sx + (int64_t)P.lo[1] * sy + P.lo[2];
-        for (int64_t j = threadIdx.x; j < P.reduced; j += 256) {
-            const int z = (int)(j % s2);
-            const int64_t r = j / s2;
-            const int y = (int)(r % s1), x = (int)(r / s1);
-            payload[P.off + j] = d[o + x * sx + y * sy + z];
-        }
+        if (P.split || P.reduced > EMIT_BIG) continue;
+        emit_rows(P, payload, d, sx, sy, 0, 1);
     }
+}
+
+// The listed large leaves: K = grid / n workgroups per leaf (at least one),
+// so a few huge leaves spread over the whole grid and many moderate ones run
+// side by side.
+__global__ __launch_bounds__(256) void k_pack_emit_big(const PNode* __restrict__ arena, const int64_t* __restrict__ big,
+                                                       const unsigned long long* __restrict__ nbig,
+                                                       uint32_t* __restrict__ payload, const uint32_t* __restrict__ d,
+                                                       int64_t sx, int64_t sy)
+{
+    const unsigned long long n = *nbig;
+    if (n == 0) return;
+    const uint32_t K = (uint32_t)max(1ull, (unsigned long long)gridDim.x / n);
+    const uint32_t lanes = gridDim.x / K;  // leaves in flight at once
+    for (unsigned long long i = blockIdx.x / K; i < n && blockIdx.x / K < lanes; i += lanes)
+        emit_rows(arena[big[i]], payload, d, sx, sy, blockIdx.x % K, K);
 }
 
 __global__ __launch_bounds__(256) void k_nonfinite(const uint32_t* __restrict__ d, int64_t n,
@@ -375,18 +417,23 @@ Packed* pack_device(const float* data, int nx, int ny, int nz, int leaf, hipStre
         const int64_t cap = attempt == 0 ? 4 * pieces + 1024 : 2 * (int64_t)depth * pieces + 1024;
         const size_t lv_bytes = (size_t)(depth + 2) * 16, ctl_bytes = 32;
         const size_t nodes_bytes = ((size_t)cap * sizeof(PNode) + 255) & ~(size_t)255;
+        // leaves above EMIT_BIG stored samples are disjoint: at most N / EMIT_BIG
+        const size_t big_bytes = ((size_t)(N / EMIT_BIG + 2) * 8 + 255) & ~(size_t)255;
         // run lengths (2 bytes per sample and axis) when they fit; else each
         // level scans its nodes' samples
         const bool runs = (size_t)6 * (size_t)N <= PACK_RUNS_MAX;
         char* base = nullptr;
-        if (hipError_t e = pack_arena(ctl_bytes + lv_bytes + nodes_bytes + (runs ? (size_t)6 * N : 0), &base)) {
+        if (hipError_t e = pack_arena(ctl_bytes + lv_bytes + nodes_bytes + big_bytes + (runs ? (size_t)6 * N : 0),
+                                      &base)) {
             fe_note_error(e, "pack arena");
             return nullptr;
         }
         unsigned long long* ctl = reinterpret_cast<unsigned long long*>(base);  // [0] cursor [1] overflow [2] non-finite
+                                                                                 // [3] large leaves listed
         int64_t* lv = reinterpret_cast<int64_t*>(base + ctl_bytes);            // per level {start, count}
         PNode* arena = reinterpret_cast<PNode*>(base + ctl_bytes + lv_bytes);
-        uint16_t* rz = runs ? reinterpret_cast<uint16_t*>(base + ctl_bytes + lv_bytes + nodes_bytes) : nullptr;
+        int64_t* big = reinterpret_cast<int64_t*>(base + ctl_bytes + lv_bytes + nodes_bytes);
+        uint16_t* rz = runs ? reinterpret_cast<uint16_t*>(base + ctl_bytes + lv_bytes + nodes_bytes + big_bytes) : nullptr;
         uint16_t* ry = runs ? rz + N : nullptr;
         uint16_t* rx = runs ? ry + N : nullptr;
         PNode root;
@@ -412,12 +459,13 @@ Packed* pack_device(const float* data, int nx, int ny, int nz, int leaf, hipStre
             !ok(hipMemcpyAsync(lv, lv0.data(), lv0.size() * 8, hipMemcpyHostToDevice, st), "pack init") ||
             !ok(hipMemcpyAsync(arena, &root, sizeof root, hipMemcpyHostToDevice, st), "pack init"))
             return nullptr;
-        hipLaunchKernelGGL(k_nonfinite, dim3((unsigned)std::min<int64_t>((N + 255) / 256, 4096)), dim3(256), 0, st, d, N,
-                           ctl + 2);
         const unsigned grid_nodes = 2048;
-        if (runs) {
+        if (!runs)
+            hipLaunchKernelGGL(k_nonfinite, dim3((unsigned)std::min<int64_t>((N + 255) / 256, 4096)), dim3(256), 0, st, d,
+                               N, ctl + 2);
+        if (runs) {  // the z run lengths also flag non-finite samples
             hipLaunchKernelGGL(k_runs_z, dim3((unsigned)std::min<int64_t>(((int64_t)nx * ny + 3) / 4, 65536)), dim3(256),
-                               0, st, d, nx, ny, nz, rz);
+                               0, st, d, nx, ny, nz, rz, ctl + 2);
             const int64_t cy = (int64_t)nx * nz, cx = (int64_t)ny * nz;
             hipLaunchKernelGGL(k_runs_yx, dim3((unsigned)std::min<int64_t>((cy + 255) / 256, 65536)), dim3(256), 0, st, d,
                                nx, ny, nz, sx, sy, ry);
@@ -425,12 +473,13 @@ Packed* pack_device(const float* data, int nx, int ny, int nz, int leaf, hipStre
                                ny, nx, nz, sy, sx, rx);
         }
         for (int l = 0; l < depth; ++l) {
-            // parts per node: about 16 K samples each at this level's node
-            // size (its faces' samples with run lengths)
+            // parts per node: about 2 K face samples (with run lengths; else
+            // 16 K samples) each at this level's node size, so the top levels'
+            // few large nodes still fill the GPU
             const int64_t per = std::max<int64_t>(1, N >> std::min(l, 62));
             if (runs) {
                 const double face = 3.0 * std::pow((double)per, 2.0 / 3.0);
-                const int S = (int)std::min(4096.0, std::max(1.0, std::ceil(face / 16384.0)));
+                const int S = (int)std::min(4096.0, std::max(1.0, std::ceil(face / 2048.0)));
                 hipLaunchKernelGGL(k_pack_inv_faces, dim3(grid_nodes * 4), dim3(256), 0, st, arena, lv + 2 * l, S, rx, ry,
                                    rz, sx, sy);
             } else {
@@ -467,7 +516,9 @@ Packed* pack_device(const float* data, int nx, int ny, int nz, int leaf, hipStre
             hipLaunchKernelGGL(k_pack_top, dim3(grid_nodes), dim3(256), 0, st, arena, lv + 2 * l);
         for (int l = 0; l < depth; ++l)
             hipLaunchKernelGGL(k_pack_emit, dim3(grid_nodes * 4), dim3(256), 0, st, arena, lv + 2 * l, out->nodes,
-                               out->payload, d, sx, sy);
+                               out->payload, d, sx, sy, big, ctl + 3);
+        hipLaunchKernelGGL(k_pack_emit_big, dim3(grid_nodes), dim3(256), 0, st, arena, big, ctl + 3, out->payload, d,
+                           sx, sy);
         if (!ok(hipGetLastError(), "pack emit") || !ok(hipStreamSynchronize(st), "pack emit")) {
             delete out;
             return nullptr;
@@ -578,18 +629,25 @@ __global__ __launch_bounds__(256) void k_query(const Node52* __restrict__ nodes,
 
 // Writes voxels [j0, j1) (z fastest) of a leaf's clipped box (hx, hy, hz),
 // threads striding by `step`.
-__device__ __forceinline__ void unpack_range(const Node52& nd, const uint32_t* __restrict__ payload, int hy, int hz,
-                                             int ny, int nz, int64_t j0, int64_t j1, int64_t step,
-                                             uint32_t* __restrict__ dense)
+// Rows (x, y) of a leaf's box (clipped to hx x hy x hz) to the dense grid:
+// lanes run along z in groups of the next power of two >= hz (at most 256),
+// the groups of workgroup slot g (of gstep) over the rows, so a lane divides
+// once per row it writes (every extent <= 4096).
+__device__ __forceinline__ void unpack_rows(const Node52& nd, const uint32_t* __restrict__ payload, int hx, int hy,
+                                            int hz, int ny, int nz, uint32_t g, uint32_t gstep,
+                                            uint32_t* __restrict__ dense)
 {
     const int64_t off = node_offset(nd);
-    for (int64_t j = j0; j < j1; j += step) {
-        const int z = (int)(j % hz);
-        const int64_t r = j / hz;
-        const int y = (int)(r % hy), x = (int)(r / hy);
-        const int64_t sx = nd.stored[0] == 1 ? 0 : x, sy = nd.stored[1] == 1 ? 0 : y, sz = nd.stored[2] == 1 ? 0 : z;
-        dense[((int64_t)(nd.lo[0] + x) * ny + nd.lo[1] + y) * nz + nd.lo[2] + z] =
-            payload[off + (sx * nd.stored[1] + sy) * nd.stored[2] + sz];
+    const int lg = hz >= 256 ? 8 : (hz <= 1 ? 0 : 32 - __clz(hz - 1));
+    const uint32_t zl = threadIdx.x & ((1u << lg) - 1u), G = 256u >> lg;
+    const uint32_t rows = (uint32_t)hx * (uint32_t)hy;
+    const int s1 = nd.stored[1], s2 = nd.stored[2];
+    const bool bx = nd.stored[0] == 1, by = s1 == 1, bz = s2 == 1;
+    for (uint32_t r = g * G + (threadIdx.x >> lg); r < rows; r += gstep * G) {
+        const uint32_t x = r / (uint32_t)hy, y = r - x * (uint32_t)hy;
+        uint32_t* dst = dense + ((int64_t)(nd.lo[0] + (int)x) * ny + nd.lo[1] + (int)y) * nz + nd.lo[2];
+        const uint32_t* src = payload + off + ((int64_t)(bx ? 0 : x) * s1 + (by ? 0 : y)) * s2;
+        for (uint32_t z = zl; z < (uint32_t)hz; z += 1u << lg) dst[z] = src[bz ? 0 : z];
     }
 }
 
@@ -613,7 +671,7 @@ __global__ __launch_bounds__(256) void k_unpack(const Node52* __restrict__ nodes
         if (threadIdx.x == 0) big[1 + atomicAdd(big, 1u)] = (unsigned int)id;
         return;
     }
-    unpack_range(nd, payload, hy, hz, ny, nz, threadIdx.x, cnt, 256, dense);
+    unpack_rows(nd, payload, hx, hy, hz, ny, nz, 0, 1, dense);
 }
 
 // The listed large leaves, one after another, each spread over every block.
@@ -626,9 +684,7 @@ __global__ __launch_bounds__(256) void k_unpack_big(const Node52* __restrict__ n
         const Node52& nd = nodes[big[1 + i]];
         const int hx = min(nd.shape[0], nx - nd.lo[0]), hy = min(nd.shape[1], ny - nd.lo[1]),
                   hz = min(nd.shape[2], nz - nd.lo[2]);
-        const int64_t cnt = (int64_t)hx * hy * hz;
-        unpack_range(nd, payload, hy, hz, ny, nz, (int64_t)blockIdx.x * 256 + threadIdx.x, cnt,
-                     (int64_t)gridDim.x * 256, dense);
+        unpack_rows(nd, payload, hx, hy, hz, ny, nz, blockIdx.x, gridDim.x, dense);
     }
 }
 

@@ -310,3 +310,25 @@ def test_csg_every_tile_read_path(fe, gpu, leaf):
     assert lib.csg_execute(nodes, counts.ctypes.data, pays, lens.ctypes.data, 3, sh.ctypes.data, prog.ctypes.data,
                            len(prog), got.ctypes.data) == 0
     assert got.tobytes() == want.tobytes(), leaf
+
+
+@pytest.mark.parametrize("leaf", [8, 4096, 1 << 17])
+def test_unpack_box_tables(fe, gpu, leaf):
+    """fe_unpack of explicit box tables: many tiny leaves (8), leaves handled
+    by one workgroup each (4096) and leaves above 65536 samples spread over
+    the whole grid (2^17), with leaves broadcasting every combination of axes
+    (constant slabs along x, y and z) and odd extents."""
+    import torch
+    rng = np.random.default_rng(77 + leaf)
+    shape = (40, 33, 301)
+    a = rng.normal(size=shape).astype(np.float32)
+    a[:, :, 100:220] = a[:, :, 100:101]  # constant along z
+    a[10:35, 5:30, :] = a[10:35, 5:6, :]  # constant along y
+    a[20:40, :, 250:] = 1.5                # constant along every axis
+    nodes, payload = box_table(a, leaf)
+    dn = torch.from_numpy(nodes.view(np.uint8).copy()).to(gpu)
+    dp = torch.from_numpy(payload.view(np.uint32).copy()).to(gpu)
+    out = torch.empty(shape, dtype=torch.float32, device=gpu)
+    assert fe._lib.lib().fe_unpack(dn.data_ptr(), len(nodes), dp.data_ptr(), *shape, out.data_ptr(), None) == 0
+    torch.cuda.synchronize()
+    assert out.cpu().numpy().tobytes() == a.tobytes(), leaf
