@@ -542,8 +542,44 @@ __global__ __launch_bounds__(NTH, 5) void k_csg_tiles(const FieldRef* __restrict
 // records inside the table (the tile reader walks the tree), set `malformed`;
 // every leaf marks the grid samples it covers, 32 per word: a bit of `once`
 // on its first cover, of `twice` on any later one.
+// Marks the samples of a leaf's box clipped to hx x hy x hz: each row (x, y)
+// is a run of hz consecutive samples, one atomic per 32-sample word the run
+// touches; lanes run over a row's words in groups of the next power of two
+// >= the words per row (at most 256), the groups of workgroup slot g (of
+// gstep) over the rows, with 32-bit row arithmetic (extents <= 4096).
+__device__ __forceinline__ void mark_rows(const Node52& nd, int hx, int hy, int hz, int ny, int nz,
+                                          uint32_t* __restrict__ once, uint32_t* __restrict__ twice, uint32_t g,
+                                          uint32_t gstep)
+{
+    const int wpr = ((hz + 31) >> 5) + 1;  // >= 2
+    const int lg = wpr >= 256 ? 8 : 32 - __clz(wpr - 1);
+    const uint32_t kl = threadIdx.x & ((1u << lg) - 1u), G = 256u >> lg;
+    const uint32_t rows = (uint32_t)hx * (uint32_t)hy;
+    for (uint32_t r = g * G + (threadIdx.x >> lg); r < rows; r += gstep * G) {
+        const uint32_t x = r / (uint32_t)hy, y = r - x * (uint32_t)hy;
+        const int64_t s0 = ((int64_t)(nd.lo[0] + (int)x) * ny + nd.lo[1] + (int)y) * nz + nd.lo[2];
+        const int64_t s1 = s0 + hz;  // exclusive
+        for (uint32_t k = kl; k < (uint32_t)wpr; k += 1u << lg) {
+            const int64_t w = (s0 >> 5) + k;
+            if (w > ((s1 - 1) >> 5)) break;
+            const int lo = (int)(max(s0, w << 5) - (w << 5)), hi = (int)(min(s1, (w << 5) + 32) - (w << 5));
+            const uint32_t mask = (hi == 32 ? 0xffffffffu : ((1u << hi) - 1u)) & ~((1u << lo) - 1u);
+            const uint32_t dup = atomicOr(&once[w], mask) & mask;
+            if (dup) atomicOr(&twice[w], dup);
+        }
+    }
+}
+
+#define CHECK_BIG 65536  // word items (rows x words per row): larger leaves are marked by k_field_mark_big
+
+// Bytes of the check buffers for n grid samples: covered-once and covered-
+// twice bit arrays, then a count and up to check_big_cap(n) large-leaf ids.
+__host__ __device__ inline int64_t check_big_cap(int64_t n) { return n / 32768 + 2; }
+inline size_t check_bits_bytes(int64_t n) { return (size_t)((n + 31) >> 5) * 8 + (size_t)(check_big_cap(n) + 1) * 4; }
+
 __global__ __launch_bounds__(256) void k_field_check(FieldRef F, int nx, int ny, int nz, uint32_t* __restrict__ once,
-                                                     uint32_t* __restrict__ twice, int* __restrict__ malformed)
+                                                     uint32_t* __restrict__ twice, int* __restrict__ malformed,
+                                                     uint32_t* __restrict__ big)
 {
     const int64_t i = blockIdx.x;
     if (i >= F.count) return;
@@ -570,22 +606,31 @@ __global__ __launch_bounds__(256) void k_field_check(FieldRef F, int nx, int ny,
     }
     const int hx = min(nd.shape[0], nx - nd.lo[0]), hy = min(nd.shape[1], ny - nd.lo[1]), hz = min(nd.shape[2], nz - nd.lo[2]);
     if (hx <= 0 || hy <= 0 || hz <= 0) return;
-    // rows (x, y) of the clipped box, each a run of hz consecutive samples:
-    // one atomic per 32-sample word the run touches
-    const int wpr = ((hz + 31) >> 5) + 1;
-    const int64_t items = (int64_t)hx * hy * wpr;
-    for (int64_t j = threadIdx.x; j < items; j += 256) {
-        const int k = (int)(j % wpr);
-        const int64_t row = j / wpr;
-        const int y = (int)(row % hy), x = (int)(row / hy);
-        const int64_t s0 = ((int64_t)(nd.lo[0] + x) * ny + nd.lo[1] + y) * nz + nd.lo[2];
-        const int64_t s1 = s0 + hz;  // exclusive
-        const int64_t w = (s0 >> 5) + k;
-        if (w > ((s1 - 1) >> 5)) continue;
-        const int lo = (int)(max(s0, w << 5) - (w << 5)), hi = (int)(min(s1, (w << 5) + 32) - (w << 5));
-        const uint32_t mask = (hi == 32 ? 0xffffffffu : ((1u << hi) - 1u)) & ~((1u << lo) - 1u);
-        const uint32_t dup = atomicOr(&once[w], mask) & mask;
-        if (dup) atomicOr(&twice[w], dup);
+    // a large leaf goes to the list (while it has room: overlapping leaves
+    // of a bad table may exceed the bound of disjoint ones)
+    if ((int64_t)hx * hy * (((hz + 31) >> 5) + 1) > CHECK_BIG) {
+        __shared__ int listed;
+        if (threadIdx.x == 0) {
+            const uint32_t c = atomicAdd(big, 1u);
+            listed = c < (uint32_t)check_big_cap((int64_t)nx * ny * nz);
+            if (listed) big[1 + c] = (uint32_t)i;
+        }
+        __syncthreads();
+        if (listed) return;
+    }
+    mark_rows(nd, hx, hy, hz, ny, nz, once, twice, 0, 1);
+}
+
+// The listed large leaves, one after another, each spread over every block.
+__global__ __launch_bounds__(256) void k_field_mark_big(FieldRef F, int nx, int ny, int nz, uint32_t* __restrict__ once,
+                                                        uint32_t* __restrict__ twice, const uint32_t* __restrict__ big)
+{
+    const uint32_t n = min(big[0], (uint32_t)check_big_cap((int64_t)nx * ny * nz));
+    for (uint32_t j = 0; j < n; ++j) {
+        const Node52& nd = F.nodes[big[1 + j]];
+        const int hx = min(nd.shape[0], nx - nd.lo[0]), hy = min(nd.shape[1], ny - nd.lo[1]),
+                  hz = min(nd.shape[2], nz - nd.lo[2]);
+        mark_rows(nd, hx, hy, hz, ny, nz, once, twice, blockIdx.x, gridDim.x);
     }
 }
 
@@ -855,9 +900,10 @@ int run_plan(const Plan& P, const std::vector<FieldRef>& refs, const int32_t* sh
     return first_bad <= nf ? 3 : 4;
 }
 
-// Checks the used operands on `stream` (cover bitmaps in `bits`, 2 * words
-// uint32; results in res[f]); first failing reference tile per operand, 0
-// for a malformed table, INT_MAX when every sample is covered exactly once.
+// Checks the used operands on `stream` (cover bitmaps and the large-leaf list
+// in `bits`, check_bits_bytes; results in res[f]); first failing reference
+// tile per operand, 0 for a malformed table, INT_MAX when every sample is
+// covered exactly once.
 hipError_t check_operands(const std::vector<FieldRef>& refs, const std::vector<char>& used, const int32_t* shape,
                           uint32_t* bits, int* dres, std::vector<int>& res, hipStream_t st)
 {
@@ -870,9 +916,12 @@ hipError_t check_operands(const std::vector<FieldRef>& refs, const std::vector<c
     if ((e = hipMemcpyAsync(dres, init.data(), sizeof(int) * 2 * nf, hipMemcpyHostToDevice, st)) != hipSuccess) return e;
     for (int f = 0; f < nf; ++f) {
         if (!used[f]) continue;
-        if ((e = hipMemsetAsync(bits, 0, (size_t)words * 8, st)) != hipSuccess) return e;
+        // both bit arrays and the large-leaf count
+        if ((e = hipMemsetAsync(bits, 0, (size_t)words * 8 + 4, st)) != hipSuccess) return e;
         hipLaunchKernelGGL(k_field_check, dim3((unsigned)refs[f].count), dim3(256), 0, st, refs[f], nx, ny, nz, bits,
-                           bits + words, dres + 2 * f);
+                           bits + words, dres + 2 * f, bits + 2 * words);
+        hipLaunchKernelGGL(k_field_mark_big, dim3(2048), dim3(256), 0, st, refs[f], nx, ny, nz, bits, bits + words,
+                           bits + 2 * words);
         hipLaunchKernelGGL(k_cover_scan, dim3((unsigned)std::min<int64_t>((words + 255) / 256, 4096)), dim3(256), 0,
                            st, bits, bits + words, nx, ny, nz, dres + 2 * f + 1);
         if ((e = hipGetLastError()) != hipSuccess) return e;
@@ -948,9 +997,8 @@ int csg_device(const void* const* nodes, const int64_t* counts, const void* cons
     }
     const std::vector<FieldRef> refs = field_refs(nodes, counts, payloads, lengths, fields);
     std::lock_guard<std::mutex> hold(cache_lock);
-    const int64_t words = ((int64_t)shape[0] * shape[1] * shape[2] + 31) >> 5;
     char *bits = nullptr, *res = nullptr, *ws = nullptr;
-    FE_TRY(cached(cache_bits, (size_t)words * 8, &bits));
+    FE_TRY(cached(cache_bits, check_bits_bytes((int64_t)shape[0] * shape[1] * shape[2]), &bits));
     FE_TRY(cached(cache_res, sizeof(int) * 2 * fields, &res));
     std::vector<int> bad;
     FE_TRY(check_operands(refs, used, shape, reinterpret_cast<uint32_t*>(bits), reinterpret_cast<int*>(res), bad, st));
@@ -978,12 +1026,11 @@ extern "C" int fe_csg_check(const void* nodes, int64_t count, const void* payloa
         for (int a = 0; a < 3; ++a)
             if (shape[a] < 1 || shape[a] > 4096) return 1;
         hipStream_t st = static_cast<hipStream_t>(stream);
-        const int64_t words = ((int64_t)shape[0] * shape[1] * shape[2] + 31) >> 5;
         const std::vector<FieldRef> refs = {{static_cast<const Node52*>(nodes), static_cast<const uint32_t*>(payload),
                                              count, samples}};
         std::lock_guard<std::mutex> hold(cache_lock);
         char *bits = nullptr, *res = nullptr;
-        FE_TRY(cached(cache_bits, (size_t)words * 8, &bits));
+        FE_TRY(cached(cache_bits, check_bits_bytes((int64_t)shape[0] * shape[1] * shape[2]), &bits));
         FE_TRY(cached(cache_res, sizeof(int) * 2, &res));
         std::vector<int> bad;
         FE_TRY(check_operands(refs, std::vector<char>(1, 1), shape, reinterpret_cast<uint32_t*>(bits),
