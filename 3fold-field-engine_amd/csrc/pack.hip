@@ -651,7 +651,8 @@ __device__ __forceinline__ void unpack_rows(const Node52& nd, const uint32_t* __
     }
 }
 
-#define UNPACK_BIG 65536  // voxels: larger leaves are spread over the whole grid
+#define UNPACK_BIG 8192       // voxels: larger leaves go to k_unpack_big
+#define UNPACK_HUGE (1 << 20)  // voxels: larger leaves each take the whole grid there
 
 // Dense reconstruction of a packed field: one workgroup per leaf writes its
 // box; leaves of more than UNPACK_BIG voxels are listed in big[1 + i]
@@ -668,24 +669,35 @@ __global__ __launch_bounds__(256) void k_unpack(const Node52* __restrict__ nodes
     if (hx <= 0 || hy <= 0 || hz <= 0) return;
     const int64_t cnt = (int64_t)hx * hy * hz;
     if (cnt > UNPACK_BIG) {
-        if (threadIdx.x == 0) big[1 + atomicAdd(big, 1u)] = (unsigned int)id;
+        // two lists of count + 1 words each: [0] = length, then leaf ids
+        unsigned int* list = cnt > UNPACK_HUGE ? big + count + 1 : big;
+        if (threadIdx.x == 0) list[1 + atomicAdd(list, 1u)] = (unsigned int)id;
         return;
     }
     unpack_rows(nd, payload, hx, hy, hz, ny, nz, 0, 1, dense);
 }
 
-// The listed large leaves, one after another, each spread over every block.
-__global__ __launch_bounds__(256) void k_unpack_big(const Node52* __restrict__ nodes, const uint32_t* __restrict__ payload,
-                                                    int nx, int ny, int nz, uint32_t* __restrict__ dense,
-                                                    const unsigned int* __restrict__ big)
+// The listed leaves: the huge ones (second list) one after another, each
+// over every workgroup; then the large ones with K = grid / n workgroups per
+// leaf (at least one), side by side.
+__global__ __launch_bounds__(256) void k_unpack_big(const Node52* __restrict__ nodes, int64_t count,
+                                                    const uint32_t* __restrict__ payload, int nx, int ny, int nz,
+                                                    uint32_t* __restrict__ dense, const unsigned int* __restrict__ big)
 {
-    const unsigned int n = big[0];
-    for (unsigned int i = 0; i < n; ++i) {
-        const Node52& nd = nodes[big[1 + i]];
+    auto rows = [&](unsigned int id, uint32_t g, uint32_t gstep) {
+        const Node52& nd = nodes[id];
         const int hx = min(nd.shape[0], nx - nd.lo[0]), hy = min(nd.shape[1], ny - nd.lo[1]),
                   hz = min(nd.shape[2], nz - nd.lo[2]);
-        unpack_rows(nd, payload, hx, hy, hz, ny, nz, blockIdx.x, gridDim.x, dense);
-    }
+        unpack_rows(nd, payload, hx, hy, hz, ny, nz, g, gstep, dense);
+    };
+    const unsigned int* huge = big + count + 1;
+    for (unsigned int i = 0; i < huge[0]; ++i) rows(huge[1 + i], blockIdx.x, gridDim.x);
+    const unsigned int n = big[0];
+    if (n == 0) return;
+    const uint32_t K = max(1u, gridDim.x / n);
+    const uint32_t lanes = gridDim.x / K;  // leaves in flight at once
+    if (blockIdx.x / K >= lanes) return;
+    for (unsigned int i = blockIdx.x / K; i < n; i += lanes) rows(big[1 + i], blockIdx.x % K, K);
 }
 
 struct HostPacked {
@@ -758,17 +770,18 @@ extern "C" int fe_unpack(const void* nodes, int64_t count, const uint32_t* paylo
     if (!nodes || !payload || !dense || count < 1 || count > INT32_MAX || nx < 1 || ny < 1 || nz < 1)
         return FE_INVALID;
     hipStream_t st = static_cast<hipStream_t>(stream);
-    // the big-leaf list (at most one entry per leaf) lives in the pack arena,
+    // the two large-leaf lists (at most one entry per leaf each) live in the pack arena,
     // held for the whole call
     std::lock_guard<std::mutex> hold(arena_lock);
     char* base = nullptr;
-    FE_TRY(pack_arena((size_t)(count + 1) * 4, &base));
+    FE_TRY(pack_arena((size_t)(count + 1) * 8, &base));
     unsigned int* big = reinterpret_cast<unsigned int*>(base);
     FE_TRY(hipMemsetAsync(big, 0, 4, st));
+    FE_TRY(hipMemsetAsync(big + count + 1, 0, 4, st));
     hipLaunchKernelGGL(k_unpack, dim3((unsigned)count), dim3(256), 0, st, static_cast<const Node52*>(nodes), count,
                        payload, nx, ny, nz, reinterpret_cast<uint32_t*>(dense), big);
-    hipLaunchKernelGGL(k_unpack_big, dim3(2048), dim3(256), 0, st, static_cast<const Node52*>(nodes), payload, nx, ny,
-                       nz, reinterpret_cast<uint32_t*>(dense), big);
+    hipLaunchKernelGGL(k_unpack_big, dim3(2048), dim3(256), 0, st, static_cast<const Node52*>(nodes), count, payload, nx,
+                       ny, nz, reinterpret_cast<uint32_t*>(dense), big);
     FE_TRY(hipGetLastError());
     // the arena may be reused by the next call once this stream is done
     FE_TRY(hipStreamSynchronize(st));
