@@ -748,6 +748,14 @@ __global__ __launch_bounds__(W* T) void k_edt_lines(const int32_t* __restrict__ 
     if (active) store_chunk<CMAX>(v, dst, lbase, p0, p1, axis_stride);
 }
 
+// Tiles per XCD group in the y pass of k_edt_cand (the x pass keeps 32: with
+// positions 1 MB apart it prefers each XCD to stream nearby rows).  At 512^3
+// groups of 64 read the z summaries faster (y 0.39 -> 0.36 ms); 16 slowed x
+// to 0.79 ms.
+#ifndef CAND_YG
+#define CAND_YG 64
+#endif
+
 // The lower-envelope pass without a merge of the chunk hulls (every axis <=
 // 1024, so entries are (F << 10) | s).  Tiles, chunks and hulls are those of
 // k_edt_lines.  With h(p, s) = f(s) + (p - s)^2:
@@ -789,7 +797,9 @@ __global__ __launch_bounds__(W* T, CMAX <= 16 ? 8 : (CMAX <= 24 ? 6 : 4)) void k
     auto high_bit = [](Mask m) {
         return MB == 64 ? 63 - __builtin_clzll((unsigned long long)m) : 31 - __builtin_clz((uint32_t)m);
     };
-    const uint32_t tile = xcd_group_tile<32>(blockIdx.x);  // grid: a multiple of 256
+    // groups of CAND_YG tiles per XCD in the y pass, 32 in the x pass (grid: a
+    // multiple of 8 times that)
+    const uint32_t tile = axis == 1 ? xcd_group_tile<CAND_YG>(blockIdx.x) : xcd_group_tile<32>(blockIdx.x);
     if (tile >= tiles) return;  // uniform per block
     const int pass = axis == 1 ? 1 : 2;
     const EdtVol V = load_vol(table, nvol, one, pass, tile);
@@ -1137,7 +1147,7 @@ template <int W, int T, int CMAX, bool ZSUM>
 hipError_t launch_cand_t(const int32_t* src, int32_t* dst, const EdtVol* table, int nvol, const EdtVol& one, int axis,
                          int Lmax, uint64_t tiles, hipStream_t st)
 {
-    const uint32_t grid = round_up(tiles, 256);  // whole groups of 32 tiles on each XCD
+    const uint32_t grid = round_up(tiles, axis == 1 ? 8 * CAND_YG : 256);  // whole groups on each XCD
     const int kmax = (Lmax + T - 1) / T;  // <= CMAX by the choice of shape
     const int region = kmax * W * T;      // hull stacks
     // hulls; minima at both boundaries (8 bytes per thread), line extremes
