@@ -119,14 +119,21 @@ __device__ __forceinline__ void eval_points(const Prim* __restrict__ prims, cons
         const int cnt = min(64, n - jb);
         const int mine = list ? (lane < cnt ? (int)list[jb + lane] : 0) : jb + lane;
         Prim pn = load_prim(cp, __builtin_amdgcn_readlane(mine, 0));
-        for (int t = 0; t < cnt; ++t) {
+        int t = 0;
+        if (jb == 0) {  // the first primitive starts the fold
             const Prim p = pn;
-            if (t + 1 < cnt) pn = load_prim(cp, __builtin_amdgcn_readlane(mine, t + 1));
+            pn = load_prim(cp, __builtin_amdgcn_readlane(mine, min(1, cnt - 1)));
 #pragma unroll
-            for (int k = 0; k < K; ++k) {
-                const float v = prim_eval(p.kind, p.cx, p.cy, p.cz, p.a, p.b, p.c, x[k], y[k], z[k]);
-                f[k] = jb + t == 0 ? v : fold(p.op, f[k], v);
-            }
+            for (int k = 0; k < K; ++k) f[k] = prim_eval(p.kind, p.cx, p.cy, p.cz, p.a, p.b, p.c, x[k], y[k], z[k]);
+            t = 1;
+        }
+        // the next record is read ahead without a branch (the last one twice)
+        for (; t < cnt; ++t) {
+            const Prim p = pn;
+            pn = load_prim(cp, __builtin_amdgcn_readlane(mine, min(t + 1, cnt - 1)));
+#pragma unroll
+            for (int k = 0; k < K; ++k)
+                f[k] = fold(p.op, f[k], prim_eval(p.kind, p.cx, p.cy, p.cz, p.a, p.b, p.c, x[k], y[k], z[k]));
         }
     }
 }
