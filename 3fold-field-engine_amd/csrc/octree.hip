@@ -16,6 +16,7 @@
 #include "fe_common.hpp"
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 namespace {
@@ -119,8 +120,18 @@ __global__ __launch_bounds__(256) void k_oct_level(const int32_t* __restrict__ x
     }
     codes[t] = (uint8_t)code;
     flag[t] = code == (level == 3 ? 3 : 2);
-    atomicAdd(&stats[level * 4 + code], 1ull);
-    atomicMin(&first[level * 4 + code], (unsigned long long)t);
+    // one count and one first index per code and wave (every thread's own
+    // atomics on these few words serialised the pass): the lowest lane with
+    // a code holds the wave's first node with it (t grows with the lane)
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const uint64_t m = __ballot(code == c);
+        if (m && lane == __builtin_ctzll(m)) {
+            atomicAdd(&stats[level * 4 + c], (unsigned long long)__popcll(m));
+            atomicMin(&first[level * 4 + c], (unsigned long long)t);
+        }
+    }
 }
 
 // Children of the branches of a level: branch b (exclusive rank pos[t]) writes
@@ -291,6 +302,34 @@ hipError_t dalloc(T** p, int64_t count)
     return hipMalloc(reinterpret_cast<void**>(p), sizeof(T) * (size_t)(count > 0 ? count : 1));
 }
 
+// Temporary device buffers of fe_octree_build, kept between builds (grown
+// as needed, one set per device; builds are serialised by oct_lock), so a
+// build allocates only the arrays its tree keeps.
+enum { S_CODES0, S_CODES1, S_STATS, S_FLAG, S_POS, S_SUMS, S_XYZ0, S_XYZ1, S_C2, S_C3, S_COUNT };
+std::mutex oct_lock;
+void* oct_buf[64][S_COUNT];
+size_t oct_len[64][S_COUNT];
+
+template <class T>
+hipError_t scratch(int slot, T** p, int64_t count)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= 64) return hipErrorInvalidDevice;
+    const size_t need = sizeof(T) * (size_t)(count > 0 ? count : 1);
+    if (oct_len[dev][slot] < need) {
+        if (oct_buf[dev][slot] && (e = hipFree(oct_buf[dev][slot])) != hipSuccess) return e;
+        oct_buf[dev][slot] = nullptr;
+        oct_len[dev][slot] = 0;
+        const size_t grown = need + need / 4;
+        if ((e = hipMalloc(&oct_buf[dev][slot], grown)) != hipSuccess) return e;
+        oct_len[dev][slot] = grown;
+    }
+    *p = static_cast<T*>(oct_buf[dev][slot]);
+    return hipSuccess;
+}
+
 }  // namespace
 
 // Builds the tree of a device dense field (nx,ny,nz) from its dense-source
@@ -313,6 +352,8 @@ extern "C" void* fe_octree_build(const float* dense, int nx, int ny, int nz, con
     int32_t *flag = nullptr, *pos = nullptr, *sums = nullptr, *xyz = nullptr, *next = nullptr;
     unsigned long long* stats = nullptr;
     hipError_t e = hipSuccess;
+    std::lock_guard<std::mutex> hold(oct_lock);
+    int cs = S_CODES0, xs = S_XYZ0;  // the slots holding codes and xyz now
     try {
         t = new Octree();
         t->shape[0] = nx; t->shape[1] = ny; t->shape[2] = nz;
@@ -322,8 +363,8 @@ extern "C" void* fe_octree_build(const float* dense, int nx, int ny, int nz, con
         // level sizes are known one level ahead; node codes of all levels go
         // into one array grown as needed
         int64_t cap_codes = roots * 2 + 1024, off = 0;
-        if ((e = dalloc(&codes, cap_codes)) != hipSuccess) throw e;
-        if ((e = dalloc(&stats, 32)) != hipSuccess) throw e;
+        if ((e = scratch(cs, &codes, cap_codes)) != hipSuccess) throw e;
+        if ((e = scratch(S_STATS, &stats, 32)) != hipSuccess) throw e;
         std::vector<unsigned long long> init(32, 0ull);
         for (int i = 16; i < 32; ++i) init[i] = ~0ull;
         if ((e = hipMemcpyAsync(stats, init.data(), 32 * sizeof(unsigned long long), hipMemcpyHostToDevice, st)) !=
@@ -337,16 +378,16 @@ extern "C" void* fe_octree_build(const float* dense, int nx, int ny, int nz, con
             if (off + n > cap_codes) {  // grow the code array
                 uint8_t* bigger = nullptr;
                 const int64_t cap2 = (off + n) * 2;
-                if ((e = dalloc(&bigger, cap2)) != hipSuccess) throw e;
+                const int other = cs == S_CODES0 ? S_CODES1 : S_CODES0;
+                if ((e = scratch(other, &bigger, cap2)) != hipSuccess) throw e;
                 if ((e = hipMemcpyAsync(bigger, codes, (size_t)off, hipMemcpyDeviceToDevice, st)) != hipSuccess) throw e;
-                if ((e = hipStreamSynchronize(st)) != hipSuccess) throw e;
-                (void)hipFree(codes);
                 codes = bigger;
+                cs = other;
                 cap_codes = cap2;
             }
-            if ((e = dalloc(&flag, n)) != hipSuccess) throw e;
-            if ((e = dalloc(&pos, n)) != hipSuccess) throw e;
-            if ((e = dalloc(&sums, (n + SCAN_ITEMS - 1) / SCAN_ITEMS + 1)) != hipSuccess) throw e;
+            if ((e = scratch(S_FLAG, &flag, n)) != hipSuccess) throw e;
+            if ((e = scratch(S_POS, &pos, n)) != hipSuccess) throw e;
+            if ((e = scratch(S_SUMS, &sums, (n + SCAN_ITEMS - 1) / SCAN_ITEMS + 1)) != hipSuccess) throw e;
             const unsigned grid = (unsigned)((n + 255) / 256);
             hipLaunchKernelGGL(k_oct_level, dim3(grid), dim3(256), 0, st, xyz, n, level, side, rx, ry, rz, nx, ny, nz,
                                kinds[level], nby, nbz, codes + off, flag, stats, stats + 16);
@@ -354,7 +395,7 @@ extern "C" void* fe_octree_build(const float* dense, int nx, int ny, int nz, con
             int32_t total = 0;
             if ((e = scan(flag, n, pos, sums, &total, st)) != hipSuccess) throw e;
             if (level < 3) {
-                if ((e = dalloc(&next, (int64_t)total * 24)) != hipSuccess) throw e;
+                if ((e = scratch(xs == S_XYZ0 ? S_XYZ1 : S_XYZ0, &next, (int64_t)total * 24)) != hipSuccess) throw e;
                 hipLaunchKernelGGL(k_oct_children, dim3(grid), dim3(256), 0, st, xyz, n, level, rx, ry, rz, side / 2,
                                    codes + off, pos, next);
             } else {
@@ -365,10 +406,11 @@ extern "C" void* fe_octree_build(const float* dense, int nx, int ny, int nz, con
             }
             if ((e = hipGetLastError()) != hipSuccess) throw e;
             if ((e = hipStreamSynchronize(st)) != hipSuccess) throw e;
-            (void)hipFree(flag); flag = nullptr;
-            (void)hipFree(pos); pos = nullptr;
-            (void)hipFree(sums); sums = nullptr;
-            (void)hipFree(xyz); xyz = next; next = nullptr;
+            if (next) {
+                xyz = next;
+                xs = xs == S_XYZ0 ? S_XYZ1 : S_XYZ0;
+                next = nullptr;
+            }
             off += n;
             n = level < 3 ? (int64_t)total * 8 : 0;
         }
@@ -382,19 +424,14 @@ extern "C" void* fe_octree_build(const float* dense, int nx, int ny, int nz, con
         if ((e = dalloc(&t->bpre, t->groups + 1)) != hipSuccess) throw e;
         if ((e = dalloc(&t->vpre, t->groups + 1)) != hipSuccess) throw e;
         int32_t *c2 = nullptr, *c3 = nullptr;
-        if ((e = dalloc(&c2, t->groups)) != hipSuccess) throw e;
-        if ((e = dalloc(&c3, t->groups)) != hipSuccess) {
-            (void)hipFree(c2);
-            throw e;
-        }
+        if ((e = scratch(S_C2, &c2, t->groups)) != hipSuccess) throw e;
+        if ((e = scratch(S_C3, &c3, t->groups)) != hipSuccess) throw e;
         hipLaunchKernelGGL(k_oct_group_counts, dim3((unsigned)((t->groups + 3) / 4)), dim3(256), 0, st,
                            reinterpret_cast<const uint32_t*>(t->packed), t->groups, c2, c3);
         int32_t tot = 0;
-        if ((e = dalloc(&sums, (t->groups + SCAN_ITEMS - 1) / SCAN_ITEMS + 1)) == hipSuccess &&
+        if ((e = scratch(S_SUMS, &sums, (t->groups + SCAN_ITEMS - 1) / SCAN_ITEMS + 1)) == hipSuccess &&
             (e = scan(c2, t->groups, t->bpre, sums, &tot, st)) == hipSuccess)
             e = scan(c3, t->groups, t->vpre, sums, &tot, st);
-        (void)hipFree(c2);
-        (void)hipFree(c3);
         if (e != hipSuccess) throw e;
         unsigned long long host[32];
         if ((e = hipMemcpyAsync(host, stats, sizeof host, hipMemcpyDeviceToHost, st)) != hipSuccess) throw e;
@@ -403,23 +440,14 @@ extern "C" void* fe_octree_build(const float* dense, int nx, int ny, int nz, con
             t->stats[i] = (int64_t)host[i];
             t->first[i] = host[16 + i] == ~0ull ? -1 : (int64_t)host[16 + i];
         }
-        (void)hipFree(sums);
-        (void)hipFree(codes);
-        (void)hipFree(stats);
-        (void)hipFree(xyz);
         st_out = FE_OK;
         return t;
     } catch (hipError_t err) {
         fe_note_error(err, "fe_octree_build");
     } catch (...) {
     }
-    (void)hipFree(codes);
-    (void)hipFree(flag);
-    (void)hipFree(pos);
-    (void)hipFree(sums);
-    (void)hipFree(xyz);
-    (void)hipFree(next);
-    (void)hipFree(stats);
+    // the temporaries stay cached; let queued work on them finish first
+    (void)hipStreamSynchronize(st);
     oct_free(t);
     return nullptr;
 }
