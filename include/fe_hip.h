@@ -173,7 +173,9 @@ int fe_dense_tiles(const float *dense, int nx, int ny, int nz, int B, const int3
 /* Point SDF queries on a resident field: the recipe's fp32 fold at device
  * points (n,3), with the culled list of the containing superblock.  Replaces
  * the per-batch eval_batch launch of ikarus_v1/eval_warp.py:478-489 for a
- * built field. */
+ * built field.  From 2^18 points the points are first ordered by
+ * superblock in stream-ordered scratch (hipMallocAsync / hipFreeAsync on
+ * `stream`, 4 * (8194 + 3n) bytes); fewer points allocate nothing. */
 int fe_query_points(const void *desc, const float *points, int64_t n, float *out, void *stream);
 /* No counterpart in the reference (its builds are one per call): `nseg`
  * device-to-device copies in one launch, for a server moving every built
